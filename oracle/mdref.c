@@ -1,0 +1,1724 @@
+/* mdref.c - CPU FP64 oracle for the short-range MD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP code in
+ * lammps_amd/ is compared with; the product never links, loads or calls it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * It restates, function by function, the serial (1 process, STUBS MPI)
+ * behaviour of LAMMPS "8 Aug 2014" on this path.  Citations are relative to
+ * the reference source tree:
+ *   RNG              src/random_park.cpp:42-130
+ *   units            src/update.cpp:118-180
+ *   lattice/atoms    src/lattice.cpp:80-300,362-485; src/create_atoms.cpp:269-326,659-770
+ *   velocity         src/velocity.cpp:136-340,666-716; src/group.cpp:782-805,1090-1128
+ *   temperature      src/compute_temp.cpp:59-95; src/compute.cpp:81
+ *   pbc              src/domain.cpp:478-560
+ *   ghosts           src/comm_brick.cpp:151-398 (setup), 444-506 (forward),
+ *                    513-560 (reverse), 696-864 (borders), 871-943 (pair comm);
+ *                    src/atom_vec_atomic.cpp:105-139,268-313
+ *   sort             src/atom.cpp:1575-1746
+ *   neighbour        src/neighbor.cpp:225-279 (cutoffs), 1294-1461 (decide,
+ *                    check_distance, build), 1538-1725 (bins, stencil extents),
+ *                    1868-1983 (bin_atoms, coord2bin);
+ *                    src/neigh_stencil.cpp:84-98,143-158,434-448;
+ *                    src/neigh_half_bin.cpp:32-133,285-419; src/neigh_full.cpp:241-340
+ *   pair base        src/pair.cpp:173-234 (init), 639-664 (mixing),
+ *                    679-742 (ev_setup), 769-855 (ev_tally), 1402-1450 (fdotr)
+ *   lj/cut           src/pair_lj_cut.cpp:68-141,428-476,545-602
+ *   eam              src/MANYBODY/pair_eam.cpp:125-306,346-427,433-481,488-751
+ *   fix nve          src/fix_nve.cpp:52-142
+ *   verlet           src/verlet.cpp:88-142,207-309,325-371; src/integrate.cpp:114-149
+ *   thermo           src/thermo.cpp:1595-1620; src/compute_pressure.cpp:177-211;
+ *                    src/compute_pe.cpp:71-95
+ * It must be compiled with -ffp-contract=off so that a*b+c rounds twice, as
+ * the reference's x86-64 build does.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#include "mdref.h"
+
+#define MAXT 8
+#define MAXSWAP 64
+#define BIG 1.0e20
+#define SMALL 1.0e-6
+#define CUT2BIN_RATIO 100
+#define MIN(a, b) ((a) < (b) ? (a) : (b))
+#define MAX(a, b) ((a) > (b) ? (a) : (b))
+#define NEIGHMASK 0x3FFFFFFF
+
+struct mdref {
+  char err[256];
+  /* units */
+  int units;
+  double boltz, mvv2e, ftm2v, nktv2p, dt;
+  /* box */
+  double boxlo[3], boxhi[3], prd[3];
+  int periodic[3];
+  int box_exists;
+  /* atoms: x,v,f are nmax*3; ghosts follow the nlocal owned atoms */
+  int natoms, nlocal, nghost, nmax;
+  double *x, *v, *f;
+  int *type, *tag, *image; /* image: nmax*3 */
+  int ntypes;
+  double mass[MAXT + 1];
+  int mass_set[MAXT + 1];
+  /* comm */
+  double cutghost;
+  int maxneed[3], nswap;
+  int *sendlist[MAXSWAP], maxsendlist[MAXSWAP];
+  int sendnum[MAXSWAP], recvnum[MAXSWAP], firstrecv[MAXSWAP];
+  double slablo[MAXSWAP], slabhi[MAXSWAP];
+  int pbc_flag[MAXSWAP], pbc[MAXSWAP][3], swapdim[MAXSWAP];
+  /* neighbour settings */
+  double skin;
+  int every, delay, dist_check, one;
+  int newton_pair, list_style;
+  double cutneighsq[MAXT + 1][MAXT + 1], cutneighmax, cutneighmaxsq, triggersq;
+  int ago, ncalls, ndanger;
+  double *xhold;
+  int maxhold;
+  /* bins */
+  double bboxlo[3], bboxhi[3];
+  int nbinx, nbiny, nbinz, mbinx, mbiny, mbinz, mbinxlo, mbinylo, mbinzlo, mbins;
+  double binsizex, binsizey, binsizez, bininvx, bininvy, bininvz;
+  int *binhead, *bins, maxbinhead, maxbins;
+  int *stencil, nstencil;
+  /* list (CSR) */
+  int inum, *ilist, *numneigh, *firstneigh, maxlist;
+  int *neighbors;
+  long nneigh_alloc;
+  /* sort */
+  int sortfreq;
+  long nextsort;
+  int snbinx, snbiny, snbinz, snbins;
+  double sbboxlo[3], sbboxhi[3], sbininvx, sbininvy, sbininvz;
+  /* pair */
+  int pair_style;
+  int offset_flag;
+  double cut_global;
+  double epsilon[MAXT + 1][MAXT + 1], sigma[MAXT + 1][MAXT + 1], cut[MAXT + 1][MAXT + 1];
+  int setflag[MAXT + 1][MAXT + 1];
+  double lj1[MAXT + 1][MAXT + 1], lj2[MAXT + 1][MAXT + 1], lj3[MAXT + 1][MAXT + 1],
+      lj4[MAXT + 1][MAXT + 1], offset[MAXT + 1][MAXT + 1], cutsq[MAXT + 1][MAXT + 1];
+  double cutforce;
+  /* eam funcfl (one file) */
+  int ff_nrho, ff_nr;
+  double ff_drho, ff_dr, ff_cut, ff_mass;
+  double *ff_frho, *ff_zr, *ff_rhor;
+  int eam_map[MAXT + 1];
+  int nrho, nr, nfrho, nrhor, nz2r;
+  double drho, dr, rdr, rdrho, rhomax, cutmax, cutforcesq;
+  double *frho, *rhor, *z2r;                         /* [n][nr+1] */
+  double *frho_spline, *rhor_spline, *z2r_spline;   /* [n][n+1][7] */
+  int type2frho[MAXT + 1], type2rhor[MAXT + 1][MAXT + 1], type2z2r[MAXT + 1][MAXT + 1];
+  double *rho, *fp;
+  /* energy / virial */
+  int evflag, eflag_either, eflag_global, eflag_atom, vflag_either, vflag_global, vflag_atom,
+      vflag_fdotr;
+  double eng_vdwl, virial[6];
+  double *eatom, *vatom;
+  /* run state */
+  long ntimestep;
+  int thermo_every;
+  int setup_done;
+  int eflag, vflag;
+};
+
+/* ------------------------------------------------------------------ */
+/* RanPark (src/random_park.cpp)                                      */
+
+typedef struct {
+  int seed, save;
+  double second;
+} ranpark;
+
+#define IA 16807
+#define IM 2147483647
+#define AM (1.0 / IM)
+#define IQ 127773
+#define IR 2836
+
+/* random_park.cpp:42-49 */
+static double rp_uniform(ranpark *r) {
+  int k = r->seed / IQ;
+  r->seed = IA * (r->seed - k * IQ) - IR * k;
+  if (r->seed < 0) r->seed += IM;
+  return AM * r->seed;
+}
+
+/* random_park.cpp:55-76 (Marsaglia polar method, second value cached) */
+static double rp_gaussian(ranpark *r) {
+  double first, v1, v2, rsq, fac;
+  if (!r->save) {
+    do {
+      v1 = 2.0 * rp_uniform(r) - 1.0;
+      v2 = 2.0 * rp_uniform(r) - 1.0;
+      rsq = v1 * v1 + v2 * v2;
+    } while (!(rsq < 1.0 && rsq != 0.0));
+    fac = sqrt(-2.0 * log(rsq) / rsq);
+    r->second = v1 * fac;
+    first = v2 * fac;
+    r->save = 1;
+  } else {
+    first = r->second;
+    r->save = 0;
+  }
+  return first;
+}
+
+/* random_park.cpp:94-130: Jenkins one-at-a-time hash over the bytes of the
+   int seed then the 24 bytes of the 3 doubles; bytes are (signed) char. */
+static void rp_reset_coord(ranpark *r, int ibase, const double *coord) {
+  const signed char *str = (const signed char *)&ibase;
+  unsigned int hash = 0;
+  for (int i = 0; i < (int)sizeof(int); i++) {
+    hash += (unsigned int)(int)str[i];
+    hash += (hash << 10);
+    hash ^= (hash >> 6);
+  }
+  str = (const signed char *)coord;
+  for (int i = 0; i < 3 * (int)sizeof(double); i++) {
+    hash += (unsigned int)(int)str[i];
+    hash += (hash << 10);
+    hash ^= (hash >> 6);
+  }
+  hash += (hash << 3);
+  hash ^= (hash >> 11);
+  hash += (hash << 15);
+  r->seed = hash & 0x7ffffff;
+  if (!r->seed) r->seed = 1;
+  for (int i = 0; i < 5; i++) rp_uniform(r);
+  r->save = 0;
+}
+
+double mdref_ranpark_uniform_seq(int seed, int n, int *state_out) {
+  ranpark r = {seed, 0, 0.0};
+  double u = 0.0;
+  for (int i = 0; i < n; i++) u = rp_uniform(&r);
+  if (state_out) *state_out = r.seed;
+  return u;
+}
+
+int mdref_ranpark_reset_coord(int ibase, const double coord[3]) {
+  ranpark r = {1, 0, 0.0};
+  rp_reset_coord(&r, ibase, coord);
+  return r.seed;
+}
+
+/* ------------------------------------------------------------------ */
+/* bookkeeping                                                        */
+
+static int fail(mdref *s, const char *msg) {
+  snprintf(s->err, sizeof(s->err), "%s", msg);
+  return -1;
+}
+
+const char *mdref_error(mdref *s) { return s->err; }
+
+mdref *mdref_new(void) {
+  mdref *s = (mdref *)calloc(1, sizeof(mdref));
+  if (!s) return NULL;
+  mdref_units(s, MDREF_UNITS_LJ);
+  s->every = 1;
+  s->delay = 10; /* neighbor.cpp:78-82 defaults */
+  s->dist_check = 1;
+  s->one = 2000;
+  s->newton_pair = 1;
+  s->list_style = MDREF_LIST_HALF_NEWTON;
+  s->sortfreq = 1000; /* atom.cpp:63 */
+  s->thermo_every = 0;
+  for (int i = 0; i <= MAXT; i++) s->eam_map[i] = -1;
+  return s;
+}
+
+void mdref_free(mdref *s) {
+  if (!s) return;
+  free(s->x); free(s->v); free(s->f); free(s->type); free(s->tag); free(s->image);
+  for (int i = 0; i < MAXSWAP; i++) free(s->sendlist[i]);
+  free(s->xhold); free(s->binhead); free(s->bins); free(s->stencil);
+  free(s->ilist); free(s->numneigh); free(s->firstneigh); free(s->neighbors);
+  free(s->ff_frho); free(s->ff_zr); free(s->ff_rhor);
+  free(s->frho); free(s->rhor); free(s->z2r);
+  free(s->frho_spline); free(s->rhor_spline); free(s->z2r_spline);
+  free(s->rho); free(s->fp); free(s->eatom); free(s->vatom);
+  free(s);
+}
+
+static void grow_atoms(mdref *s, int n) {
+  if (n <= s->nmax) return;
+  int nm = s->nmax ? s->nmax : 1024;
+  while (nm < n) nm *= 2;
+  s->x = (double *)realloc(s->x, sizeof(double) * 3 * nm);
+  s->v = (double *)realloc(s->v, sizeof(double) * 3 * nm);
+  s->f = (double *)realloc(s->f, sizeof(double) * 3 * nm);
+  s->type = (int *)realloc(s->type, sizeof(int) * nm);
+  s->tag = (int *)realloc(s->tag, sizeof(int) * nm);
+  s->image = (int *)realloc(s->image, sizeof(int) * 3 * nm);
+  s->rho = (double *)realloc(s->rho, sizeof(double) * nm);
+  s->fp = (double *)realloc(s->fp, sizeof(double) * nm);
+  s->eatom = (double *)realloc(s->eatom, sizeof(double) * nm);
+  s->vatom = (double *)realloc(s->vatom, sizeof(double) * 6 * nm);
+  for (int i = s->nmax; i < nm; i++) {
+    s->v[3 * i] = s->v[3 * i + 1] = s->v[3 * i + 2] = 0.0;
+    s->image[3 * i] = s->image[3 * i + 1] = s->image[3 * i + 2] = 0;
+  }
+  s->nmax = nm;
+}
+
+/* update.cpp:118-180 */
+int mdref_units(mdref *s, int style) {
+  s->units = style;
+  if (style == MDREF_UNITS_LJ) {
+    s->boltz = 1.0; s->mvv2e = 1.0; s->ftm2v = 1.0; s->nktv2p = 1.0;
+    s->dt = 0.005; s->skin = 0.3;
+  } else if (style == MDREF_UNITS_METAL) {
+    s->boltz = 8.617343e-5; s->mvv2e = 1.0364269e-4; s->ftm2v = 1.0 / 1.0364269e-4;
+    s->nktv2p = 1.6021765e6; s->dt = 0.001; s->skin = 2.0;
+  } else return fail(s, "unknown units");
+  return 0;
+}
+
+int mdref_box(mdref *s, const double lo[3], const double hi[3], const int periodic[3], int ntypes) {
+  if (ntypes < 1 || ntypes > MAXT) return fail(s, "ntypes out of range");
+  for (int d = 0; d < 3; d++) {
+    s->boxlo[d] = lo[d];
+    s->boxhi[d] = hi[d];
+    s->prd[d] = hi[d] - lo[d];
+    s->periodic[d] = periodic[d];
+  }
+  s->ntypes = ntypes;
+  s->box_exists = 1;
+  return 0;
+}
+
+/* lattice.cpp:80-300 (basis, lj scale, spacings) + region block + create_box +
+   create_atoms.cpp:269-326,659-770 (loop order k,j,i,m; epsilon-shrunk subbox;
+   IDs in creation order, atom.cpp:618-650). Orientation is the default x,y,z
+   and origin 0, so lattice2box (lattice.cpp:468-485) is (i+b)*scale. */
+int mdref_lattice_box(mdref *s, int style, double arg, const int lo[3], const int hi[3],
+                      const int periodic[3], int ntypes) {
+  double basis[4][3];
+  int nbasis;
+  if (style == 0) {
+    nbasis = 1;
+    basis[0][0] = basis[0][1] = basis[0][2] = 0.0;
+  } else if (style == 1) {
+    nbasis = 2;
+    double b[2][3] = {{0, 0, 0}, {0.5, 0.5, 0.5}};
+    memcpy(basis, b, sizeof(b));
+  } else if (style == 2) {
+    nbasis = 4;
+    double b[4][3] = {{0, 0, 0}, {0.5, 0.5, 0.0}, {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
+    memcpy(basis, b, sizeof(b));
+  } else return fail(s, "lattice style");
+  double scale = arg;
+  if (s->units == MDREF_UNITS_LJ) {
+    double volume = 1.0;
+    scale = pow(nbasis / volume / scale, 1.0 / 3);
+  }
+  double xlat = scale; /* bbox extent of the unit cell in box coords */
+  double blo[3], bhi[3];
+  for (int d = 0; d < 3; d++) {
+    blo[d] = xlat * (double)lo[d];
+    bhi[d] = xlat * (double)hi[d];
+  }
+  if (mdref_box(s, blo, bhi, periodic, ntypes)) return -1;
+  double sublo[3], subhi[3];
+  for (int d = 0; d < 3; d++) {
+    sublo[d] = s->boxlo[d];
+    subhi[d] = s->boxhi[d];
+    if (s->periodic[d]) {
+      double eps = s->prd[d] * 1.0e-6;
+      sublo[d] -= eps;
+      subhi[d] -= 2.0 * eps;
+    }
+  }
+  int n0 = s->nlocal;
+  for (int k = lo[2] - 2; k <= hi[2] + 1; k++)
+    for (int j = lo[1] - 2; j <= hi[1] + 1; j++)
+      for (int i = lo[0] - 2; i <= hi[0] + 1; i++)
+        for (int m = 0; m < nbasis; m++) {
+          double xx[3];
+          xx[0] = (i + basis[m][0]) * scale;
+          xx[1] = (j + basis[m][1]) * scale;
+          xx[2] = (k + basis[m][2]) * scale;
+          if (xx[0] < sublo[0] || xx[0] >= subhi[0] || xx[1] < sublo[1] || xx[1] >= subhi[1] ||
+              xx[2] < sublo[2] || xx[2] >= subhi[2])
+            continue;
+          grow_atoms(s, s->nlocal + 1);
+          int a = s->nlocal++;
+          s->x[3 * a] = xx[0]; s->x[3 * a + 1] = xx[1]; s->x[3 * a + 2] = xx[2];
+          s->v[3 * a] = s->v[3 * a + 1] = s->v[3 * a + 2] = 0.0;
+          s->type[a] = 1;
+          s->tag[a] = a + 1;
+          s->image[3 * a] = s->image[3 * a + 1] = s->image[3 * a + 2] = 0;
+        }
+  s->natoms = s->nlocal;
+  (void)n0;
+  return 0;
+}
+
+int mdref_add_atoms(mdref *s, int n, const double *x, const int *type) {
+  if (!s->box_exists) return fail(s, "box not defined");
+  grow_atoms(s, s->nlocal + n);
+  for (int k = 0; k < n; k++) {
+    int a = s->nlocal++;
+    for (int d = 0; d < 3; d++) {
+      s->x[3 * a + d] = x[3 * k + d];
+      s->v[3 * a + d] = 0.0;
+      s->image[3 * a + d] = 0;
+    }
+    if (type[k] < 1 || type[k] > s->ntypes) return fail(s, "atom type out of range");
+    s->type[a] = type[k];
+    s->tag[a] = a + 1;
+  }
+  s->natoms = s->nlocal;
+  return 0;
+}
+
+int mdref_set_type(mdref *s, int n, const int *type) {
+  if (n != s->nlocal) return fail(s, "set_type size");
+  for (int i = 0; i < n; i++) {
+    if (type[i] < 1 || type[i] > s->ntypes) return fail(s, "atom type out of range");
+    s->type[i] = type[i];
+  }
+  return 0;
+}
+
+int mdref_mass(mdref *s, int type, double m) {
+  if (type < 1 || type > s->ntypes) return fail(s, "mass type");
+  s->mass[type] = m;
+  s->mass_set[type] = 1;
+  return 0;
+}
+
+double mdref_mass_of(mdref *s, int type) { return s->mass[type]; }
+
+int mdref_set_velocities(mdref *s, int n, const double *v) {
+  if (n != s->nlocal) return fail(s, "set_velocities size");
+  memcpy(s->v, v, sizeof(double) * 3 * n);
+  return 0;
+}
+
+/* compute_temp.cpp:59-95 with dof = 3N - 3 (compute.cpp:81, no fixes) */
+static double temp_dof(mdref *s) { return 3.0 * s->natoms - 3.0; }
+
+static double compute_temp(mdref *s) {
+  double t = 0.0;
+  for (int i = 0; i < s->nlocal; i++) {
+    double *vi = &s->v[3 * i];
+    t += (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]) * s->mass[s->type[i]];
+  }
+  double dof = temp_dof(s);
+  double tfactor = dof > 0.0 ? s->mvv2e / (dof * s->boltz) : 0.0;
+  return t * tfactor;
+}
+
+/* velocity.cpp:136-340 (create), 666-716 (rescale, zero_momentum);
+   group.cpp:782-805 (mass), 1090-1128 (vcm) */
+int mdref_velocity_create(mdref *s, double t_desired, int seed, int dist, int loop_geom, int mom) {
+  if (seed <= 0) return fail(s, "Illegal velocity create command");
+  for (int t = 1; t <= s->ntypes; t++)
+    if (!s->mass_set[t]) return fail(s, "mass not set");
+  int n = s->nlocal;
+  if (!loop_geom) {
+    /* loop all: one RNG stream over IDs 1..natoms; 1 process owns them all */
+    ranpark r = {seed, 0, 0.0};
+    int *map = (int *)malloc(sizeof(int) * (s->natoms + 1));
+    for (int i = 0; i <= s->natoms; i++) map[i] = -1;
+    for (int i = 0; i < n; i++) map[s->tag[i]] = i;
+    for (int i = 1; i <= s->natoms; i++) {
+      double vx, vy, vz;
+      if (dist == 0) { vx = rp_uniform(&r); vy = rp_uniform(&r); vz = rp_uniform(&r); }
+      else { vx = rp_gaussian(&r); vy = rp_gaussian(&r); vz = rp_gaussian(&r); }
+      int m = map[i];
+      if (m >= 0 && m < n) {
+        double factor = 1.0 / sqrt(s->mass[s->type[m]]);
+        s->v[3 * m] = vx * factor;
+        s->v[3 * m + 1] = vy * factor;
+        s->v[3 * m + 2] = vz * factor;
+      }
+    }
+    free(map);
+  } else {
+    ranpark r = {1, 0, 0.0};
+    for (int i = 0; i < n; i++) {
+      rp_reset_coord(&r, seed, &s->x[3 * i]);
+      double vx, vy, vz;
+      if (dist == 0) { vx = rp_uniform(&r); vy = rp_uniform(&r); vz = rp_uniform(&r); }
+      else { vx = rp_gaussian(&r); vy = rp_gaussian(&r); vz = rp_gaussian(&r); }
+      double factor = 1.0 / sqrt(s->mass[s->type[i]]);
+      s->v[3 * i] = vx * factor;
+      s->v[3 * i + 1] = vy * factor;
+      s->v[3 * i + 2] = vz * factor;
+    }
+  }
+  if (mom) {
+    double masstotal = 0.0;
+    for (int i = 0; i < n; i++) masstotal += s->mass[s->type[i]];
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; i++) {
+      double massone = s->mass[s->type[i]];
+      p[0] += s->v[3 * i] * massone;
+      p[1] += s->v[3 * i + 1] * massone;
+      p[2] += s->v[3 * i + 2] * massone;
+    }
+    if (masstotal > 0.0) { p[0] /= masstotal; p[1] /= masstotal; p[2] /= masstotal; }
+    for (int i = 0; i < n; i++) {
+      s->v[3 * i] -= p[0];
+      s->v[3 * i + 1] -= p[1];
+      s->v[3 * i + 2] -= p[2];
+    }
+  }
+  double t = compute_temp(s);
+  if (t == 0.0) return fail(s, "Attempting to rescale a 0.0 temperature");
+  double factor = sqrt(t_desired / t);
+  for (int i = 0; i < n; i++) {
+    s->v[3 * i] *= factor;
+    s->v[3 * i + 1] *= factor;
+    s->v[3 * i + 2] *= factor;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* pair settings                                                      */
+
+/* pair_lj_cut.cpp:428-442 (settings), 448-476 (coeff) */
+int mdref_pair_lj(mdref *s, double cut_global, int shift) {
+  s->pair_style = MDREF_PAIR_LJ;
+  s->cut_global = cut_global;
+  s->offset_flag = shift;
+  memset(s->setflag, 0, sizeof(s->setflag));
+  return 0;
+}
+
+int mdref_pair_lj_coeff(mdref *s, int i, int j, double eps, double sigma, double cut) {
+  if (s->pair_style != MDREF_PAIR_LJ) return fail(s, "pair style is not lj/cut");
+  if (i < 1 || j < 1 || i > s->ntypes || j > s->ntypes) return fail(s, "pair_coeff types");
+  if (i > j) { int t = i; i = j; j = t; }
+  s->epsilon[i][j] = eps;
+  s->sigma[i][j] = sigma;
+  s->cut[i][j] = cut < 0.0 ? s->cut_global : cut;
+  s->setflag[i][j] = 1;
+  return 0;
+}
+
+/* pair.cpp:639-664 geometric mixing (the default mix_flag) */
+static double mix_energy(double e1, double e2) { return sqrt(e1 * e2); }
+static double mix_distance(double s1, double s2) { return sqrt(s1 * s2); }
+
+/* pair_lj_cut.cpp:545-602 */
+static double lj_init_one(mdref *s, int i, int j) {
+  if (s->setflag[i][j] == 0) {
+    s->epsilon[i][j] = mix_energy(s->epsilon[i][i], s->epsilon[j][j]);
+    s->sigma[i][j] = mix_distance(s->sigma[i][i], s->sigma[j][j]);
+    s->cut[i][j] = mix_distance(s->cut[i][i], s->cut[j][j]);
+  }
+  s->lj1[i][j] = 48.0 * s->epsilon[i][j] * pow(s->sigma[i][j], 12.0);
+  s->lj2[i][j] = 24.0 * s->epsilon[i][j] * pow(s->sigma[i][j], 6.0);
+  s->lj3[i][j] = 4.0 * s->epsilon[i][j] * pow(s->sigma[i][j], 12.0);
+  s->lj4[i][j] = 4.0 * s->epsilon[i][j] * pow(s->sigma[i][j], 6.0);
+  if (s->offset_flag) {
+    double ratio = s->sigma[i][j] / s->cut[i][j];
+    s->offset[i][j] = 4.0 * s->epsilon[i][j] * (pow(ratio, 12.0) - pow(ratio, 6.0));
+  } else s->offset[i][j] = 0.0;
+  s->lj1[j][i] = s->lj1[i][j];
+  s->lj2[j][i] = s->lj2[i][j];
+  s->lj3[j][i] = s->lj3[i][j];
+  s->lj4[j][i] = s->lj4[i][j];
+  s->offset[j][i] = s->offset[i][j];
+  return s->cut[i][j];
+}
+
+/* pair_eam.cpp:433-481 (read_file) + grab(): funcfl header, then frho[nrho],
+   zr[nr], rhor[nr], stored 1-based */
+static int read_doubles(FILE *fp, int n, double *out) {
+  for (int i = 0; i < n; i++)
+    if (fscanf(fp, "%lg", &out[i]) != 1) return -1;
+  return 0;
+}
+
+int mdref_pair_eam_file(mdref *s, const char *path) {
+  FILE *fp = fopen(path, "r");
+  if (!fp) return fail(s, "Cannot open EAM potential file");
+  char line[1024];
+  int tmp;
+  if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(s, "eam header"); }
+  if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(s, "eam header"); }
+  sscanf(line, "%d %lg", &tmp, &s->ff_mass);
+  if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(s, "eam header"); }
+  if (sscanf(line, "%d %lg %d %lg %lg", &s->ff_nrho, &s->ff_drho, &s->ff_nr, &s->ff_dr, &s->ff_cut) != 5) {
+    fclose(fp);
+    return fail(s, "eam header line 3");
+  }
+  free(s->ff_frho); free(s->ff_zr); free(s->ff_rhor);
+  s->ff_frho = (double *)calloc(s->ff_nrho + 1, sizeof(double));
+  s->ff_zr = (double *)calloc(s->ff_nr + 1, sizeof(double));
+  s->ff_rhor = (double *)calloc(s->ff_nr + 1, sizeof(double));
+  if (read_doubles(fp, s->ff_nrho, &s->ff_frho[1]) || read_doubles(fp, s->ff_nr, &s->ff_zr[1]) ||
+      read_doubles(fp, s->ff_nr, &s->ff_rhor[1])) {
+    fclose(fp);
+    return fail(s, "eam table truncated");
+  }
+  fclose(fp);
+  s->pair_style = MDREF_PAIR_EAM;
+  /* pair_coeff * * file: every type maps to file 0 and takes its mass
+     (pair_eam.cpp:346-391) */
+  for (int t = 1; t <= s->ntypes; t++) {
+    s->eam_map[t] = 0;
+    s->mass[t] = s->ff_mass;
+    s->mass_set[t] = 1;
+  }
+  return 0;
+}
+
+/* pair_eam.cpp:723-751 */
+static void eam_interpolate(int n, double delta, const double *f, double *spline /* (n+1)*7 */) {
+#define SP(m, k) spline[(m)*7 + (k)]
+  for (int m = 1; m <= n; m++) SP(m, 6) = f[m];
+  SP(1, 5) = SP(2, 6) - SP(1, 6);
+  SP(2, 5) = 0.5 * (SP(3, 6) - SP(1, 6));
+  SP(n - 1, 5) = 0.5 * (SP(n, 6) - SP(n - 2, 6));
+  SP(n, 5) = SP(n, 6) - SP(n - 1, 6);
+  for (int m = 3; m <= n - 2; m++)
+    SP(m, 5) = ((SP(m - 2, 6) - SP(m + 2, 6)) + 8.0 * (SP(m + 1, 6) - SP(m - 1, 6))) / 12.0;
+  for (int m = 1; m <= n - 1; m++) {
+    SP(m, 4) = 3.0 * (SP(m + 1, 6) - SP(m, 6)) - 2.0 * SP(m, 5) - SP(m + 1, 5);
+    SP(m, 3) = SP(m, 5) + SP(m + 1, 5) - 2.0 * (SP(m + 1, 6) - SP(m, 6));
+  }
+  SP(n, 4) = 0.0;
+  SP(n, 3) = 0.0;
+  for (int m = 1; m <= n; m++) {
+    SP(m, 2) = SP(m, 5) / delta;
+    SP(m, 1) = 2.0 * SP(m, 4) / delta;
+    SP(m, 0) = 3.0 * SP(m, 3) / delta;
+  }
+#undef SP
+}
+
+/* 4-point Lagrange regrid of one funcfl table (pair_eam.cpp:536-553 pattern) */
+static double eam_regrid(double r, double fdelta, int fn, const double *tab) {
+  const double sixth = 1.0 / 6.0;
+  double p = r / fdelta + 1.0;
+  int k = (int)p;
+  k = MIN(k, fn - 2);
+  k = MAX(k, 2);
+  p -= k;
+  p = MIN(p, 2.0);
+  double cof1 = -sixth * p * (p - 1.0) * (p - 2.0);
+  double cof2 = 0.5 * (p * p - 1.0) * (p - 2.0);
+  double cof3 = -0.5 * p * (p + 1.0) * (p - 2.0);
+  double cof4 = sixth * p * (p * p - 1.0);
+  return cof1 * tab[k - 1] + cof2 * tab[k] + cof3 * tab[k + 1] + cof4 * tab[k + 2];
+}
+
+/* pair_eam.cpp:397-405 (init_style) -> file2array 488-694 -> array2spline 698-719;
+   restricted to one funcfl file */
+static void eam_init_style(mdref *s) {
+  s->dr = s->ff_dr;
+  s->drho = s->ff_drho;
+  double rmax = (s->ff_nr - 1) * s->ff_dr;
+  s->rhomax = (s->ff_nrho - 1) * s->ff_drho;
+  s->nr = (int)(rmax / s->dr + 0.5);
+  s->nrho = (int)(s->rhomax / s->drho + 0.5);
+  s->nfrho = 2;
+  s->nrhor = 1;
+  s->nz2r = 1;
+  free(s->frho); free(s->rhor); free(s->z2r);
+  s->frho = (double *)calloc((size_t)s->nfrho * (s->nrho + 1), sizeof(double));
+  s->rhor = (double *)calloc((size_t)s->nrhor * (s->nr + 1), sizeof(double));
+  s->z2r = (double *)calloc((size_t)s->nz2r * (s->nr + 1), sizeof(double));
+  for (int m = 1; m <= s->nrho; m++) {
+    double r = (m - 1) * s->drho;
+    s->frho[m] = eam_regrid(r, s->ff_drho, s->ff_nrho, s->ff_frho);
+  }
+  for (int m = 1; m <= s->nrho; m++) s->frho[(s->nrho + 1) + m] = 0.0;
+  for (int i = 1; i <= s->ntypes; i++) s->type2frho[i] = s->eam_map[i] >= 0 ? s->eam_map[i] : s->nfrho - 1;
+  for (int m = 1; m <= s->nr; m++) {
+    double r = (m - 1) * s->dr;
+    s->rhor[m] = eam_regrid(r, s->ff_dr, s->ff_nr, s->ff_rhor);
+  }
+  for (int i = 1; i <= s->ntypes; i++)
+    for (int j = 1; j <= s->ntypes; j++) s->type2rhor[i][j] = s->eam_map[i];
+  for (int m = 1; m <= s->nr; m++) {
+    double r = (m - 1) * s->dr;
+    double zri = eam_regrid(r, s->ff_dr, s->ff_nr, s->ff_zr);
+    double zrj = eam_regrid(r, s->ff_dr, s->ff_nr, s->ff_zr);
+    s->z2r[m] = 27.2 * 0.529 * zri * zrj;
+  }
+  for (int i = 1; i <= s->ntypes; i++)
+    for (int j = 1; j <= s->ntypes; j++) s->type2z2r[i][j] = 0;
+  /* array2spline */
+  s->rdr = 1.0 / s->dr;
+  s->rdrho = 1.0 / s->drho;
+  free(s->frho_spline); free(s->rhor_spline); free(s->z2r_spline);
+  s->frho_spline = (double *)calloc((size_t)s->nfrho * (s->nrho + 1) * 7, sizeof(double));
+  s->rhor_spline = (double *)calloc((size_t)s->nrhor * (s->nr + 1) * 7, sizeof(double));
+  s->z2r_spline = (double *)calloc((size_t)s->nz2r * (s->nr + 1) * 7, sizeof(double));
+  for (int i = 0; i < s->nfrho; i++)
+    eam_interpolate(s->nrho, s->drho, &s->frho[(size_t)i * (s->nrho + 1)],
+                    &s->frho_spline[(size_t)i * (s->nrho + 1) * 7]);
+  for (int i = 0; i < s->nrhor; i++)
+    eam_interpolate(s->nr, s->dr, &s->rhor[(size_t)i * (s->nr + 1)],
+                    &s->rhor_spline[(size_t)i * (s->nr + 1) * 7]);
+  for (int i = 0; i < s->nz2r; i++)
+    eam_interpolate(s->nr, s->dr, &s->z2r[(size_t)i * (s->nr + 1)],
+                    &s->z2r_spline[(size_t)i * (s->nr + 1) * 7]);
+}
+
+/* pair_eam.cpp:411-427 */
+static double eam_init_one(mdref *s) {
+  s->cutmax = s->ff_cut;
+  s->cutforcesq = s->cutmax * s->cutmax;
+  return s->cutmax;
+}
+
+int mdref_eam_info(mdref *s, int *nrho, double *drho, int *nr, double *dr, double *rhomax, double *cutmax,
+                   int *nfrho, int *nrhor, int *nz2r) {
+  if (s->pair_style != MDREF_PAIR_EAM || !s->frho_spline) return fail(s, "eam not initialised");
+  *nrho = s->nrho; *drho = s->drho; *nr = s->nr; *dr = s->dr; *rhomax = s->rhomax;
+  *cutmax = s->cutmax; *nfrho = s->nfrho; *nrhor = s->nrhor; *nz2r = s->nz2r;
+  return 0;
+}
+
+int mdref_eam_splines(mdref *s, double *frho_spline, double *rhor_spline, double *z2r_spline,
+                      int *type2frho, int *type2rhor, int *type2z2r) {
+  if (s->pair_style != MDREF_PAIR_EAM || !s->frho_spline) return fail(s, "eam not initialised");
+  memcpy(frho_spline, s->frho_spline, sizeof(double) * s->nfrho * (s->nrho + 1) * 7);
+  memcpy(rhor_spline, s->rhor_spline, sizeof(double) * s->nrhor * (s->nr + 1) * 7);
+  memcpy(z2r_spline, s->z2r_spline, sizeof(double) * s->nz2r * (s->nr + 1) * 7);
+  int nt = s->ntypes + 1;
+  for (int i = 0; i < nt; i++) {
+    type2frho[i] = s->type2frho[i];
+    for (int j = 0; j < nt; j++) {
+      type2rhor[i * nt + j] = s->type2rhor[i][j];
+      type2z2r[i * nt + j] = s->type2z2r[i][j];
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* settings                                                           */
+
+int mdref_neighbor(mdref *s, double skin, int every, int delay, int check, int one) {
+  if (skin < 0.0 || every <= 0 || delay < 0 || one <= 0) return fail(s, "Illegal neigh_modify");
+  s->skin = skin;
+  s->every = every;
+  s->delay = delay;
+  s->dist_check = check;
+  s->one = one;
+  return 0;
+}
+int mdref_newton(mdref *s, int newton) {
+  s->newton_pair = newton;
+  if (s->list_style != MDREF_LIST_FULL)
+    s->list_style = newton ? MDREF_LIST_HALF_NEWTON : MDREF_LIST_HALF_NONEWTON;
+  return 0;
+}
+int mdref_list_style(mdref *s, int style) { s->list_style = style; return 0; }
+int mdref_sortfreq(mdref *s, int freq) { s->sortfreq = freq; return 0; }
+int mdref_timestep(mdref *s, double dt) { s->dt = dt; return 0; }
+int mdref_thermo_every(mdref *s, int n) { s->thermo_every = n; return 0; }
+
+/* ------------------------------------------------------------------ */
+/* init: pair.cpp:173-234 and neighbor.cpp:225-279                     */
+
+static int init_all(mdref *s) {
+  if (s->pair_style == MDREF_PAIR_NONE) return fail(s, "no pair style");
+  for (int t = 1; t <= s->ntypes; t++)
+    if (!s->mass_set[t]) return fail(s, "mass not set");
+  if (s->pair_style == MDREF_PAIR_LJ) {
+    for (int i = 1; i <= s->ntypes; i++)
+      if (!s->setflag[i][i]) return fail(s, "All pair coeffs are not set");
+  } else {
+    eam_init_style(s);
+  }
+  s->cutforce = 0.0;
+  for (int i = 1; i <= s->ntypes; i++)
+    for (int j = i; j <= s->ntypes; j++) {
+      double c = s->pair_style == MDREF_PAIR_LJ ? lj_init_one(s, i, j) : eam_init_one(s);
+      s->cutsq[i][j] = s->cutsq[j][i] = c * c;
+      s->cutforce = MAX(s->cutforce, c);
+    }
+  s->triggersq = 0.25 * s->skin * s->skin;
+  s->cutneighmax = 0.0;
+  for (int i = 1; i <= s->ntypes; i++)
+    for (int j = 1; j <= s->ntypes; j++) {
+      double cutoff = sqrt(s->cutsq[i][j]);
+      double delta = cutoff > 0.0 ? s->skin : 0.0;
+      double c = cutoff + delta;
+      s->cutneighsq[i][j] = c * c;
+      s->cutneighmax = MAX(s->cutneighmax, c);
+    }
+  s->cutneighmaxsq = s->cutneighmax * s->cutneighmax;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* domain + comm                                                      */
+
+/* domain.cpp:478-560 (image flags kept as three plain counters) */
+static void domain_pbc(mdref *s) {
+  for (int i = 0; i < s->nlocal; i++)
+    for (int d = 0; d < 3; d++) {
+      if (!s->periodic[d]) continue;
+      double *xi = &s->x[3 * i + d];
+      if (*xi < s->boxlo[d]) {
+        *xi += s->prd[d];
+        s->image[3 * i + d]--;
+      }
+      if (*xi >= s->boxhi[d]) {
+        *xi -= s->prd[d];
+        *xi = MAX(*xi, s->boxlo[d]);
+        s->image[3 * i + d]++;
+      }
+    }
+}
+
+/* comm_brick.cpp:151-398, one process: procgrid = 1x1x1, myloc = 0 */
+static void comm_setup(mdref *s) {
+  double cut = s->cutneighmax;
+  s->cutghost = cut;
+  for (int d = 0; d < 3; d++) {
+    s->maxneed[d] = (int)(cut * 1 / s->prd[d]) + 1;
+    if (!s->periodic[d]) s->maxneed[d] = MIN(s->maxneed[d], 0);
+  }
+  s->nswap = 2 * (s->maxneed[0] + s->maxneed[1] + s->maxneed[2]);
+  int iswap = 0;
+  for (int dim = 0; dim < 3; dim++)
+    for (int ineed = 0; ineed < 2 * s->maxneed[dim]; ineed++) {
+      s->pbc_flag[iswap] = 0;
+      s->pbc[iswap][0] = s->pbc[iswap][1] = s->pbc[iswap][2] = 0;
+      s->swapdim[iswap] = dim;
+      double sublo = s->boxlo[dim], subhi = s->boxhi[dim];
+      if (ineed % 2 == 0) {
+        s->slablo[iswap] = ineed < 2 ? -BIG : 0.5 * (sublo + subhi);
+        s->slabhi[iswap] = sublo + s->cutghost;
+        s->pbc_flag[iswap] = 1;
+        s->pbc[iswap][dim] = 1;
+      } else {
+        s->slablo[iswap] = subhi - s->cutghost;
+        s->slabhi[iswap] = ineed < 2 ? BIG : 0.5 * (sublo + subhi);
+        s->pbc_flag[iswap] = 1;
+        s->pbc[iswap][dim] = -1;
+      }
+      iswap++;
+    }
+}
+
+/* comm_brick.cpp:696-864 with atom_vec_atomic.cpp:268-313 (pack_border) */
+static void comm_borders(mdref *s) {
+  int iswap = 0;
+  s->nghost = 0;
+  for (int dim = 0; dim < 3; dim++) {
+    int nfirst = 0, nlast = 0;
+    int twoneed = 2 * s->maxneed[dim];
+    for (int ineed = 0; ineed < twoneed; ineed++) {
+      double lo = s->slablo[iswap], hi = s->slabhi[iswap];
+      if (ineed % 2 == 0) {
+        nfirst = nlast;
+        nlast = s->nlocal + s->nghost;
+      }
+      int nsend = 0;
+      for (int i = nfirst; i < nlast; i++) {
+        double c = s->x[3 * i + dim];
+        if (c >= lo && c <= hi) {
+          if (nsend == s->maxsendlist[iswap]) {
+            s->maxsendlist[iswap] = s->maxsendlist[iswap] ? 2 * s->maxsendlist[iswap] : 1024;
+            s->sendlist[iswap] = (int *)realloc(s->sendlist[iswap], sizeof(int) * s->maxsendlist[iswap]);
+          }
+          s->sendlist[iswap][nsend++] = i;
+        }
+      }
+      double dx = s->pbc[iswap][0] * s->prd[0];
+      double dy = s->pbc[iswap][1] * s->prd[1];
+      double dz = s->pbc[iswap][2] * s->prd[2];
+      int first = s->nlocal + s->nghost;
+      grow_atoms(s, first + nsend);
+      for (int k = 0; k < nsend; k++) {
+        int j = s->sendlist[iswap][k];
+        int g = first + k;
+        if (s->pbc_flag[iswap]) {
+          s->x[3 * g] = s->x[3 * j] + dx;
+          s->x[3 * g + 1] = s->x[3 * j + 1] + dy;
+          s->x[3 * g + 2] = s->x[3 * j + 2] + dz;
+        } else {
+          s->x[3 * g] = s->x[3 * j];
+          s->x[3 * g + 1] = s->x[3 * j + 1];
+          s->x[3 * g + 2] = s->x[3 * j + 2];
+        }
+        s->tag[g] = s->tag[j];
+        s->type[g] = s->type[j];
+      }
+      s->sendnum[iswap] = nsend;
+      s->recvnum[iswap] = nsend;
+      s->firstrecv[iswap] = first;
+      s->nghost += nsend;
+      iswap++;
+    }
+  }
+}
+
+/* comm_brick.cpp:444-506 with atom_vec_atomic.cpp:105-139 (pack_comm) */
+static void comm_forward(mdref *s) {
+  for (int iswap = 0; iswap < s->nswap; iswap++) {
+    double dx = s->pbc[iswap][0] * s->prd[0];
+    double dy = s->pbc[iswap][1] * s->prd[1];
+    double dz = s->pbc[iswap][2] * s->prd[2];
+    int first = s->firstrecv[iswap];
+    for (int k = 0; k < s->sendnum[iswap]; k++) {
+      int j = s->sendlist[iswap][k];
+      int g = first + k;
+      if (s->pbc_flag[iswap]) {
+        s->x[3 * g] = s->x[3 * j] + dx;
+        s->x[3 * g + 1] = s->x[3 * j + 1] + dy;
+        s->x[3 * g + 2] = s->x[3 * j + 2] + dz;
+      } else {
+        s->x[3 * g] = s->x[3 * j];
+        s->x[3 * g + 1] = s->x[3 * j + 1];
+        s->x[3 * g + 2] = s->x[3 * j + 2];
+      }
+    }
+  }
+}
+
+/* comm_brick.cpp:513-560 (reverse_comm of f) */
+static void comm_reverse(mdref *s) {
+  for (int iswap = s->nswap - 1; iswap >= 0; iswap--) {
+    int first = s->firstrecv[iswap];
+    for (int k = 0; k < s->sendnum[iswap]; k++) {
+      int j = s->sendlist[iswap][k];
+      int g = first + k;
+      s->f[3 * j] += s->f[3 * g];
+      s->f[3 * j + 1] += s->f[3 * g + 1];
+      s->f[3 * j + 2] += s->f[3 * g + 2];
+    }
+  }
+}
+
+/* comm_brick.cpp:911-943 + pair_eam.cpp:835-856 (reverse rho) */
+static void comm_reverse_rho(mdref *s) {
+  for (int iswap = s->nswap - 1; iswap >= 0; iswap--) {
+    int first = s->firstrecv[iswap];
+    for (int k = 0; k < s->sendnum[iswap]; k++) s->rho[s->sendlist[iswap][k]] += s->rho[first + k];
+  }
+}
+
+/* comm_brick.cpp:871-904 + pair_eam.cpp:809-831 (forward fp) */
+static void comm_forward_fp(mdref *s) {
+  for (int iswap = 0; iswap < s->nswap; iswap++) {
+    int first = s->firstrecv[iswap];
+    for (int k = 0; k < s->sendnum[iswap]; k++) s->fp[first + k] = s->fp[s->sendlist[iswap][k]];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* atom sort: atom.cpp:1575-1746                                     */
+
+static void setup_sort_bins(mdref *s) {
+  double binsize = 0.5 * s->cutneighmax;
+  double bininv = 1.0 / binsize;
+  for (int d = 0; d < 3; d++) {
+    s->sbboxlo[d] = s->boxlo[d];
+    s->sbboxhi[d] = s->boxhi[d];
+  }
+  s->snbinx = (int)((s->sbboxhi[0] - s->sbboxlo[0]) * bininv);
+  s->snbiny = (int)((s->sbboxhi[1] - s->sbboxlo[1]) * bininv);
+  s->snbinz = (int)((s->sbboxhi[2] - s->sbboxlo[2]) * bininv);
+  if (s->snbinx == 0) s->snbinx = 1;
+  if (s->snbiny == 0) s->snbiny = 1;
+  if (s->snbinz == 0) s->snbinz = 1;
+  s->sbininvx = s->snbinx / (s->sbboxhi[0] - s->sbboxlo[0]);
+  s->sbininvy = s->snbiny / (s->sbboxhi[1] - s->sbboxlo[1]);
+  s->sbininvz = s->snbinz / (s->sbboxhi[2] - s->sbboxlo[2]);
+  s->snbins = s->snbinx * s->snbiny * s->snbinz;
+}
+
+static void atom_sort(mdref *s) {
+  s->nextsort = (s->ntimestep / s->sortfreq) * s->sortfreq + s->sortfreq;
+  if (s->snbins == 1) return;
+  int n = s->nlocal;
+  int *head = (int *)malloc(sizeof(int) * s->snbins);
+  int *next = (int *)malloc(sizeof(int) * n);
+  int *permute = (int *)malloc(sizeof(int) * n);
+  for (int i = 0; i < s->snbins; i++) head[i] = -1;
+  for (int i = n - 1; i >= 0; i--) {
+    int ix = (int)((s->x[3 * i] - s->sbboxlo[0]) * s->sbininvx);
+    int iy = (int)((s->x[3 * i + 1] - s->sbboxlo[1]) * s->sbininvy);
+    int iz = (int)((s->x[3 * i + 2] - s->sbboxlo[2]) * s->sbininvz);
+    ix = MAX(ix, 0); iy = MAX(iy, 0); iz = MAX(iz, 0);
+    ix = MIN(ix, s->snbinx - 1); iy = MIN(iy, s->snbiny - 1); iz = MIN(iz, s->snbinz - 1);
+    int ibin = iz * s->snbiny * s->snbinx + iy * s->snbinx + ix;
+    next[i] = head[ibin];
+    head[ibin] = i;
+  }
+  int m = 0;
+  for (int b = 0; b < s->snbins; b++)
+    for (int i = head[b]; i >= 0; i = next[i]) permute[m++] = i;
+  /* new atom k = old atom permute[k]; the in-place cycle walk of the
+     reference produces exactly this permutation */
+  double *nx = (double *)malloc(sizeof(double) * 3 * n), *nv = (double *)malloc(sizeof(double) * 3 * n);
+  int *nt = (int *)malloc(sizeof(int) * n), *ng = (int *)malloc(sizeof(int) * n),
+      *nim = (int *)malloc(sizeof(int) * 3 * n);
+  for (int k = 0; k < n; k++) {
+    int o = permute[k];
+    memcpy(&nx[3 * k], &s->x[3 * o], 3 * sizeof(double));
+    memcpy(&nv[3 * k], &s->v[3 * o], 3 * sizeof(double));
+    memcpy(&nim[3 * k], &s->image[3 * o], 3 * sizeof(int));
+    nt[k] = s->type[o];
+    ng[k] = s->tag[o];
+  }
+  memcpy(s->x, nx, sizeof(double) * 3 * n);
+  memcpy(s->v, nv, sizeof(double) * 3 * n);
+  memcpy(s->image, nim, sizeof(int) * 3 * n);
+  memcpy(s->type, nt, sizeof(int) * n);
+  memcpy(s->tag, ng, sizeof(int) * n);
+  free(nx); free(nv); free(nt); free(ng); free(nim);
+  free(head); free(next); free(permute);
+}
+
+/* ------------------------------------------------------------------ */
+/* neighbour bins and stencils                                        */
+
+/* neighbor.cpp:1708-1725 */
+static double bin_distance(mdref *s, int i, int j, int k) {
+  double delx, dely, delz;
+  if (i > 0) delx = (i - 1) * s->binsizex;
+  else if (i == 0) delx = 0.0;
+  else delx = (i + 1) * s->binsizex;
+  if (j > 0) dely = (j - 1) * s->binsizey;
+  else if (j == 0) dely = 0.0;
+  else dely = (j + 1) * s->binsizey;
+  if (k > 0) delz = (k - 1) * s->binsizez;
+  else if (k == 0) delz = 0.0;
+  else delz = (k + 1) * s->binsizez;
+  return delx * delx + dely * dely + delz * delz;
+}
+
+/* neighbor.cpp:1538-1702 + stencils neigh_stencil.cpp:84-98,143-158,434-448 */
+static int setup_bins(mdref *s) {
+  double bbox[3], bsubboxlo[3], bsubboxhi[3];
+  for (int d = 0; d < 3; d++) {
+    s->bboxlo[d] = s->boxlo[d];
+    s->bboxhi[d] = s->boxhi[d];
+    bsubboxlo[d] = s->boxlo[d] - s->cutghost;
+    bsubboxhi[d] = s->boxhi[d] + s->cutghost;
+    bbox[d] = s->bboxhi[d] - s->bboxlo[d];
+  }
+  double binsize_optimal = 0.5 * s->cutneighmax;
+  if (binsize_optimal == 0.0) binsize_optimal = bbox[0];
+  double binsizeinv = 1.0 / binsize_optimal;
+  s->nbinx = (int)(bbox[0] * binsizeinv);
+  s->nbiny = (int)(bbox[1] * binsizeinv);
+  s->nbinz = (int)(bbox[2] * binsizeinv);
+  if (s->nbinx == 0) s->nbinx = 1;
+  if (s->nbiny == 0) s->nbiny = 1;
+  if (s->nbinz == 0) s->nbinz = 1;
+  s->binsizex = bbox[0] / s->nbinx;
+  s->binsizey = bbox[1] / s->nbiny;
+  s->binsizez = bbox[2] / s->nbinz;
+  s->bininvx = 1.0 / s->binsizex;
+  s->bininvy = 1.0 / s->binsizey;
+  s->bininvz = 1.0 / s->binsizez;
+  if (binsize_optimal * s->bininvx > CUT2BIN_RATIO || binsize_optimal * s->bininvy > CUT2BIN_RATIO ||
+      binsize_optimal * s->bininvz > CUT2BIN_RATIO)
+    return fail(s, "Cannot use neighbor bins - box size << cutoff");
+  int mbinxhi, mbinyhi, mbinzhi;
+  double coord;
+  coord = bsubboxlo[0] - SMALL * bbox[0];
+  s->mbinxlo = (int)((coord - s->bboxlo[0]) * s->bininvx);
+  if (coord < s->bboxlo[0]) s->mbinxlo = s->mbinxlo - 1;
+  coord = bsubboxhi[0] + SMALL * bbox[0];
+  mbinxhi = (int)((coord - s->bboxlo[0]) * s->bininvx);
+  coord = bsubboxlo[1] - SMALL * bbox[1];
+  s->mbinylo = (int)((coord - s->bboxlo[1]) * s->bininvy);
+  if (coord < s->bboxlo[1]) s->mbinylo = s->mbinylo - 1;
+  coord = bsubboxhi[1] + SMALL * bbox[1];
+  mbinyhi = (int)((coord - s->bboxlo[1]) * s->bininvy);
+  coord = bsubboxlo[2] - SMALL * bbox[2];
+  s->mbinzlo = (int)((coord - s->bboxlo[2]) * s->bininvz);
+  if (coord < s->bboxlo[2]) s->mbinzlo = s->mbinzlo - 1;
+  coord = bsubboxhi[2] + SMALL * bbox[2];
+  mbinzhi = (int)((coord - s->bboxlo[2]) * s->bininvz);
+  s->mbinxlo -= 1; mbinxhi += 1; s->mbinx = mbinxhi - s->mbinxlo + 1;
+  s->mbinylo -= 1; mbinyhi += 1; s->mbiny = mbinyhi - s->mbinylo + 1;
+  s->mbinzlo -= 1; mbinzhi += 1; s->mbinz = mbinzhi - s->mbinzlo + 1;
+  s->mbins = s->mbinx * s->mbiny * s->mbinz;
+  if (s->mbins > s->maxbinhead) {
+    s->maxbinhead = s->mbins;
+    free(s->binhead);
+    s->binhead = (int *)malloc(sizeof(int) * s->mbins);
+  }
+  int sx = (int)(s->cutneighmax * s->bininvx);
+  if (sx * s->binsizex < s->cutneighmax) sx++;
+  int sy = (int)(s->cutneighmax * s->bininvy);
+  if (sy * s->binsizey < s->cutneighmax) sy++;
+  int sz = (int)(s->cutneighmax * s->bininvz);
+  if (sz * s->binsizez < s->cutneighmax) sz++;
+  int smax = (2 * sx + 1) * (2 * sy + 1) * (2 * sz + 1);
+  free(s->stencil);
+  s->stencil = (int *)malloc(sizeof(int) * smax);
+  s->nstencil = 0;
+  if (s->list_style == MDREF_LIST_HALF_NEWTON) {
+    for (int k = 0; k <= sz; k++)
+      for (int j = -sy; j <= sy; j++)
+        for (int i = -sx; i <= sx; i++)
+          if (k > 0 || j > 0 || (j == 0 && i > 0))
+            if (bin_distance(s, i, j, k) < s->cutneighmaxsq)
+              s->stencil[s->nstencil++] = k * s->mbiny * s->mbinx + j * s->mbinx + i;
+  } else {
+    for (int k = -sz; k <= sz; k++)
+      for (int j = -sy; j <= sy; j++)
+        for (int i = -sx; i <= sx; i++)
+          if (bin_distance(s, i, j, k) < s->cutneighmaxsq)
+            s->stencil[s->nstencil++] = k * s->mbiny * s->mbinx + j * s->mbinx + i;
+  }
+  return 0;
+}
+
+/* neighbor.cpp:1918-1983 */
+static int coord2bin(mdref *s, const double *x) {
+  int ix, iy, iz;
+  if (x[0] >= s->bboxhi[0]) ix = (int)((x[0] - s->bboxhi[0]) * s->bininvx) + s->nbinx;
+  else if (x[0] >= s->bboxlo[0]) {
+    ix = (int)((x[0] - s->bboxlo[0]) * s->bininvx);
+    ix = MIN(ix, s->nbinx - 1);
+  } else ix = (int)((x[0] - s->bboxlo[0]) * s->bininvx) - 1;
+  if (x[1] >= s->bboxhi[1]) iy = (int)((x[1] - s->bboxhi[1]) * s->bininvy) + s->nbiny;
+  else if (x[1] >= s->bboxlo[1]) {
+    iy = (int)((x[1] - s->bboxlo[1]) * s->bininvy);
+    iy = MIN(iy, s->nbiny - 1);
+  } else iy = (int)((x[1] - s->bboxlo[1]) * s->bininvy) - 1;
+  if (x[2] >= s->bboxhi[2]) iz = (int)((x[2] - s->bboxhi[2]) * s->bininvz) + s->nbinz;
+  else if (x[2] >= s->bboxlo[2]) {
+    iz = (int)((x[2] - s->bboxlo[2]) * s->bininvz);
+    iz = MIN(iz, s->nbinz - 1);
+  } else iz = (int)((x[2] - s->bboxlo[2]) * s->bininvz) - 1;
+  return (iz - s->mbinzlo) * s->mbiny * s->mbinx + (iy - s->mbinylo) * s->mbinx + (ix - s->mbinxlo);
+}
+
+/* neighbor.cpp:1868-1904 */
+static void bin_atoms(mdref *s) {
+  int nall = s->nlocal + s->nghost;
+  for (int i = 0; i < s->mbins; i++) s->binhead[i] = -1;
+  if (nall > s->maxbins) {
+    s->maxbins = s->nmax;
+    free(s->bins);
+    s->bins = (int *)malloc(sizeof(int) * s->maxbins);
+  }
+  for (int i = nall - 1; i >= 0; i--) {
+    int ibin = coord2bin(s, &s->x[3 * i]);
+    s->bins[i] = s->binhead[ibin];
+    s->binhead[ibin] = i;
+  }
+}
+
+static int list_push(mdref *s, long *used, int j) {
+  if (*used >= s->nneigh_alloc) {
+    s->nneigh_alloc = s->nneigh_alloc ? 2 * s->nneigh_alloc : (1 << 20);
+    s->neighbors = (int *)realloc(s->neighbors, sizeof(int) * s->nneigh_alloc);
+  }
+  s->neighbors[(*used)++] = j;
+  return 0;
+}
+
+/* neigh_half_bin.cpp:285-419 (newton), 32-133 (no newton); neigh_full.cpp:241-340 */
+static int build_list(mdref *s) {
+  bin_atoms(s);
+  int nlocal = s->nlocal;
+  if (nlocal > s->maxlist) {
+    s->maxlist = s->nmax;
+    free(s->ilist); free(s->numneigh); free(s->firstneigh);
+    s->ilist = (int *)malloc(sizeof(int) * s->maxlist);
+    s->numneigh = (int *)malloc(sizeof(int) * s->maxlist);
+    s->firstneigh = (int *)malloc(sizeof(int) * s->maxlist);
+  }
+  long used = 0;
+  int inum = 0;
+  const double *x = s->x;
+  for (int i = 0; i < nlocal; i++) {
+    long start = used;
+    int itype = s->type[i];
+    double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+    if (s->list_style == MDREF_LIST_HALF_NEWTON) {
+      for (int j = s->bins[i]; j >= 0; j = s->bins[j]) {
+        if (j >= nlocal) {
+          if (x[3 * j + 2] < ztmp) continue;
+          if (x[3 * j + 2] == ztmp) {
+            if (x[3 * j + 1] < ytmp) continue;
+            if (x[3 * j + 1] == ytmp && x[3 * j] < xtmp) continue;
+          }
+        }
+        double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq <= s->cutneighsq[itype][s->type[j]]) list_push(s, &used, j);
+      }
+      int ibin = coord2bin(s, &x[3 * i]);
+      for (int k = 0; k < s->nstencil; k++)
+        for (int j = s->binhead[ibin + s->stencil[k]]; j >= 0; j = s->bins[j]) {
+          double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+          double rsq = delx * delx + dely * dely + delz * delz;
+          if (rsq <= s->cutneighsq[itype][s->type[j]]) list_push(s, &used, j);
+        }
+    } else {
+      int ibin = coord2bin(s, &x[3 * i]);
+      for (int k = 0; k < s->nstencil; k++)
+        for (int j = s->binhead[ibin + s->stencil[k]]; j >= 0; j = s->bins[j]) {
+          if (s->list_style == MDREF_LIST_HALF_NONEWTON) {
+            if (j <= i) continue;
+          } else if (i == j) continue;
+          double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+          double rsq = delx * delx + dely * dely + delz * delz;
+          if (rsq <= s->cutneighsq[itype][s->type[j]]) list_push(s, &used, j);
+        }
+    }
+    s->ilist[inum++] = i;
+    s->firstneigh[i] = (int)start;
+    s->numneigh[i] = (int)(used - start);
+    if (s->numneigh[i] > s->one) return fail(s, "Neighbor list overflow, boost neigh_modify one");
+  }
+  s->inum = inum;
+  return 0;
+}
+
+/* neighbor.cpp:1380-1461 */
+static int neigh_build(mdref *s) {
+  s->ago = 0;
+  s->ncalls++;
+  if (s->dist_check) {
+    if (s->nlocal > s->maxhold) {
+      s->maxhold = s->nmax;
+      free(s->xhold);
+      s->xhold = (double *)malloc(sizeof(double) * 3 * s->maxhold);
+    }
+    memcpy(s->xhold, s->x, sizeof(double) * 3 * s->nlocal);
+  }
+  return build_list(s);
+}
+
+/* neighbor.cpp:1322-1372 (fixed box) */
+static int check_distance(mdref *s) {
+  int flag = 0;
+  for (int i = 0; i < s->nlocal; i++) {
+    double delx = s->x[3 * i] - s->xhold[3 * i];
+    double dely = s->x[3 * i + 1] - s->xhold[3 * i + 1];
+    double delz = s->x[3 * i + 2] - s->xhold[3 * i + 2];
+    double rsq = delx * delx + dely * dely + delz * delz;
+    if (rsq > s->triggersq) flag = 1;
+  }
+  if (flag && s->ago == MAX(s->every, s->delay)) s->ndanger++;
+  return flag;
+}
+
+/* neighbor.cpp:1294-1309 */
+static int neigh_decide(mdref *s) {
+  s->ago++;
+  if (s->ago >= s->delay && s->ago % s->every == 0) {
+    if (s->dist_check == 0) return 1;
+    return check_distance(s);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* energy / virial tally: pair.cpp:679-742, 769-855, 1402-1450         */
+
+static void ev_setup(mdref *s, int eflag, int vflag) {
+  s->evflag = 1;
+  s->eflag_either = eflag;
+  s->eflag_global = eflag % 2;
+  s->eflag_atom = eflag / 2;
+  s->vflag_either = vflag;
+  s->vflag_global = vflag % 4;
+  s->vflag_atom = vflag / 4;
+  if (s->eflag_global) s->eng_vdwl = 0.0;
+  if (s->vflag_global)
+    for (int i = 0; i < 6; i++) s->virial[i] = 0.0;
+  int n = s->nlocal + (s->newton_pair ? s->nghost : 0);
+  if (s->eflag_atom)
+    for (int i = 0; i < n; i++) s->eatom[i] = 0.0;
+  if (s->vflag_atom)
+    for (int i = 0; i < 6 * n; i++) s->vatom[i] = 0.0;
+  if (s->vflag_global == 2) {
+    s->vflag_fdotr = 1;
+    s->vflag_global = 0;
+    if (s->vflag_atom == 0) s->vflag_either = 0;
+    if (s->vflag_either == 0 && s->eflag_either == 0) s->evflag = 0;
+  } else s->vflag_fdotr = 0;
+}
+
+static void ev_tally(mdref *s, int i, int j, int nlocal, int newton_pair, double evdwl, double fpair,
+                     double delx, double dely, double delz) {
+  double v[6];
+  if (s->eflag_either) {
+    if (s->eflag_global) {
+      if (newton_pair) s->eng_vdwl += evdwl;
+      else {
+        double evdwlhalf = 0.5 * evdwl;
+        if (i < nlocal) s->eng_vdwl += evdwlhalf;
+        if (j < nlocal) s->eng_vdwl += evdwlhalf;
+      }
+    }
+    if (s->eflag_atom) {
+      double epairhalf = 0.5 * evdwl;
+      if (newton_pair || i < nlocal) s->eatom[i] += epairhalf;
+      if (newton_pair || j < nlocal) s->eatom[j] += epairhalf;
+    }
+  }
+  if (s->vflag_either) {
+    v[0] = delx * delx * fpair;
+    v[1] = dely * dely * fpair;
+    v[2] = delz * delz * fpair;
+    v[3] = delx * dely * fpair;
+    v[4] = delx * delz * fpair;
+    v[5] = dely * delz * fpair;
+    if (s->vflag_global) {
+      if (newton_pair) {
+        for (int k = 0; k < 6; k++) s->virial[k] += v[k];
+      } else {
+        if (i < nlocal)
+          for (int k = 0; k < 6; k++) s->virial[k] += 0.5 * v[k];
+        if (j < nlocal)
+          for (int k = 0; k < 6; k++) s->virial[k] += 0.5 * v[k];
+      }
+    }
+    if (s->vflag_atom) {
+      if (newton_pair || i < nlocal)
+        for (int k = 0; k < 6; k++) s->vatom[6 * i + k] += 0.5 * v[k];
+      if (newton_pair || j < nlocal)
+        for (int k = 0; k < 6; k++) s->vatom[6 * j + k] += 0.5 * v[k];
+    }
+  }
+}
+
+static void virial_fdotr_compute(mdref *s) {
+  int nall = s->nlocal + s->nghost;
+  for (int i = 0; i < nall; i++) {
+    const double *fi = &s->f[3 * i], *xi = &s->x[3 * i];
+    s->virial[0] += fi[0] * xi[0];
+    s->virial[1] += fi[1] * xi[1];
+    s->virial[2] += fi[2] * xi[2];
+    s->virial[3] += fi[1] * xi[0];
+    s->virial[4] += fi[2] * xi[0];
+    s->virial[5] += fi[2] * xi[1];
+  }
+  s->vflag_fdotr = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* pair compute                                                       */
+
+/* pair_lj_cut.cpp:68-141 (special_lj factors are 1 for an atomic system) */
+static void lj_compute(mdref *s, int eflag, int vflag) {
+  double evdwl = 0.0;
+  if (eflag || vflag) ev_setup(s, eflag, vflag);
+  else s->evflag = s->vflag_fdotr = 0;
+  double *x = s->x, *f = s->f;
+  int nlocal = s->nlocal, newton_pair = s->newton_pair;
+  int full = s->list_style == MDREF_LIST_FULL;
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+    int itype = s->type[i];
+    const int *jlist = &s->neighbors[s->firstneigh[i]];
+    int jnum = s->numneigh[i];
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj] & NEIGHMASK;
+      double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      int jtype = s->type[j];
+      if (rsq < s->cutsq[itype][jtype]) {
+        double r2inv = 1.0 / rsq;
+        double r6inv = r2inv * r2inv * r2inv;
+        double forcelj = r6inv * (s->lj1[itype][jtype] * r6inv - s->lj2[itype][jtype]);
+        double fpair = forcelj * r2inv;
+        f[3 * i] += delx * fpair;
+        f[3 * i + 1] += dely * fpair;
+        f[3 * i + 2] += delz * fpair;
+        if (!full && (newton_pair || j < nlocal)) {
+          f[3 * j] -= delx * fpair;
+          f[3 * j + 1] -= dely * fpair;
+          f[3 * j + 2] -= delz * fpair;
+        }
+        if (eflag) evdwl = r6inv * (s->lj3[itype][jtype] * r6inv - s->lj4[itype][jtype]) - s->offset[itype][jtype];
+        if (s->evflag) {
+          if (full) {
+            /* full list: every pair is visited from both sides; tally half
+               (pair.cpp:862-901 ev_tally_full) */
+            if (s->eflag_either) {
+              if (s->eflag_global) s->eng_vdwl += 0.5 * evdwl;
+              if (s->eflag_atom) s->eatom[i] += 0.5 * evdwl;
+            }
+            if (s->vflag_either) {
+              double v[6] = {0.5 * delx * delx * fpair, 0.5 * dely * dely * fpair, 0.5 * delz * delz * fpair,
+                             0.5 * delx * dely * fpair, 0.5 * delx * delz * fpair, 0.5 * dely * delz * fpair};
+              if (s->vflag_global)
+                for (int k = 0; k < 6; k++) s->virial[k] += v[k];
+              if (s->vflag_atom)
+                for (int k = 0; k < 6; k++) s->vatom[6 * i + k] += v[k];
+            }
+          } else ev_tally(s, i, j, nlocal, newton_pair, evdwl, fpair, delx, dely, delz);
+        }
+      }
+    }
+  }
+  if (s->vflag_fdotr) virial_fdotr_compute(s);
+}
+
+/* pair_eam.cpp:125-306 */
+static void eam_compute(mdref *s, int eflag, int vflag) {
+  double evdwl = 0.0;
+  if (eflag || vflag) ev_setup(s, eflag, vflag);
+  else s->evflag = s->vflag_fdotr = s->eflag_global = s->eflag_atom = 0;
+  double *x = s->x, *f = s->f;
+  int nlocal = s->nlocal, nall = nlocal + s->nghost, newton_pair = s->newton_pair;
+  int full = s->list_style == MDREF_LIST_FULL;
+  const int nr = s->nr, nrho = s->nrho;
+  const double rdr = s->rdr, rdrho = s->rdrho;
+#define RHOR(t, m) (&s->rhor_spline[((size_t)(t) * (nr + 1) + (m)) * 7])
+#define Z2R(t, m) (&s->z2r_spline[((size_t)(t) * (nr + 1) + (m)) * 7])
+#define FRHO(t, m) (&s->frho_spline[((size_t)(t) * (nrho + 1) + (m)) * 7])
+  if (newton_pair && !full)
+    for (int i = 0; i < nall; i++) s->rho[i] = 0.0;
+  else
+    for (int i = 0; i < nlocal; i++) s->rho[i] = 0.0;
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+    int itype = s->type[i];
+    const int *jlist = &s->neighbors[s->firstneigh[i]];
+    int jnum = s->numneigh[i];
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj] & NEIGHMASK;
+      double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < s->cutforcesq) {
+        int jtype = s->type[j];
+        double p = sqrt(rsq) * rdr + 1.0;
+        int m = (int)p;
+        m = MIN(m, nr - 1);
+        p -= m;
+        p = MIN(p, 1.0);
+        const double *coeff = RHOR(s->type2rhor[jtype][itype], m);
+        s->rho[i] += ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+        if (!full && (newton_pair || j < nlocal)) {
+          coeff = RHOR(s->type2rhor[itype][jtype], m);
+          s->rho[j] += ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+        }
+      }
+    }
+  }
+  if (newton_pair && !full) comm_reverse_rho(s);
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    double p = s->rho[i] * rdrho + 1.0;
+    int m = (int)p;
+    m = MAX(1, MIN(m, nrho - 1));
+    p -= m;
+    p = MIN(p, 1.0);
+    const double *coeff = FRHO(s->type2frho[s->type[i]], m);
+    s->fp[i] = (coeff[0] * p + coeff[1]) * p + coeff[2];
+    if (eflag) {
+      double phi = ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+      if (s->rho[i] > s->rhomax) phi += s->fp[i] * (s->rho[i] - s->rhomax);
+      if (s->eflag_global) s->eng_vdwl += phi;
+      if (s->eflag_atom) s->eatom[i] += phi;
+    }
+  }
+  comm_forward_fp(s);
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+    int itype = s->type[i];
+    const int *jlist = &s->neighbors[s->firstneigh[i]];
+    int jnum = s->numneigh[i];
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj] & NEIGHMASK;
+      double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < s->cutforcesq) {
+        int jtype = s->type[j];
+        double r = sqrt(rsq);
+        double p = r * rdr + 1.0;
+        int m = (int)p;
+        m = MIN(m, nr - 1);
+        p -= m;
+        p = MIN(p, 1.0);
+        const double *coeff = RHOR(s->type2rhor[itype][jtype], m);
+        double rhoip = (coeff[0] * p + coeff[1]) * p + coeff[2];
+        coeff = RHOR(s->type2rhor[jtype][itype], m);
+        double rhojp = (coeff[0] * p + coeff[1]) * p + coeff[2];
+        coeff = Z2R(s->type2z2r[itype][jtype], m);
+        double z2p = (coeff[0] * p + coeff[1]) * p + coeff[2];
+        double z2 = ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+        double recip = 1.0 / r;
+        double phi = z2 * recip;
+        double phip = z2p * recip - phi * recip;
+        double psip = s->fp[i] * rhojp + s->fp[j] * rhoip + phip;
+        double fpair = -psip * recip;
+        f[3 * i] += delx * fpair;
+        f[3 * i + 1] += dely * fpair;
+        f[3 * i + 2] += delz * fpair;
+        if (!full && (newton_pair || j < nlocal)) {
+          f[3 * j] -= delx * fpair;
+          f[3 * j + 1] -= dely * fpair;
+          f[3 * j + 2] -= delz * fpair;
+        }
+        if (eflag) evdwl = phi;
+        if (s->evflag) {
+          if (full) {
+            if (s->eflag_either) {
+              if (s->eflag_global) s->eng_vdwl += 0.5 * evdwl;
+              if (s->eflag_atom) s->eatom[i] += 0.5 * evdwl;
+            }
+            if (s->vflag_either) {
+              double v[6] = {0.5 * delx * delx * fpair, 0.5 * dely * dely * fpair, 0.5 * delz * delz * fpair,
+                             0.5 * delx * dely * fpair, 0.5 * delx * delz * fpair, 0.5 * dely * delz * fpair};
+              if (s->vflag_global)
+                for (int k = 0; k < 6; k++) s->virial[k] += v[k];
+              if (s->vflag_atom)
+                for (int k = 0; k < 6; k++) s->vatom[6 * i + k] += v[k];
+            }
+          } else ev_tally(s, i, j, nlocal, newton_pair, evdwl, fpair, delx, dely, delz);
+        }
+      }
+    }
+  }
+  if (s->vflag_fdotr) virial_fdotr_compute(s);
+#undef RHOR
+#undef Z2R
+#undef FRHO
+}
+
+/* verlet.cpp:325-371 */
+static void force_clear(mdref *s) {
+  int n = s->nlocal + (s->newton_pair ? s->nghost : 0);
+  memset(s->f, 0, sizeof(double) * 3 * n);
+}
+
+static void pair_compute(mdref *s, int eflag, int vflag) {
+  if (s->pair_style == MDREF_PAIR_LJ) lj_compute(s, eflag, vflag);
+  else eam_compute(s, eflag, vflag);
+}
+
+/* fix_nve.cpp:65-105, 109-142 */
+static void nve_initial(mdref *s) {
+  double dtv = s->dt, dtf = 0.5 * s->dt * s->ftm2v;
+  for (int i = 0; i < s->nlocal; i++) {
+    double dtfm = dtf / s->mass[s->type[i]];
+    s->v[3 * i] += dtfm * s->f[3 * i];
+    s->v[3 * i + 1] += dtfm * s->f[3 * i + 1];
+    s->v[3 * i + 2] += dtfm * s->f[3 * i + 2];
+    s->x[3 * i] += dtv * s->v[3 * i];
+    s->x[3 * i + 1] += dtv * s->v[3 * i + 1];
+    s->x[3 * i + 2] += dtv * s->v[3 * i + 2];
+  }
+}
+
+static void nve_final(mdref *s) {
+  double dtf = 0.5 * s->dt * s->ftm2v;
+  for (int i = 0; i < s->nlocal; i++) {
+    double dtfm = dtf / s->mass[s->type[i]];
+    s->v[3 * i] += dtfm * s->f[3 * i];
+    s->v[3 * i + 1] += dtfm * s->f[3 * i + 1];
+    s->v[3 * i + 2] += dtfm * s->f[3 * i + 2];
+  }
+}
+
+/* thermo row: Step Temp E_pair E_mol TotEng Press (thermo.cpp:64,1595-1620;
+   compute_pressure.cpp:177-211; energies normalised per atom in lj units) */
+static void thermo_row(mdref *s, double *row) {
+  double t = compute_temp(s);
+  double dof = temp_dof(s);
+  double pe = s->eng_vdwl;
+  int norm = s->units == MDREF_UNITS_LJ;
+  double natoms = s->natoms;
+  double epair = norm ? pe / natoms : pe;
+  double ke = t * 0.5 * dof * s->boltz;
+  if (norm) ke /= natoms;
+  double inv_volume = 1.0 / (s->prd[0] * s->prd[1] * s->prd[2]);
+  double press = (dof * s->boltz * t + s->virial[0] + s->virial[1] + s->virial[2]) / 3.0 * inv_volume * s->nktv2p;
+  row[0] = (double)s->ntimestep;
+  row[1] = t;
+  row[2] = epair;
+  row[3] = 0.0;
+  row[4] = epair + ke;
+  row[5] = press;
+}
+
+/* verlet.cpp:88-142 */
+int mdref_setup(mdref *s) {
+  if (!s->box_exists) return fail(s, "no box");
+  if (init_all(s)) return -1;
+  if (s->sortfreq > 0) setup_sort_bins(s);
+  domain_pbc(s);
+  comm_setup(s);
+  if (setup_bins(s)) return -1;
+  if (s->sortfreq > 0) atom_sort(s);
+  comm_borders(s);
+  if (neigh_build(s)) return -1;
+  s->ncalls = 0;
+  s->ndanger = 0;
+  s->eflag = 1;
+  s->vflag = s->newton_pair && s->list_style != MDREF_LIST_FULL ? 2 : 1;
+  force_clear(s);
+  pair_compute(s, s->eflag, s->vflag);
+  if (s->newton_pair && s->list_style != MDREF_LIST_FULL) comm_reverse(s);
+  s->setup_done = 1;
+  return 0;
+}
+
+/* verlet.cpp:207-309; eflag/vflag only on thermo steps (integrate.cpp:114-149) */
+int mdref_run(mdref *s, int nsteps, double *thermo, int max_rows, int *nrows) {
+  if (!s->setup_done) return fail(s, "setup not done");
+  int rows = 0;
+  long first = s->ntimestep, last = s->ntimestep + nsteps;
+  if (thermo && rows < max_rows) thermo_row(s, &thermo[6 * rows++]);
+  int newton_comm = s->newton_pair && s->list_style != MDREF_LIST_FULL;
+  int virial_style = newton_comm ? 2 : 1;
+  for (int n = 0; n < nsteps; n++) {
+    long step = ++s->ntimestep;
+    int out = (step == last) || (s->thermo_every > 0 && (step - first) % s->thermo_every == 0);
+    int eflag = out ? 1 : 0, vflag = out ? virial_style : 0;
+    nve_initial(s);
+    int nflag = neigh_decide(s);
+    if (nflag == 0) {
+      comm_forward(s);
+    } else {
+      domain_pbc(s);
+      if (s->sortfreq > 0 && step >= s->nextsort) atom_sort(s);
+      comm_borders(s);
+      if (neigh_build(s)) return -1;
+    }
+    force_clear(s);
+    pair_compute(s, eflag, vflag);
+    if (newton_comm) comm_reverse(s);
+    nve_final(s);
+    if (out && thermo && rows < max_rows) thermo_row(s, &thermo[6 * rows++]);
+  }
+  if (nrows) *nrows = rows;
+  return 0;
+}
+
+int mdref_compute(mdref *s, int eflag, int vflag) {
+  if (!s->setup_done) {
+    if (!s->box_exists) return fail(s, "no box");
+    if (init_all(s)) return -1;
+    comm_setup(s);
+    if (setup_bins(s)) return -1;
+    s->setup_done = 1;
+  }
+  domain_pbc(s);
+  comm_borders(s);
+  if (neigh_build(s)) return -1;
+  force_clear(s);
+  pair_compute(s, eflag, vflag);
+  int newton_comm = s->newton_pair && s->list_style != MDREF_LIST_FULL;
+  if (newton_comm) {
+    comm_reverse(s);
+    /* per-atom ghost contributions back to owners (comm->reverse_comm_compute) */
+    for (int iswap = s->nswap - 1; iswap >= 0; iswap--) {
+      int fr = s->firstrecv[iswap];
+      for (int k = 0; k < s->sendnum[iswap]; k++) {
+        int j = s->sendlist[iswap][k];
+        if (s->eflag_atom) s->eatom[j] += s->eatom[fr + k];
+        if (s->vflag_atom)
+          for (int c = 0; c < 6; c++) s->vatom[6 * j + c] += s->vatom[6 * (fr + k) + c];
+      }
+    }
+  }
+  s->eflag = eflag;
+  s->vflag = vflag;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* accessors                                                          */
+
+int mdref_natoms(mdref *s) { return s->natoms; }
+int mdref_nghost(mdref *s) { return s->nghost; }
+long mdref_ntimestep(mdref *s) { return s->ntimestep; }
+
+int mdref_get_by_tag(mdref *s, double *x, double *v, double *f, int *type, int *image) {
+  for (int i = 0; i < s->nlocal; i++) {
+    int r = s->tag[i] - 1;
+    if (r < 0 || r >= s->natoms) return fail(s, "bad tag");
+    for (int d = 0; d < 3; d++) {
+      if (x) x[3 * r + d] = s->x[3 * i + d];
+      if (v) v[3 * r + d] = s->v[3 * i + d];
+      if (f) f[3 * r + d] = s->f[3 * i + d];
+      if (image) image[3 * r + d] = s->image[3 * i + d];
+    }
+    if (type) type[r] = s->type[i];
+  }
+  return 0;
+}
+
+int mdref_get_peratom_by_tag(mdref *s, double *eatom, double *vatom) {
+  for (int i = 0; i < s->nlocal; i++) {
+    int r = s->tag[i] - 1;
+    if (eatom) eatom[r] = s->eatom[i];
+    if (vatom)
+      for (int c = 0; c < 6; c++) vatom[6 * r + c] = s->vatom[6 * i + c];
+  }
+  return 0;
+}
+
+int mdref_get_eam_by_tag(mdref *s, double *rho, double *fp) {
+  if (s->pair_style != MDREF_PAIR_EAM) return fail(s, "not eam");
+  for (int i = 0; i < s->nlocal; i++) {
+    int r = s->tag[i] - 1;
+    if (rho) rho[r] = s->rho[i];
+    if (fp) fp[r] = s->fp[i];
+  }
+  return 0;
+}
+
+int mdref_energy_virial(mdref *s, double *eng, double *virial6) {
+  if (eng) *eng = s->eng_vdwl;
+  if (virial6)
+    for (int k = 0; k < 6; k++) virial6[k] = s->virial[k];
+  return 0;
+}
+
+int mdref_thermo_now(mdref *s, double *row6) {
+  thermo_row(s, row6);
+  return 0;
+}
+
+int mdref_neigh_stats(mdref *s, long *total, int *builds, int *dangerous, int *maxneigh) {
+  long t = 0;
+  int mx = 0;
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    t += s->numneigh[i];
+    mx = MAX(mx, s->numneigh[i]);
+  }
+  if (total) *total = t;
+  if (builds) *builds = s->ncalls;
+  if (dangerous) *dangerous = s->ndanger;
+  if (maxneigh) *maxneigh = mx;
+  return 0;
+}
+
+long mdref_list_pairs(mdref *s, int *pairs, long maxpairs) {
+  long n = 0;
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    const int *jlist = &s->neighbors[s->firstneigh[i]];
+    for (int jj = 0; jj < s->numneigh[i]; jj++) {
+      int j = jlist[jj] & NEIGHMASK;
+      if (pairs && n < maxpairs) {
+        pairs[2 * n] = s->tag[i];
+        pairs[2 * n + 1] = s->tag[j];
+      }
+      n++;
+    }
+  }
+  return n;
+}
