@@ -1,0 +1,158 @@
+/* mdx.h - C ABI of the MI355X (gfx950) short-range MD hot path.
+ *
+ * The library replaces, behind the reference's own plugin surface, the work
+ * LAMMPS does in Neighbor::build (binned lists), PairLJCut::compute,
+ * PairEAM::compute (rho -> fp -> force) and FixNVE::initial/final_integrate.
+ * Reference citations are relative to the LAMMPS "8 Aug 2014" source tree.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: MDX_OK (0) or a negative
+ *    MDX_ERR_* code; mdx_last_error() returns the message of the last failure
+ *    on the calling thread.  (The reference's GPU package reports failure the
+ *    same way through an int return of *_gpu_init and a bool& success of
+ *    *_gpu_compute: src/GPU/pair_lj_cut_gpu.cpp:42-65.)
+ *  - Host arrays are borrowed for the duration of a call only.  Per-atom
+ *    arrays are AoS [n][3] doubles laid out as LAMMPS's atom->x[0]
+ *    (src/memory.h 2-d create), in the caller's atom order.
+ *  - Per-type tables are (ntypes+1)^2 row-major doubles, indexed [i][j] with
+ *    1-based types, exactly as Pair::init_one fills them.
+ *  - A context owns all device memory; one context per host thread.
+ *    Calls are synchronous at return.
+ *  - Plain pointers and sizes only; no HIP or torch types cross the ABI.
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_VERSION 1
+
+enum {
+  MDX_OK = 0,
+  MDX_ERR_ARG = -1,      /* invalid argument or state */
+  MDX_ERR_HIP = -2,      /* HIP runtime failure (no device, launch failure) */
+  MDX_ERR_NOMEM = -3,    /* device allocation failed ("Insufficient memory on accelerator") */
+  MDX_ERR_OVERFLOW = -4  /* an atom has more than neigh_modify `one` neighbours */
+};
+
+enum { MDX_FP64 = 0, MDX_MIXED = 1 };        /* pair arithmetic precision */
+enum { MDX_PAIR_NONE = 0, MDX_PAIR_LJ = 1, MDX_PAIR_EAM = 2 };
+
+typedef struct mdx_ctx mdx_ctx;
+
+/* Thermo callback: step, then {temp-independent raw sums}:
+   row[0] = sum_i m_i v_i^2 (mass*velocity^2, no unit factor)
+   row[1] = pair energy (eng_vdwl), row[2..7] = pair virial xx yy zz xy xz yz */
+typedef void (*mdx_thermo_fn)(void *user, long step, const double *row);
+
+typedef struct {
+  long ncalls;          /* neighbour builds during mdx_run (Neighbor::ncalls) */
+  long ndanger;         /* dangerous builds (neighbor.cpp:1370) */
+  long total_neighbors; /* full-list entries at the last build (= 2x half-list pairs) */
+  int max_neighbors;    /* largest per-atom count at the last build */
+  int capacity;         /* per-atom list capacity currently allocated */
+  int nbins[3];         /* device bin grid */
+  double ms_neigh;      /* accumulated device time of list builds (ms) */
+  double ms_pair;       /* accumulated device time of pair kernels (ms) */
+  double ms_integrate;  /* accumulated device time of integrator kernels (ms) */
+} mdx_stats;
+
+const char *mdx_last_error(void);
+int mdx_version(void);
+
+/* Creates a context on HIP device `device` (the accelerator set-up that
+   `package gpu` performs in src/GPU/fix_gpu.cpp:77-150). */
+int mdx_create(int device, mdx_ctx **out);
+int mdx_destroy(mdx_ctx *ctx);
+
+/* Orthogonal box and periodicity (Domain boxlo/boxhi/periodicity,
+   src/domain.h).  Non-periodic dimensions behave as `boundary f`. */
+int mdx_set_box(mdx_ctx *ctx, const double lo[3], const double hi[3], const int periodic[3]);
+
+/* Atoms: n owned atoms with x, v ([n][3]), 1-based type, tag (atom ID), and
+   per-type mass[ntypes+1] (src/atom.h x/v/type/tag/mass).  v may be NULL. */
+int mdx_set_atoms(mdx_ctx *ctx, int n, int ntypes, const double *x, const double *v, const int *type,
+                  const int *tag, const double *mass);
+/* Per-step position update in caller order: the role of the H2D copy that
+   ljl_gpu_compute performs on atom->x each step (src/GPU/pair_lj_cut_gpu.cpp:87-122). */
+int mdx_set_positions(mdx_ctx *ctx, const double *x);
+/* Copies back state in caller order; any pointer may be NULL.  image is
+   3 ints per atom (periodic image counts, src/domain.cpp:478 pbc). */
+int mdx_get_atoms(mdx_ctx *ctx, double *x, double *v, double *f, int *image);
+
+/* lj/cut coefficients as PairLJCut::init_one leaves them
+   (src/pair_lj_cut.cpp:545-602; replaces ljl_gpu_init, src/GPU/pair_lj_cut_gpu.cpp:42-48). */
+int mdx_pair_lj_init(mdx_ctx *ctx, int ntypes, const double *cutsq, const double *lj1, const double *lj2,
+                     const double *lj3, const double *lj4, const double *offset);
+
+/* eam spline tables as PairEAM::array2spline leaves them
+   (src/MANYBODY/pair_eam.cpp:698-751; members pair_eam.h:37-45):
+   frho_spline[nfrho][nrho+1][7], rhor_spline[nrhor][nr+1][7], z2r_spline[nz2r][nr+1][7],
+   type2frho[ntypes+1], type2rhor/type2z2r[(ntypes+1)^2].
+   Replaces eam_gpu_init (src/GPU/pair_eam_gpu.cpp:38-47). */
+int mdx_pair_eam_init(mdx_ctx *ctx, int ntypes, int nrho, double drho, int nr, double dr, double rhomax,
+                      double cutforcesq, int nfrho, const double *frho_spline, int nrhor,
+                      const double *rhor_spline, int nz2r, const double *z2r_spline, const int *type2frho,
+                      const int *type2rhor, const int *type2z2r);
+
+/* FP64, or mixed: FP64 positions and distances, FP32 potential arithmetic,
+   FP64 force/energy/virial accumulation. */
+int mdx_set_precision(mdx_ctx *ctx, int precision);
+
+/* `neighbor <skin> bin` + `neigh_modify every N delay N check yes|no one N`
+   (src/neighbor.cpp:1731-1862). */
+int mdx_neigh_modify(mdx_ctx *ctx, double skin, int every, int delay, int check, int one);
+
+/* Neighbor::build on the current positions (wrap into the box, bin-sort,
+   list build; src/neighbor.cpp:1380-1461, src/neigh_half_bin.cpp:285-419). */
+int mdx_neigh_build(mdx_ctx *ctx);
+/* Pair set of the current list as (tag_i, tag_j) with tag_i < tag_j, each
+   pair once (the half-list pair set), in no particular order.  *npairs gets
+   the count; pairs may be NULL to query it. */
+int mdx_neigh_pairs(mdx_ctx *ctx, int *pairs, long cap, long *npairs);
+
+/* Pair::compute(eflag, vflag) on the current positions (src/pair.h:135;
+   eflag/vflag encoding of src/integrate.cpp:99-112: eflag 1 global, 2 per-atom;
+   vflag 1|2 global, 4 per-atom).  Builds the list first if none exists.
+   Outputs are optional: f [n][3] (overwritten, caller order), eng_vdwl,
+   virial[6], eatom[n], vatom[n][6].  Replaces ljl_gpu_compute /
+   eam_gpu_compute + eam_gpu_compute_force + lmp_gpu_forces
+   (src/GPU/pair_lj_cut_gpu.cpp:87-122, src/GPU/pair_eam_gpu.cpp:96-140,
+   src/GPU/fix_gpu.cpp:236-257). */
+int mdx_compute(mdx_ctx *ctx, int eflag, int vflag, double *f, double *eng_vdwl, double *virial,
+                double *eatom, double *vatom);
+
+/* EAM intermediate arrays after the last compute (rho, fp), caller order. */
+int mdx_eam_rho_fp(mdx_ctx *ctx, double *rho, double *fp);
+
+/* Verlet::setup (src/verlet.cpp:88-142): wrap, sort and build the list,
+   compute forces with energy and virial, and report the current step to the
+   callback.  Resets the build counters, as setup does (neighbor->ncalls = 0). */
+int mdx_setup(mdx_ctx *ctx, mdx_thermo_fn cb, void *user);
+
+/* Verlet::run with fix nve, fully device resident
+   (src/verlet.cpp:88-142,207-309; src/fix_nve.cpp:65-142;
+   src/neighbor.cpp:1294-1372).  Step numbering continues across calls.
+   dt and ftm2v as Update/Force hold them (src/update.cpp:118-180).
+   Calls mdx_setup first if it has not run since the state last changed.
+   The callback (may be NULL) receives every step that is a multiple of
+   thermo_every (0: none) and the last step. */
+int mdx_run(mdx_ctx *ctx, int nsteps, double dt, double ftm2v, int thermo_every, mdx_thermo_fn cb,
+            void *user);
+
+int mdx_get_stats(mdx_ctx *ctx, mdx_stats *out);
+
+/* Times every list build, pair launch and step with HIP events on the
+   context's stream (adds a sync per phase); resets the accumulated times. */
+int mdx_set_profiling(mdx_ctx *ctx, int on);
+
+/* Kernel shape overrides for the autotuner (0 = library default):
+   block = threads per workgroup, tpa = threads per atom in pair kernels. */
+int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

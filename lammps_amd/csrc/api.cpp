@@ -1,0 +1,651 @@
+// api.cpp - the extern "C" boundary of libmdx (declared in include/mdx.h).
+//
+// Host-side orchestration of the device path: context lifetime, uploads in
+// the caller's atom order, Neighbor::decide (src/neighbor.cpp:1294-1309) and
+// the Verlet::setup / Verlet::run step order (src/verlet.cpp:88-142,207-309)
+// around the kernels of neigh.hip, pair.hip and integrate.hip.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "mdx_internal.h"
+
+namespace mdx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? MDX_ERR_NOMEM : MDX_ERR_HIP;
+}
+
+// device-order gathers into caller order (integrate.hip)
+int gather_w(mdx_ctx *c, const double4 *dev, double *host_out);
+int gather_soa(mdx_ctx *c, const double *dev, int ncols, double *host_out);
+
+}  // namespace mdx
+
+using namespace mdx;
+
+namespace {
+
+template <typename T>
+int dev_alloc(T **p, size_t count) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count);
+  if (e != hipSuccess) {
+    set_error("Insufficient memory on accelerator");
+    return MDX_ERR_NOMEM;
+  }
+  return MDX_OK;
+}
+
+int check_ctx(mdx_ctx *c) {
+  if (!c) {
+    set_error("null context");
+    return MDX_ERR_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return MDX_OK;
+}
+
+void free_atoms(mdx_ctx *c) {
+  void *ptrs[] = {c->x, c->xb, c->v, c->vb, c->f, c->xhold, c->tag, c->tagb, c->orig, c->origb, c->image,
+                  c->imageb, c->inv, c->vatom, c->dscratch, c->bkey, c->bkey_s, c->bidx, c->bidx_s,
+                  c->numneigh, c->nbr, c->rho, c->fp, c->eemb, c->partials};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  c->x = c->xb = c->v = c->vb = c->f = c->xhold = nullptr;
+  c->tag = c->tagb = c->orig = c->origb = c->image = c->imageb = c->inv = nullptr;
+  c->vatom = c->dscratch = nullptr;
+  c->bkey = c->bkey_s = nullptr;
+  c->bidx = c->bidx_s = c->numneigh = c->nbr = nullptr;
+  c->rho = c->fp = c->eemb = c->partials = nullptr;
+  if (c->host_stage) hipHostFree(c->host_stage);
+  if (c->host_istage) hipHostFree(c->host_istage);
+  c->host_stage = nullptr;
+  c->host_istage = nullptr;
+  c->cap_atoms = 0;
+  c->maxneigh = 0;
+  c->nbr_elems = 0;
+  c->cap_partials = 0;
+}
+
+// Pair::init + Neighbor::init cutoffs (src/pair.cpp:219-231, src/neighbor.cpp:239-279)
+int prepare(mdx_ctx *c) {
+  if (!c->box_set) {
+    set_error("box not set (mdx_set_box)");
+    return MDX_ERR_ARG;
+  }
+  if (c->pair_style == MDX_PAIR_NONE) {
+    set_error("no pair style (mdx_pair_lj_init / mdx_pair_eam_init)");
+    return MDX_ERR_ARG;
+  }
+  if (c->ntypes < 1) {
+    set_error("atoms not set (mdx_set_atoms)");
+    return MDX_ERR_ARG;
+  }
+  const int nt1 = c->ntypes + 1;
+  c->triggersq = 0.25 * c->skin * c->skin;
+  c->cutneighmax = 0.0;
+  c->cutforce = 0.0;
+  for (int i = 1; i <= c->ntypes; i++)
+    for (int j = 1; j <= c->ntypes; j++) {
+      const double cutoff = std::sqrt(c->cutsq[i * nt1 + j]);
+      const double cut = cutoff + (cutoff > 0.0 ? c->skin : 0.0);
+      c->na.cutneighsq[i * nt1 + j] = cut * cut;
+      c->cutneighmax = std::max(c->cutneighmax, cut);
+      c->cutforce = std::max(c->cutforce, cutoff);
+    }
+  int rc = neigh_setup_grid(c);
+  if (rc) return rc;
+  // first-guess list capacity from the mean density; neigh_build regrows
+  if (!c->nbr && c->n > 0) {
+    const double vol = c->prd[0] * c->prd[1] * c->prd[2];
+    const double rc3 = c->cutneighmax * c->cutneighmax * c->cutneighmax;
+    int guess = (int)(1.5 * 4.18879 * rc3 * c->n / vol) + 16;
+    guess = std::min(((guess + 15) / 16) * 16, c->one);
+    if ((rc = dev_alloc(&c->nbr, (size_t)guess * c->npad))) return rc;
+    c->maxneigh = guess;
+    c->nbr_elems = (long)guess * c->npad;
+  }
+  const int G = (c->n + 63) / 64 + 1;  // enough for any block size >= 64
+  if (G * 8 > c->cap_partials) {
+    if ((rc = dev_alloc(&c->partials, (size_t)G * 8))) return rc;
+    c->cap_partials = G * 8;
+  }
+  return MDX_OK;
+}
+
+int ensure_built(mdx_ctx *c) {
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (!c->built) return neigh_build(c);
+  return MDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mdx_last_error(void) { return g_err.c_str(); }
+
+int mdx_version(void) { return MDX_VERSION; }
+
+int mdx_create(int device, mdx_ctx **out) {
+  if (!out) {
+    set_error("null output pointer");
+    return MDX_ERR_ARG;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    set_error("no HIP device available");
+    return MDX_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("device index out of range");
+    return MDX_ERR_ARG;
+  }
+  if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+  mdx_ctx *c = new mdx_ctx();
+  c->device = device;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    delete c;
+    return hip_fail(e, "stream");
+  }
+  for (auto &ev : c->ev) hipEventCreate(&ev);
+  if (hipMalloc(&c->dflags, 64) != hipSuccess || hipMalloc(&c->red_out, sizeof(double) * MDX_RED_SLOTS) != hipSuccess ||
+      hipHostMalloc(&c->red_host, sizeof(double) * MDX_RED_SLOTS, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&c->lj_dev, sizeof(LJTab)) != hipSuccess) {
+    set_error("Insufficient memory on accelerator");
+    return MDX_ERR_NOMEM;
+  }
+  hipMemset(c->dflags, 0, 64);
+  *out = c;
+  return MDX_OK;
+}
+
+int mdx_destroy(mdx_ctx *c) {
+  if (!c) return MDX_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_atoms(c);
+  void *ptrs[] = {c->dflags, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->bin_count, c->bin_start,
+                  c->sort_tmp, c->scan_tmp};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->red_host) hipHostFree(c->red_host);
+  for (auto &ev : c->ev)
+    if (ev) hipEventDestroy(ev);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MDX_OK;
+}
+
+int mdx_set_box(mdx_ctx *c, const double lo[3], const double hi[3], const int periodic[3]) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!lo || !hi || !periodic) {
+    set_error("null box argument");
+    return MDX_ERR_ARG;
+  }
+  for (int d = 0; d < 3; d++) {
+    if (!(hi[d] > lo[d])) {
+      set_error("box hi must exceed lo");
+      return MDX_ERR_ARG;
+    }
+    c->lo[d] = lo[d];
+    c->hi[d] = hi[d];
+    c->prd[d] = hi[d] - lo[d];
+    c->periodic[d] = periodic[d] ? 1 : 0;
+  }
+  c->box_set = true;
+  c->built = false;
+  c->run_setup_done = false;
+  return MDX_OK;
+}
+
+int mdx_set_atoms(mdx_ctx *c, int n, int ntypes, const double *x, const double *v, const int *type, const int *tag,
+                  const double *mass) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n < 0 || n > MDX_IDX_MASK || ntypes < 1 || ntypes > MDX_MAXT || (n > 0 && (!x || !type)) || !mass) {
+    set_error("mdx_set_atoms: invalid arguments (n < 2^27, 1 <= ntypes <= 8, x/type/mass required)");
+    return MDX_ERR_ARG;
+  }
+  for (int t = 1; t <= ntypes; t++)
+    if (!(mass[t] > 0.0)) {
+      set_error("mass not set for all atom types");
+      return MDX_ERR_ARG;
+    }
+  for (int i = 0; i < n; i++)
+    if (type[i] < 1 || type[i] > ntypes) {
+      set_error("atom type out of range");
+      return MDX_ERR_ARG;
+    }
+  free_atoms(c);
+  c->n = n;
+  c->ntypes = ntypes;
+  for (int t = 0; t < MDX_NT1; t++) c->mass[t] = t <= ntypes ? mass[t] : 0.0;
+  const int npad = std::max(256, ((n + 255) / 256) * 256);
+  c->npad = npad;
+  c->cap_atoms = npad;
+  if ((rc = dev_alloc(&c->x, npad)) || (rc = dev_alloc(&c->xb, npad)) || (rc = dev_alloc(&c->v, npad)) ||
+      (rc = dev_alloc(&c->vb, npad)) || (rc = dev_alloc(&c->f, npad)) || (rc = dev_alloc(&c->xhold, npad)) ||
+      (rc = dev_alloc(&c->tag, npad)) || (rc = dev_alloc(&c->tagb, npad)) || (rc = dev_alloc(&c->orig, npad)) ||
+      (rc = dev_alloc(&c->origb, npad)) || (rc = dev_alloc(&c->image, npad)) ||
+      (rc = dev_alloc(&c->imageb, npad)) || (rc = dev_alloc(&c->inv, npad)) ||
+      (rc = dev_alloc(&c->vatom, (size_t)6 * npad)) || (rc = dev_alloc(&c->dscratch, (size_t)6 * npad)) ||
+      (rc = dev_alloc(&c->bkey, npad)) || (rc = dev_alloc(&c->bkey_s, npad)) || (rc = dev_alloc(&c->bidx, npad)) ||
+      (rc = dev_alloc(&c->bidx_s, npad)) || (rc = dev_alloc(&c->numneigh, npad)) ||
+      (rc = dev_alloc(&c->rho, npad)) || (rc = dev_alloc(&c->fp, npad)) || (rc = dev_alloc(&c->eemb, npad)))
+    return rc;
+  if (hipHostMalloc(&c->host_stage, sizeof(double) * 6 * npad, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->host_istage, sizeof(int) * 3 * npad, hipHostMallocDefault) != hipSuccess) {
+    set_error("pinned host allocation failed");
+    return MDX_ERR_NOMEM;
+  }
+  std::vector<double4> hx(n), hv(n);
+  std::vector<int> ht(n), ho(n), him(n);
+  const int img0 = (MDX_IMGMAX << 20) | (MDX_IMGMAX << 10) | MDX_IMGMAX;
+  for (int i = 0; i < n; i++) {
+    hx[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], (double)type[i]);
+    hv[i] = v ? make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
+    ht[i] = tag ? tag[i] : i + 1;
+    ho[i] = i;
+    him[i] = img0;
+  }
+  hipError_t e;
+  if (n > 0) {
+    if ((e = hipMemcpy(c->x, hx.data(), sizeof(double4) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->v, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->tag, ht.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->orig, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->inv, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->image, him.data(), sizeof(int) * n, hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "atom upload");
+  }
+  if ((e = hipMemset(c->f, 0, sizeof(double4) * npad)) != hipSuccess) return hip_fail(e, "clear forces");
+  c->built = false;
+  c->run_setup_done = false;
+  return MDX_OK;
+}
+
+int mdx_set_positions(mdx_ctx *c, const double *x) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!x && c->n > 0) {
+    set_error("null positions");
+    return MDX_ERR_ARG;
+  }
+  if (c->n == 0) return MDX_OK;
+  return scatter_positions(c, x);
+}
+
+int mdx_get_atoms(mdx_ctx *c, double *x, double *v, double *f, int *image) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->n == 0) return MDX_OK;
+  if (x && (rc = gather_vec3(c, c->x, x))) return rc;
+  if (v && (rc = gather_vec3(c, c->v, v))) return rc;
+  if (f && (rc = gather_vec3(c, c->f, f))) return rc;
+  if (image) {
+    std::vector<int> im(c->n), org(c->n);
+    hipError_t e;
+    if ((e = hipMemcpy(im.data(), c->image, sizeof(int) * c->n, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(org.data(), c->orig, sizeof(int) * c->n, hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(e, "image copy");
+    for (int i = 0; i < c->n; i++) {
+      const int o = org[i];
+      image[3 * o] = (im[i] & MDX_IMGMASK) - MDX_IMGMAX;
+      image[3 * o + 1] = ((im[i] >> 10) & MDX_IMGMASK) - MDX_IMGMAX;
+      image[3 * o + 2] = (im[i] >> 20) - MDX_IMGMAX;
+    }
+  }
+  return MDX_OK;
+}
+
+int mdx_pair_lj_init(mdx_ctx *c, int ntypes, const double *cutsq, const double *lj1, const double *lj2,
+                     const double *lj3, const double *lj4, const double *offset) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (ntypes < 1 || ntypes > MDX_MAXT || !cutsq || !lj1 || !lj2 || !lj3 || !lj4 || !offset) {
+    set_error("mdx_pair_lj_init: invalid arguments");
+    return MDX_ERR_ARG;
+  }
+  if (c->ntypes && ntypes != c->ntypes) {
+    set_error("mdx_pair_lj_init: ntypes differs from the atoms' ntypes");
+    return MDX_ERR_ARG;
+  }
+  const int nt1 = ntypes + 1;
+  LJTab &t = c->lj_host;
+  memset(&t, 0, sizeof(t));
+  for (int q = 0; q < nt1 * nt1; q++) {
+    t.cutsq[q] = cutsq[q];
+    t.lj1[q] = lj1[q];
+    t.lj2[q] = lj2[q];
+    t.lj3[q] = lj3[q];
+    t.lj4[q] = lj4[q];
+    t.offset[q] = offset[q];
+    c->cutsq[q] = cutsq[q];
+  }
+  hipError_t e = hipMemcpy(c->lj_dev, &t, sizeof(LJTab), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "lj table upload");
+  c->pair_style = MDX_PAIR_LJ;
+  c->built = false;
+  c->run_setup_done = false;
+  return MDX_OK;
+}
+
+int mdx_pair_eam_init(mdx_ctx *c, int ntypes, int nrho, double drho, int nr, double dr, double rhomax,
+                      double cutforcesq, int nfrho, const double *frho_spline, int nrhor, const double *rhor_spline,
+                      int nz2r, const double *z2r_spline, const int *type2frho, const int *type2rhor,
+                      const int *type2z2r) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (ntypes < 1 || ntypes > MDX_MAXT || nrho < 2 || nr < 2 || !(drho > 0) || !(dr > 0) || nfrho < 1 ||
+      nrhor < 1 || nz2r < 1 || !frho_spline || !rhor_spline || !z2r_spline || !type2frho || !type2rhor ||
+      !type2z2r || !(cutforcesq > 0)) {
+    set_error("mdx_pair_eam_init: invalid arguments");
+    return MDX_ERR_ARG;
+  }
+  if (c->ntypes && ntypes != c->ntypes) {
+    set_error("mdx_pair_eam_init: ntypes differs from the atoms' ntypes");
+    return MDX_ERR_ARG;
+  }
+  const int nt1 = ntypes + 1;
+  for (int i = 1; i <= ntypes; i++) {
+    if (type2frho[i] < 0 || type2frho[i] >= nfrho) {
+      set_error("type2frho out of range");
+      return MDX_ERR_ARG;
+    }
+    for (int j = 1; j <= ntypes; j++)
+      if (type2rhor[i * nt1 + j] < 0 || type2rhor[i * nt1 + j] >= nrhor || type2z2r[i * nt1 + j] < 0 ||
+          type2z2r[i * nt1 + j] >= nz2r) {
+        set_error("type2rhor/type2z2r out of range");
+        return MDX_ERR_ARG;
+      }
+  }
+  const size_t nf = (size_t)nfrho * (nrho + 1) * 7, nrr = (size_t)nrhor * (nr + 1) * 7,
+               nz = (size_t)nz2r * (nr + 1) * 7;
+  std::vector<double> all(nf + nrr + nz);
+  memcpy(all.data(), frho_spline, sizeof(double) * nf);
+  memcpy(all.data() + nf, rhor_spline, sizeof(double) * nrr);
+  memcpy(all.data() + nf + nrr, z2r_spline, sizeof(double) * nz);
+  std::vector<float> allf(all.size());
+  for (size_t k = 0; k < all.size(); k++) allf[k] = (float)all[k];
+  if ((rc = dev_alloc(&c->eam_tab, all.size())) || (rc = dev_alloc(&c->eam_tab_f, all.size()))) return rc;
+  hipError_t e;
+  if ((e = hipMemcpy(c->eam_tab, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(c->eam_tab_f, allf.data(), sizeof(float) * allf.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "eam table upload");
+  EAMArgs &a = c->eam;
+  memset(&a, 0, sizeof(a));
+  a.nrho = nrho;
+  a.nr = nr;
+  a.nfrho = nfrho;
+  a.nrhor = nrhor;
+  a.nz2r = nz2r;
+  a.ntypes1 = nt1;
+  a.rdr = 1.0 / dr;
+  a.rdrho = 1.0 / drho;
+  a.rhomax = rhomax;
+  a.cutforcesq = cutforcesq;
+  for (int i = 0; i < nt1; i++) {
+    a.type2frho[i] = i >= 1 ? type2frho[i] : 0;
+    for (int j = 0; j < nt1; j++) {
+      a.type2rhor[i * nt1 + j] = (i >= 1 && j >= 1) ? type2rhor[i * nt1 + j] : 0;
+      a.type2z2r[i * nt1 + j] = (i >= 1 && j >= 1) ? type2z2r[i * nt1 + j] : 0;
+    }
+  }
+  a.frho = c->eam_tab;
+  a.rhor = c->eam_tab + nf;
+  a.z2r = c->eam_tab + nf + nrr;
+  a.frho_f = c->eam_tab_f;
+  a.rhor_f = c->eam_tab_f + nf;
+  a.z2r_f = c->eam_tab_f + nf + nrr;
+  // PairEAM::init_one: one global cutoff for every type pair (pair_eam.cpp:411-427)
+  for (int q = 0; q < MDX_NT1 * MDX_NT1; q++) c->cutsq[q] = 0.0;
+  for (int i = 1; i <= ntypes; i++)
+    for (int j = 1; j <= ntypes; j++) c->cutsq[i * nt1 + j] = cutforcesq;
+  c->pair_style = MDX_PAIR_EAM;
+  c->built = false;
+  c->run_setup_done = false;
+  return MDX_OK;
+}
+
+int mdx_set_precision(mdx_ctx *c, int precision) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (precision != MDX_FP64 && precision != MDX_MIXED) {
+    set_error("precision must be MDX_FP64 or MDX_MIXED");
+    return MDX_ERR_ARG;
+  }
+  c->precision = precision;
+  return MDX_OK;
+}
+
+int mdx_neigh_modify(mdx_ctx *c, double skin, int every, int delay, int check, int one) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!(skin >= 0.0) || every <= 0 || delay < 0 || one <= 0) {
+    set_error("Illegal neigh_modify command");
+    return MDX_ERR_ARG;
+  }
+  c->skin = skin;
+  c->every = every;
+  c->delay = delay;
+  c->check = check ? 1 : 0;
+  c->one = one;
+  c->built = false;
+  return MDX_OK;
+}
+
+int mdx_neigh_build(mdx_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = prepare(c))) return rc;
+  return neigh_build(c);
+}
+
+int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->built) {
+    set_error("no neighbour list built");
+    return MDX_ERR_ARG;
+  }
+  const int n = c->n;
+  std::vector<int> num(n), tag(n);
+  std::vector<int> nb((size_t)c->maxneigh * c->npad);
+  hipError_t e;
+  if (n > 0 &&
+      ((e = hipMemcpy(num.data(), c->numneigh, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+       (e = hipMemcpy(tag.data(), c->tag, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+       (e = hipMemcpy(nb.data(), c->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost)) != hipSuccess))
+    return hip_fail(e, "list copy");
+  long cnt = 0;
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < num[i]; k++) {
+      const int j = nb[(size_t)k * c->npad + i] & MDX_IDX_MASK;
+      if (tag[i] < tag[j]) {
+        if (pairs && cnt < cap) {
+          pairs[2 * cnt] = tag[i];
+          pairs[2 * cnt + 1] = tag[j];
+        }
+        cnt++;
+      }
+    }
+  if (npairs) *npairs = cnt;
+  return MDX_OK;
+}
+
+int mdx_compute(mdx_ctx *c, int eflag, int vflag, double *f, double *eng_vdwl, double *virial, double *eatom,
+                double *vatom) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = ensure_built(c))) return rc;
+  const int ev = (eflag || vflag) ? 1 : 0;
+  if ((rc = pair_compute(c, ev))) return rc;
+  if (ev) {
+    c->eng_vdwl = c->red_host[0];
+    for (int k = 0; k < 6; k++) c->virial[k] = c->red_host[1 + k];
+    if (eng_vdwl) *eng_vdwl = c->eng_vdwl;
+    if (virial)
+      for (int k = 0; k < 6; k++) virial[k] = c->virial[k];
+  }
+  if (c->n == 0) return MDX_OK;
+  if (f && (rc = gather_vec3(c, c->f, f))) return rc;
+  if (eatom && ev && (rc = gather_w(c, c->f, eatom))) return rc;
+  if (vatom && ev && (rc = gather_soa(c, c->vatom, 6, vatom))) return rc;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "compute");
+}
+
+int mdx_eam_rho_fp(mdx_ctx *c, double *rho, double *fp) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->pair_style != MDX_PAIR_EAM) {
+    set_error("pair style is not eam");
+    return MDX_ERR_ARG;
+  }
+  if (c->n == 0) return MDX_OK;
+  if (rho && (rc = gather_soa(c, c->rho, 1, rho))) return rc;
+  if (fp && (rc = gather_soa(c, c->fp, 1, fp))) return rc;
+  return MDX_OK;
+}
+
+static int thermo_out(mdx_ctx *c, mdx_thermo_fn cb, void *user) {
+  // energies/virial of this step's force call are in c->eng_vdwl/virial
+  int rc = reduce_ke(c);
+  if (rc) return rc;
+  double row[8];
+  row[0] = c->red_host[0];
+  row[1] = c->eng_vdwl;
+  for (int k = 0; k < 6; k++) row[2 + k] = c->virial[k];
+  if (cb) cb(user, c->ntimestep, row);
+  return MDX_OK;
+}
+
+static int force_step(mdx_ctx *c, int ev) {
+  int rc = pair_compute(c, ev);
+  if (rc) return rc;
+  if (ev) {
+    c->eng_vdwl = c->red_host[0];
+    for (int k = 0; k < 6; k++) c->virial[k] = c->red_host[1 + k];
+  }
+  return MDX_OK;
+}
+
+int mdx_setup(mdx_ctx *c, mdx_thermo_fn cb, void *user) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = prepare(c))) return rc;
+  if ((rc = neigh_build(c))) return rc;
+  c->ncalls = 0;
+  c->ndanger = 0;
+  if ((rc = force_step(c, 1))) return rc;
+  c->run_setup_done = true;
+  return thermo_out(c, cb, user);
+}
+
+int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, mdx_thermo_fn cb, void *user) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (nsteps < 0 || !(dt > 0.0)) {
+    set_error("mdx_run: nsteps >= 0 and dt > 0 required");
+    return MDX_ERR_ARG;
+  }
+  if (!c->run_setup_done && (rc = mdx_setup(c, cb, user))) return rc;
+  const double dtv = dt, dtf = 0.5 * dt * ftm2v;
+  const long last = c->ntimestep + nsteps;
+  for (int s = 0; s < nsteps; s++) {
+    const long step = ++c->ntimestep;
+    const bool out = step == last || (thermo_every > 0 && step % thermo_every == 0);
+    // Neighbor::decide: ago++ then test (neighbor.cpp:1294-1309)
+    const int ago_next = c->ago + 1;
+    const bool decide_step = ago_next >= c->delay && ago_next % c->every == 0;
+    const int check_now = decide_step && c->check;
+    if (c->profile) hipEventRecord(c->ev[4], c->stream);
+    if ((rc = nve_initial(c, dtv, dtf, check_now))) return rc;
+    c->ago = ago_next;
+    bool rebuild = false;
+    if (decide_step) {
+      if (!c->check) rebuild = true;
+      else {
+        int flag = 0;
+        if ((rc = neigh_check_displacement(c, &flag))) return rc;
+        rebuild = flag != 0;
+        if (rebuild && c->ago == std::max(c->every, c->delay)) c->ndanger++;
+      }
+    }
+    if (rebuild) {
+      if ((rc = neigh_build(c))) return rc;
+      c->ncalls++;
+    }
+    if ((rc = force_step(c, out ? 1 : 0))) return rc;
+    if ((rc = nve_final(c, dtf))) return rc;
+    if (c->profile) {
+      hipEventRecord(c->ev[5], c->stream);
+      hipEventSynchronize(c->ev[5]);
+      float ms = 0;
+      hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+      c->ms_integrate += ms;
+    }
+    if (out && (rc = thermo_out(c, cb, user))) return rc;
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "run");
+}
+
+int mdx_get_stats(mdx_ctx *c, mdx_stats *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) {
+    set_error("null stats pointer");
+    return MDX_ERR_ARG;
+  }
+  out->ncalls = c->ncalls;
+  out->ndanger = c->ndanger;
+  out->total_neighbors = c->total_neighbors;
+  out->max_neighbors = c->max_neighbors;
+  out->capacity = c->maxneigh;
+  for (int d = 0; d < 3; d++) out->nbins[d] = c->na.nbin[d];
+  out->ms_neigh = c->ms_neigh;
+  out->ms_pair = c->ms_pair;
+  out->ms_integrate = c->ms_integrate;
+  return MDX_OK;
+}
+
+int mdx_set_launch(mdx_ctx *c, int block, int tpa) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (block != 0 && (block < 64 || block > 256 || block % 64 != 0)) {
+    set_error("block must be a multiple of 64 in [64, 256]");
+    return MDX_ERR_ARG;
+  }
+  if (tpa != 0 && tpa != 1) {
+    set_error("threads per atom must be 1 in this build");
+    return MDX_ERR_ARG;
+  }
+  c->launch.block = block ? block : 256;
+  c->launch.tpa = tpa ? tpa : 1;
+  return MDX_OK;
+}
+
+int mdx_set_profiling(mdx_ctx *c, int on) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  c->profile = on != 0;
+  c->ms_neigh = c->ms_pair = c->ms_integrate = 0.0;
+  return MDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,246 @@
+// integrate.hip - fix nve, the rebuild trigger, reductions and host<->device
+// reordering for gfx950.
+//
+// FixNVE::initial_integrate / final_integrate (src/fix_nve.cpp:65-142) run
+// one thread per atom on the FP64 x, v, f; contraction is disabled so the
+// updates round exactly as the reference's  v += dtfm*f;  x += dtv*v.
+// On steps where Neighbor::decide would call check_distance
+// (src/neighbor.cpp:1294-1372) the initial-integrate kernel also tests
+// |x - xhold|^2 > (skin/2)^2 and raises a device flag, so the trigger costs
+// no extra pass over x.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+struct MassTab {
+  double m[MDX_NT1];
+};
+
+__global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double dtv, double dtf,
+                                                    double4 *__restrict__ x, double4 *__restrict__ v,
+                                                    const double4 *__restrict__ f,
+                                                    const double4 *__restrict__ xhold, int check,
+                                                    double triggersq, int *__restrict__ dflags) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = x[i];
+  double4 q = v[i];
+  const double4 fi = f[i];
+  const double dtfm = dtf / mt.m[static_cast<int>(p.w)];
+  q.x += dtfm * fi.x;
+  q.y += dtfm * fi.y;
+  q.z += dtfm * fi.z;
+  p.x += dtv * q.x;
+  p.y += dtv * q.y;
+  p.z += dtv * q.z;
+  x[i] = p;
+  v[i] = q;
+  if (check) {
+    const double4 h = xhold[i];
+    const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    if (rsq > triggersq) atomicOr(&dflags[1], 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nve_final(int n, MassTab mt, double dtf, const double4 *__restrict__ x,
+                                                  double4 *__restrict__ v, const double4 *__restrict__ f) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 q = v[i];
+  const double4 fi = f[i];
+  const double dtfm = dtf / mt.m[static_cast<int>(x[i].w)];
+  q.x += dtfm * fi.x;
+  q.y += dtfm * fi.y;
+  q.z += dtfm * fi.z;
+  v[i] = q;
+}
+
+// sum_i m_i |v_i|^2 (ComputeTemp::compute_scalar, compute_temp.cpp:71-95)
+__global__ __launch_bounds__(256) void k_ke(int n, MassTab mt, const double4 *__restrict__ x,
+                                           const double4 *__restrict__ v, double *__restrict__ partials) {
+  __shared__ double s[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double t = 0.0;
+  if (i < n) {
+    const double4 q = v[i];
+    t = (q.x * q.x + q.y * q.y + q.z * q.z) * mt.m[static_cast<int>(x[i].w)];
+  }
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; q++) acc += s[q];
+    partials[blockIdx.x] = acc;
+  }
+}
+
+// fixed-order final reduction of per-block partials [nblocks][nslots]
+__global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__ partials, int nblocks, int nslots,
+                                                     double *__restrict__ out) {
+  __shared__ double s[256];
+  for (int slot = 0; slot < nslots; slot++) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) acc += partials[(size_t)b * nslots + slot];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[slot] = s[0];
+    __syncthreads();
+  }
+}
+
+// caller-order host positions -> device (sorted) order, type kept in w
+__global__ void k_scatter_x(int n, const double *__restrict__ hx, const int *__restrict__ orig,
+                            double4 *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = orig[i];
+  double4 p = x[i];
+  p.x = hx[3 * o];
+  p.y = hx[3 * o + 1];
+  p.z = hx[3 * o + 2];
+  x[i] = p;
+}
+
+// device order -> caller order [n][3]
+__global__ void k_gather3(int n, const int *__restrict__ orig, const double4 *__restrict__ src,
+                          double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = orig[i];
+  const double4 p = src[i];
+  out[3 * o] = p.x;
+  out[3 * o + 1] = p.y;
+  out[3 * o + 2] = p.z;
+}
+
+// device order -> caller order for the w component (per-atom energy)
+__global__ void k_gather_w(int n, const int *__restrict__ orig, const double4 *__restrict__ src,
+                           double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[orig[i]] = src[i].w;
+}
+
+// device order SoA [ncols][npad] -> caller order [n][ncols]
+__global__ void k_gather_soa(int n, int npad, int ncols, const int *__restrict__ orig, const double *__restrict__ src,
+                             double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = orig[i];
+  for (int c = 0; c < ncols; c++) out[(size_t)o * ncols + c] = src[(size_t)c * npad + i];
+}
+
+MassTab mass_tab(const mdx_ctx *c) {
+  MassTab t;
+  for (int k = 0; k < MDX_NT1; k++) t.m[k] = c->mass[k];
+  return t;
+}
+
+}  // namespace
+
+int nve_initial(mdx_ctx *c, double dtv, double dtf, int check) {
+  if (c->n == 0) return MDX_OK;
+  const int B = 256, G = (c->n + B - 1) / B;
+  if (check) {
+    hipError_t e = hipMemsetAsync(c->dflags + 1, 0, sizeof(int), c->stream);
+    if (e != hipSuccess) return hip_fail(e, "clear trigger flag");
+  }
+  hipLaunchKernelGGL(k_nve_initial, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtv, dtf, c->x, c->v, c->f,
+                     c->xhold, check, c->triggersq, c->dflags);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "nve initial");
+}
+
+int nve_final(mdx_ctx *c, double dtf) {
+  if (c->n == 0) return MDX_OK;
+  const int B = 256, G = (c->n + B - 1) / B;
+  hipLaunchKernelGGL(k_nve_final, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtf, c->x, c->v, c->f);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "nve final");
+}
+
+int neigh_check_displacement(mdx_ctx *c, int *flag) {
+  hipError_t e = hipMemcpyAsync(c->red_host + MDX_RED_SLOTS - 1, c->dflags + 1, sizeof(int),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e != hipSuccess) return hip_fail(e, "read trigger flag");
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "trigger sync");
+  int v;
+  memcpy(&v, c->red_host + MDX_RED_SLOTS - 1, sizeof(int));
+  *flag = v;
+  return MDX_OK;
+}
+
+int finish_reduction(mdx_ctx *c, int nblocks, int nslots) {
+  hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, nblocks, nslots, c->red_out);
+  hipError_t e = hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * nslots, hipMemcpyDeviceToHost, c->stream);
+  if (e != hipSuccess) return hip_fail(e, "reduction copy");
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "reduction sync");
+  return MDX_OK;
+}
+
+int reduce_ke(mdx_ctx *c) {
+  if (c->n == 0) {
+    c->red_host[0] = 0.0;
+    return MDX_OK;
+  }
+  const int B = 256, G = (c->n + B - 1) / B;
+  hipLaunchKernelGGL(k_ke, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), c->x, c->v, c->partials);
+  return finish_reduction(c, G, 1);
+}
+
+int scatter_positions(mdx_ctx *c, const double *host_x) {
+  const int B = 256, G = (c->n + B - 1) / B;
+  double *stage = c->host_stage;
+  memcpy(stage, host_x, sizeof(double) * 3 * c->n);
+  double *dtmp = c->dscratch;
+  hipError_t e = hipMemcpyAsync(dtmp, stage, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_fail(e, "positions upload");
+  hipLaunchKernelGGL(k_scatter_x, dim3(G), dim3(B), 0, c->stream, c->n, dtmp, c->orig, c->x);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "positions scatter");
+  return MDX_OK;
+}
+
+int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out) {
+  const int B = 256, G = (c->n + B - 1) / B;
+  hipLaunchKernelGGL(k_gather3, dim3(G), dim3(B), 0, c->stream, c->n, c->orig, dev, c->dscratch);
+  hipError_t e = hipMemcpyAsync(c->host_stage, c->dscratch, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost,
+                                c->stream);
+  if (e != hipSuccess) return hip_fail(e, "gather copy");
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "gather sync");
+  memcpy(host_out, c->host_stage, sizeof(double) * 3 * c->n);
+  return MDX_OK;
+}
+
+static int copy_back(mdx_ctx *c, size_t count, double *host_out) {
+  hipError_t e = hipMemcpyAsync(c->host_stage, c->dscratch, sizeof(double) * count, hipMemcpyDeviceToHost,
+                                c->stream);
+  if (e != hipSuccess) return hip_fail(e, "gather copy");
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "gather sync");
+  memcpy(host_out, c->host_stage, sizeof(double) * count);
+  return MDX_OK;
+}
+
+int gather_w(mdx_ctx *c, const double4 *dev, double *host_out) {
+  const int B = 256, G = (c->n + B - 1) / B;
+  hipLaunchKernelGGL(k_gather_w, dim3(G), dim3(B), 0, c->stream, c->n, c->orig, dev, c->dscratch);
+  return copy_back(c, (size_t)c->n, host_out);
+}
+
+int gather_soa(mdx_ctx *c, const double *dev, int ncols, double *host_out) {
+  const int B = 256, G = (c->n + B - 1) / B;
+  hipLaunchKernelGGL(k_gather_soa, dim3(G), dim3(B), 0, c->stream, c->n, c->npad, ncols, c->orig, dev,
+                     c->dscratch);
+  return copy_back(c, (size_t)c->n * ncols, host_out);
+}
+
+}  // namespace mdx

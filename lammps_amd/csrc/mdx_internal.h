@@ -1,0 +1,166 @@
+// mdx_internal.h - device data layout and kernel launchers shared by the
+// HIP sources of libmdx (gfx950 only).
+//
+// Device layout (one context, atoms kept in bin-sorted order, re-sorted at
+// every neighbour build):
+//   x      double4[npad]  x, y, z, w = (double)type   -- 32 B gathers
+//   v      double4[npad]  vx, vy, vz, w unused
+//   f      double4[npad]  fx, fy, fz, w = per-atom energy of the last compute
+//   tag    int[npad]      atom ID;  orig int[npad]: caller index of each atom
+//   image  int[npad]      periodic images, 10 bits per dimension (IMGMAX 512)
+//   nbr    int[maxneigh][npad]  full list, column major (coalesced for
+//                                one-thread-per-atom reads); bits 0-26 hold
+//                                the neighbour index, bits 27-31 a periodic
+//                                image code (sx+1) + 3(sy+1) + 9(sz+1)
+//   vatom  double[6][npad] per-atom virial (SoA)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <utility>
+
+#include "../../include/mdx.h"
+
+#define MDX_MAXT 8
+#define MDX_NT1 (MDX_MAXT + 1)
+#define MDX_IDX_BITS 27
+#define MDX_IDX_MASK ((1 << MDX_IDX_BITS) - 1)
+#define MDX_NOSHIFT 13
+#define MDX_IMGMAX 512
+#define MDX_IMGMASK 1023
+#define MDX_RED_SLOTS 8  // reduction slots: e, v0..v5, ke
+
+namespace mdx {
+
+// Bin geometry of the reference's Neighbor::setup_bins (neighbor.cpp:1538-1702),
+// used only to decide which side of a periodic pair the reference would
+// have evaluated the distance from (matters only at exact cutoff ties).
+struct RefBins {
+  double bboxlo[3], bboxhi[3], bininv[3];
+  int nbin[3];
+};
+
+struct NeighArgs {
+  double lo[3], prd[3], bininv[3];
+  int nbin[3];
+  int S[3];
+  int periodic[3];
+  int n, npad, maxneigh, ntypes1;
+  double cutneighsq[MDX_NT1 * MDX_NT1];
+  RefBins ref;
+};
+
+struct LJTab {  // (ntypes+1)^2 row-major, as Pair::init_one fills them
+  double cutsq[MDX_NT1 * MDX_NT1], lj1[MDX_NT1 * MDX_NT1], lj2[MDX_NT1 * MDX_NT1],
+      lj3[MDX_NT1 * MDX_NT1], lj4[MDX_NT1 * MDX_NT1], offset[MDX_NT1 * MDX_NT1];
+};
+
+struct EAMArgs {
+  int nrho, nr, nfrho, nrhor, nz2r, ntypes1;
+  double rdr, rdrho, rhomax, cutforcesq;
+  int type2frho[MDX_NT1];
+  int type2rhor[MDX_NT1 * MDX_NT1];
+  int type2z2r[MDX_NT1 * MDX_NT1];
+  const double *frho, *rhor, *z2r;  // [k][n+1][7]
+  const float *frho_f, *rhor_f, *z2r_f;
+};
+
+struct Launch {
+  int block = 256;
+  int tpa = 1;
+};
+
+}  // namespace mdx
+
+struct mdx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // box
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, prd[3] = {0, 0, 0};
+  int periodic[3] = {1, 1, 1};
+  bool box_set = false;
+  // atoms
+  int n = 0, npad = 0, ntypes = 0, cap_atoms = 0;
+  double mass[MDX_NT1] = {0};
+  double4 *x = nullptr, *xb = nullptr, *v = nullptr, *vb = nullptr, *f = nullptr, *xhold = nullptr;
+  int *tag = nullptr, *tagb = nullptr, *orig = nullptr, *origb = nullptr, *image = nullptr, *imageb = nullptr;
+  int *inv = nullptr;  // caller index -> device index
+  double *vatom = nullptr;
+  double *dscratch = nullptr;    // device, 6*npad doubles (host<->device reordering)
+  double *host_stage = nullptr;  // pinned, 6*npad doubles
+  int *host_istage = nullptr;    // pinned, 3*npad ints
+  // bins / sort
+  int nbins = 0, cap_bins = 0;
+  mdx::NeighArgs na{};
+  unsigned *bkey = nullptr, *bkey_s = nullptr;
+  int *bidx = nullptr, *bidx_s = nullptr, *bin_count = nullptr, *bin_start = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  // neighbour settings and state
+  double skin = 0.3;
+  int every = 1, delay = 10, check = 1, one = 2000;
+  double triggersq = 0.0225, cutneighmax = 0.0;
+  int *nbr = nullptr, *numneigh = nullptr, *dflags = nullptr;  // dflags: [0] max count, [1] displaced, [2] lost
+  int maxneigh = 0;
+  long nbr_elems = 0;
+  bool built = false;
+  int ago = 0;
+  long ncalls = 0, ndanger = 0;
+  long ntimestep = 0;
+  bool run_setup_done = false;
+  // pair
+  int pair_style = MDX_PAIR_NONE;
+  int precision = MDX_FP64;
+  double cutforce = 0.0;
+  double cutsq[MDX_NT1 * MDX_NT1] = {0};
+  mdx::LJTab *lj_dev = nullptr;
+  mdx::LJTab lj_host{};
+  mdx::EAMArgs eam{};
+  double *eam_tab = nullptr;  // device doubles: frho | rhor | z2r
+  float *eam_tab_f = nullptr;
+  double *rho = nullptr, *fp = nullptr, *eemb = nullptr;
+  // reductions
+  double *partials = nullptr;
+  int cap_partials = 0;
+  double *red_out = nullptr;    // device, MDX_RED_SLOTS
+  double *red_host = nullptr;   // pinned, MDX_RED_SLOTS
+  // results of the last compute
+  double eng_vdwl = 0.0, virial[6] = {0, 0, 0, 0, 0, 0};
+  // launch shape and timing
+  mdx::Launch launch;
+  bool profile = false;
+  hipEvent_t ev[8] = {nullptr};
+  double ms_neigh = 0, ms_pair = 0, ms_integrate = 0;
+  long total_neighbors = 0;
+  int max_neighbors = 0;
+};
+
+namespace mdx {
+
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+// neighbour build (neigh.hip)
+int neigh_setup_grid(mdx_ctx *c);
+int neigh_build(mdx_ctx *c);
+int neigh_check_displacement(mdx_ctx *c, int *flag);
+
+// pair kernels (pair.hip); eflag/vflag as Pair::compute.  Writes f (device,
+// sorted order), per-atom energy in f.w, per-atom virial in vatom, and the
+// global energy/virial into c->red_host (after a sync) when ev is set.
+int pair_compute(mdx_ctx *c, int ev);
+
+// integrator (integrate.hip)
+int nve_initial(mdx_ctx *c, double dtv, double dtf, int check);
+int nve_final(mdx_ctx *c, double dtf);
+int reduce_ke(mdx_ctx *c);
+int finish_reduction(mdx_ctx *c, int nblocks, int nslots);
+int scatter_positions(mdx_ctx *c, const double *host_x);
+int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out);
+
+}  // namespace mdx

@@ -1,0 +1,387 @@
+// neigh.hip - device neighbour-list build for gfx950.
+//
+// Replaces Neighbor::build -> bin_atoms -> half_bin_newton
+// (src/neighbor.cpp:1380-1461,1868-1983; src/neigh_half_bin.cpp:285-419)
+// and the periodic ghost shell of CommBrick::borders (src/comm_brick.cpp:696-864).
+//
+// Design: no ghost atoms.  At each build the owned atoms are wrapped into the
+// box (Domain::pbc, src/domain.cpp:478), binned on the reference's own bin
+// grid (binsize = cutneigh/2, src/neighbor.cpp:1580-1620), and stably
+// radix-sorted by bin so that every x-row of bins is one contiguous atom
+// range.  One thread per atom then scans the 5x5 rows of its stencil as at
+// most 2 contiguous segments each and keeps j iff rsq <= cutneighsq, with rsq
+// evaluated in FP64 without contraction on the periodic image x_j + s*L, the
+// value the reference computes for a ghost atom (atom_vec_atomic.cpp:290-297).
+// The result is a FULL list (each pair at both atoms): forces become a
+// gather, deterministic and atomic-free.  The stored image code lets the
+// pair kernels rebuild the same image between builds.
+#include <hipcub/hipcub.hpp>
+
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+// closest-corner stencil extent, as Neighbor::setup_bins sx/sy/sz
+int stencil_extent(double cut, double binsize, double bininv) {
+  int s = static_cast<int>(cut * bininv);
+  if (s * binsize < cut) s++;
+  return s;
+}
+
+__device__ __forceinline__ int ref_coord2bin(double c, double lo, double hi, double bininv, int nbin) {
+  // src/neighbor.cpp:1918-1945, one dimension
+  int ib;
+  if (c >= hi)
+    ib = static_cast<int>((c - hi) * bininv) + nbin;
+  else if (c >= lo) {
+    ib = static_cast<int>((c - lo) * bininv);
+    ib = min(ib, nbin - 1);
+  } else
+    ib = static_cast<int>((c - lo) * bininv) - 1;
+  return ib;
+}
+
+// True when the reference's half_bin_newton would evaluate the pair from the
+// owned atom at xi with the ghost image xg: its bin offset lies in the upper
+// half stencil (src/neigh_stencil.cpp:143-158).
+__device__ bool ref_owner_side(const RefBins &r, double xi0, double xi1, double xi2, double g0, double g1,
+                               double g2) {
+  int di = ref_coord2bin(g0, r.bboxlo[0], r.bboxhi[0], r.bininv[0], r.nbin[0]) -
+           ref_coord2bin(xi0, r.bboxlo[0], r.bboxhi[0], r.bininv[0], r.nbin[0]);
+  int dj = ref_coord2bin(g1, r.bboxlo[1], r.bboxhi[1], r.bininv[1], r.nbin[1]) -
+           ref_coord2bin(xi1, r.bboxlo[1], r.bboxhi[1], r.bininv[1], r.nbin[1]);
+  int dk = ref_coord2bin(g2, r.bboxlo[2], r.bboxhi[2], r.bininv[2], r.nbin[2]) -
+           ref_coord2bin(xi2, r.bboxlo[2], r.bboxhi[2], r.bininv[2], r.nbin[2]);
+  return dk > 0 || (dk == 0 && (dj > 0 || (dj == 0 && di > 0)));
+}
+
+// rsq of a - b without fused multiply-adds: the reference's x86-64 rounding
+__device__ __forceinline__ double rsq_exact(double a0, double a1, double a2, double b0, double b1, double b2) {
+#pragma clang fp contract(off)
+  double dx = a0 - b0;
+  double dy = a1 - b1;
+  double dz = a2 - b2;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Domain::pbc + bin index + histogram; one thread per atom
+__global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict__ image,
+                           unsigned *__restrict__ key, int *__restrict__ idx, int *__restrict__ count,
+                           int *__restrict__ dflags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double4 p = x[i];
+  double c[3] = {p.x, p.y, p.z};
+  int img = image[i];
+  int im[3] = {img & MDX_IMGMASK, (img >> 10) & MDX_IMGMASK, img >> 20};
+  int ib[3];
+  bool lost = false;
+  for (int d = 0; d < 3; d++) {
+    double lo = a.lo[d], hi = a.lo[d] + a.prd[d];
+    if (a.periodic[d]) {
+      if (c[d] < lo) {
+        c[d] += a.prd[d];
+        im[d] = (im[d] - 1) & MDX_IMGMASK;
+      }
+      if (c[d] >= hi) {
+        c[d] -= a.prd[d];
+        c[d] = fmax(c[d], lo);
+        im[d] = (im[d] + 1) & MDX_IMGMASK;
+      }
+    } else if (c[d] < lo || c[d] >= hi) {
+      lost = true;
+    }
+    int b = static_cast<int>((c[d] - lo) * a.bininv[d]);
+    b = min(max(b, 0), a.nbin[d] - 1);
+    ib[d] = b;
+  }
+  if (lost) atomicOr(&dflags[4], 1);
+  p.x = c[0];
+  p.y = c[1];
+  p.z = c[2];
+  x[i] = p;
+  image[i] = (im[2] << 20) | (im[1] << 10) | im[0];
+  unsigned b = (unsigned)((ib[2] * a.nbin[1] + ib[1]) * a.nbin[0] + ib[0]);
+  key[i] = b;
+  idx[i] = i;
+  atomicAdd(&count[b], 1);
+}
+
+__global__ void k_permute(int n, const int *__restrict__ src, const double4 *__restrict__ x,
+                          const double4 *__restrict__ v, const int *__restrict__ tag,
+                          const int *__restrict__ orig, const int *__restrict__ image, double4 *__restrict__ xo,
+                          double4 *__restrict__ vo, int *__restrict__ tago, int *__restrict__ origo,
+                          int *__restrict__ imageo, int *__restrict__ inv, double4 *__restrict__ xhold) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int o = src[k];
+  double4 p = x[o];
+  xo[k] = p;
+  xhold[k] = p;
+  vo[k] = v[o];
+  tago[k] = tag[o];
+  int oi = orig[o];
+  origo[k] = oi;
+  imageo[k] = image[o];
+  inv[oi] = k;
+}
+
+// Full list, one thread per atom i (atoms are bin sorted, so the lanes of a
+// wavefront walk the same rows and their x_j gathers hit the same lines).
+__global__ __launch_bounds__(256) void k_build_full(NeighArgs a, const double4 *__restrict__ x,
+                                                   const unsigned *__restrict__ bkey,
+                                                   const int *__restrict__ bin_start, int *__restrict__ nbr,
+                                                   int *__restrict__ numneigh, int *__restrict__ dflags,
+                                                   unsigned long long *__restrict__ nsum) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const double4 xi = x[i];
+  const int ti = static_cast<int>(xi.w);
+  const unsigned b = bkey[i];
+  const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
+  const int ix = b % nbx, iy = (b / nbx) % nby, iz = b / (nbx * nby);
+  int cnt = 0;
+  for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
+    int jz = iz + dz, sz = 0;
+    if (jz < 0) {
+      if (!a.periodic[2]) continue;
+      jz += nbz;
+      sz = -1;
+    } else if (jz >= nbz) {
+      if (!a.periodic[2]) continue;
+      jz -= nbz;
+      sz = 1;
+    }
+    for (int dy = -a.S[1]; dy <= a.S[1]; dy++) {
+      int jy = iy + dy, sy = 0;
+      if (jy < 0) {
+        if (!a.periodic[1]) continue;
+        jy += nby;
+        sy = -1;
+      } else if (jy >= nby) {
+        if (!a.periodic[1]) continue;
+        jy -= nby;
+        sy = 1;
+      }
+      const int rowbase = (jz * nby + jy) * nbx;
+      int x0 = ix - a.S[0], x1 = ix + a.S[0];
+      // up to three segments: wrapped low part, main part, wrapped high part
+      int seg_lo[3], seg_hi[3], seg_s[3], nseg = 0;
+      if (x0 < 0 && a.periodic[0]) {
+        seg_lo[nseg] = max(x0 + nbx, 0);
+        seg_hi[nseg] = nbx - 1;
+        seg_s[nseg++] = -1;
+      }
+      seg_lo[nseg] = max(x0, 0);
+      seg_hi[nseg] = min(x1, nbx - 1);
+      seg_s[nseg++] = 0;
+      if (x1 >= nbx && a.periodic[0]) {
+        seg_lo[nseg] = 0;
+        seg_hi[nseg] = min(x1 - nbx, nbx - 1);
+        seg_s[nseg++] = 1;
+      }
+      const double shy = sy * a.prd[1], shz = sz * a.prd[2];
+      for (int s = 0; s < nseg; s++) {
+        if (seg_lo[s] > seg_hi[s]) continue;
+        const int sx = seg_s[s];
+        const double shx = sx * a.prd[0];
+        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        const int j0 = bin_start[rowbase + seg_lo[s]];
+        const int j1 = bin_start[rowbase + seg_hi[s] + 1];
+        for (int j = j0; j < j1; j++) {
+          if (code == MDX_NOSHIFT && j == i) continue;
+          const double4 xj = x[j];
+          const int tj = static_cast<int>(xj.w);
+          // ghost image coordinate as CommBrick::borders forms it
+          const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
+          const double c = a.cutneighsq[ti * a.ntypes1 + tj];
+          const double rsq = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2);
+          bool keep = rsq <= c;
+          if (code != MDX_NOSHIFT) {
+            // the reference evaluates an owned-ghost pair once, from one side
+            const double rsq_b = rsq_exact(xj.x, xj.y, xj.z, xi.x - shx, xi.y - shy, xi.z - shz);
+            const bool keep_b = rsq_b <= c;
+            if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+          }
+          if (keep) {
+            if (cnt < a.maxneigh) nbr[(size_t)cnt * a.npad + i] = j | (code << MDX_IDX_BITS);
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  numneigh[i] = cnt;
+  atomicMax(&dflags[0], cnt);
+  atomicAdd(nsum, (unsigned long long)cnt);
+}
+
+}  // namespace
+
+int neigh_setup_grid(mdx_ctx *c) {
+  NeighArgs &a = c->na;
+  const double cut = c->cutneighmax;
+  for (int d = 0; d < 3; d++) {
+    if (c->periodic[d] && c->prd[d] < 2.0 * cut) {
+      set_error("periodic box length below 2x the neighbour cutoff (one periodic image per pair required)");
+      return MDX_ERR_ARG;
+    }
+  }
+  // Neighbor::setup_bins: binsize = 0.5*cutneighmax, nbin = floor(bbox/binsize)
+  double binsize_optimal = 0.5 * cut;
+  if (binsize_optimal == 0.0) binsize_optimal = c->prd[0];
+  const double binsizeinv = 1.0 / binsize_optimal;
+  for (int d = 0; d < 3; d++) {
+    int nb = static_cast<int>(c->prd[d] * binsizeinv);
+    if (nb == 0) nb = 1;
+    const double bs = c->prd[d] / nb;
+    a.lo[d] = c->lo[d];
+    a.prd[d] = c->prd[d];
+    a.nbin[d] = nb;
+    a.bininv[d] = 1.0 / bs;
+    a.S[d] = stencil_extent(cut, bs, a.bininv[d]);
+    a.periodic[d] = c->periodic[d];
+    a.ref.bboxlo[d] = c->lo[d];
+    a.ref.bboxhi[d] = c->hi[d];
+    a.ref.bininv[d] = a.bininv[d];
+    a.ref.nbin[d] = nb;
+  }
+  long nb = (long)a.nbin[0] * a.nbin[1] * a.nbin[2];
+  if (nb > (1L << 30)) {
+    set_error("too many neighbour bins");
+    return MDX_ERR_ARG;
+  }
+  c->nbins = (int)nb;
+  if (c->nbins + 1 > c->cap_bins) {
+    hipFree(c->bin_count);
+    hipFree(c->bin_start);
+    c->cap_bins = c->nbins + 1;
+    if (hipMalloc(&c->bin_count, sizeof(int) * c->cap_bins) != hipSuccess ||
+        hipMalloc(&c->bin_start, sizeof(int) * c->cap_bins) != hipSuccess) {
+      set_error("Insufficient memory on accelerator (bins)");
+      return MDX_ERR_NOMEM;
+    }
+    hipFree(c->scan_tmp);
+    c->scan_tmp = nullptr;
+    c->scan_tmp_bytes = 0;
+  }
+  a.ntypes1 = c->ntypes + 1;
+  return MDX_OK;
+}
+
+static int end_bit_for(int nbins) {
+  int bits = 1;
+  while ((1L << bits) < nbins) bits++;
+  return bits;
+}
+
+int neigh_build(mdx_ctx *c) {
+  NeighArgs &a = c->na;
+  a.n = c->n;
+  a.npad = c->npad;
+  c->built = false;  // a failed build must never leave a stale list in use
+  if (c->n == 0) {
+    c->built = true;
+    return MDX_OK;
+  }
+  if (c->profile) hipEventRecord(c->ev[0], c->stream);
+  const int B = 256;
+  const int G = (c->n + B - 1) / B;
+  hipError_t e;
+  if ((e = hipMemsetAsync(c->bin_count, 0, sizeof(int) * (c->nbins + 1), c->stream)) != hipSuccess)
+    return hip_fail(e, "memset bins");
+  if ((e = hipMemsetAsync(c->dflags, 0, 64, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
+  hipLaunchKernelGGL(k_wrap_bin, dim3(G), dim3(B), 0, c->stream, a, c->x, c->image, c->bkey, c->bidx,
+                     c->bin_count, c->dflags);
+  // stable radix sort of (bin, current index): deterministic order in a bin
+  const int eb = end_bit_for(c->nbins);
+  size_t need = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->bkey, c->bkey_s, c->bidx, c->bidx_s, c->n, 0, eb,
+                                     c->stream);
+  if (need > c->sort_tmp_bytes) {
+    hipFree(c->sort_tmp);
+    c->sort_tmp_bytes = need;
+    if (hipMalloc(&c->sort_tmp, need) != hipSuccess) {
+      set_error("Insufficient memory on accelerator (sort)");
+      return MDX_ERR_NOMEM;
+    }
+  }
+  e = hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, need, c->bkey, c->bkey_s, c->bidx, c->bidx_s, c->n, 0,
+                                         eb, c->stream);
+  if (e != hipSuccess) return hip_fail(e, "radix sort");
+  need = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
+  if (need > c->scan_tmp_bytes) {
+    hipFree(c->scan_tmp);
+    c->scan_tmp_bytes = need;
+    if (hipMalloc(&c->scan_tmp, need) != hipSuccess) {
+      set_error("Insufficient memory on accelerator (scan)");
+      return MDX_ERR_NOMEM;
+    }
+  }
+  e = hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
+  if (e != hipSuccess) return hip_fail(e, "bin scan");
+  hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx_s, c->x, c->v, c->tag, c->orig,
+                     c->image, c->xb, c->vb, c->tagb, c->origb, c->imageb, c->inv, c->xhold);
+  std::swap(c->x, c->xb);
+  std::swap(c->v, c->vb);
+  std::swap(c->tag, c->tagb);
+  std::swap(c->orig, c->origb);
+  std::swap(c->image, c->imageb);
+  // list build; regrow the per-atom capacity and rebuild on overflow
+  for (int attempt = 0; attempt < 4; attempt++) {
+    a.maxneigh = c->maxneigh;
+    if ((e = hipMemsetAsync(c->dflags, 0, sizeof(int), c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
+    if ((e = hipMemsetAsync(c->dflags + 8, 0, 8, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
+    hipLaunchKernelGGL(k_build_full, dim3(G), dim3(B), 0, c->stream, a, c->x, c->bkey_s, c->bin_start, c->nbr,
+                       c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
+    int flags[16];
+    if ((e = hipMemcpyAsync(flags, c->dflags, 64, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+      return hip_fail(e, "read build flags");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "neighbour build");
+    if (flags[4]) {
+      set_error("Lost atoms: an atom left a non-periodic box");
+      return MDX_ERR_ARG;
+    }
+    const int mx = flags[0];
+    if (mx > c->one) {
+      set_error("Neighbor list overflow, boost neigh_modify one");
+      return MDX_ERR_OVERFLOW;
+    }
+    unsigned long long tot;
+    memcpy(&tot, &flags[8], sizeof(tot));
+    c->total_neighbors = (long)tot;
+    c->max_neighbors = mx;
+    if (mx <= c->maxneigh) break;
+    // regrow: the accelerator library's max_nbors resize (lal_neighbor.cpp:582-595)
+    int cap = ((mx + mx / 4 + 15) / 16) * 16;
+    cap = std::min(cap, c->one);
+    hipFree(c->nbr);
+    c->nbr = nullptr;
+    size_t bytes = sizeof(int) * (size_t)cap * c->npad;
+    if (hipMalloc(&c->nbr, bytes) != hipSuccess) {
+      set_error("Insufficient memory on accelerator (neighbour list)");
+      return MDX_ERR_NOMEM;
+    }
+    c->maxneigh = cap;
+    c->nbr_elems = (long)cap * c->npad;
+  }
+  if (c->max_neighbors > c->maxneigh) {
+    set_error("neighbour list capacity could not be grown to fit");
+    return MDX_ERR_OVERFLOW;
+  }
+  if (c->profile) {
+    hipEventRecord(c->ev[1], c->stream);
+    hipEventSynchronize(c->ev[1]);
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->ms_neigh += ms;
+  }
+  c->built = true;
+  c->ago = 0;
+  return MDX_OK;
+}
+
+}  // namespace mdx

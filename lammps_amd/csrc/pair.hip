@@ -1,0 +1,384 @@
+// pair.hip - lj/cut and eam force kernels for gfx950.
+//
+// Replaces PairLJCut::compute (src/pair_lj_cut.cpp:68-141) and
+// PairEAM::compute (src/MANYBODY/pair_eam.cpp:125-306).  The reference walks
+// a half list and scatters f[j] -= ...; here every atom owns its row of a
+// full list and only gathers, so forces, energies and virials are written
+// once per atom with no atomics and a fixed summation order (deterministic).
+// Energy and virial per atom are half of each pair term (pair.cpp:862-901,
+// ev_tally_full), and the global sums come from a fixed-order block tree.
+//
+// Precision modes (template MIXED):
+//   FP64  : everything in double, as the reference.
+//   MIXED : positions, the image shift, del = x_i - x_j and rsq stay FP64
+//           (so the cutoff test and the geometry match the reference); the
+//           potential arithmetic (1/r^2, r^6, splines) is FP32; forces,
+//           energies and virials accumulate in FP64.
+//
+// EAM: the reference needs rho -> reverse_comm -> fp -> forward_comm ->
+// force (pair_eam.cpp:206,232).  With a full list rho_i is complete inside
+// one thread, so the fp pass is fused into the rho kernel and the only
+// grid-wide dependency left is fp[j] for the force kernel: two launches.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+constexpr int kNT2 = MDX_NT1 * MDX_NT1;
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *__restrict__ out) {
+  __shared__ double s[NV][16];
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double acc = 0.0;
+    for (int q = 0; q < nw; q++) acc += s[threadIdx.x][q];
+    out[(size_t)blockIdx.x * NV + threadIdx.x] = acc;
+  }
+}
+
+__device__ __forceinline__ void image_shift(int code, double px, double py, double pz, double &sx, double &sy,
+                                            double &sz) {
+  sx = ((code % 3) - 1) * px;
+  sy = (((code / 3) % 3) - 1) * py;
+  sz = ((code / 9) - 1) * pz;
+}
+
+// ------------------------------------------------------------------ LJ
+template <bool MIXED, bool EV>
+__global__ __launch_bounds__(256) void k_lj(int n, int npad, int nt1, double px, double py, double pz,
+                                           const double4 *__restrict__ x, const int *__restrict__ nbr,
+                                           const int *__restrict__ numneigh, const LJTab *__restrict__ tab,
+                                           double4 *__restrict__ f, double *__restrict__ vatom,
+                                           double *__restrict__ partials) {
+  __shared__ double s_cutsq[kNT2], s_lj1[kNT2], s_lj2[kNT2], s_lj3[kNT2], s_lj4[kNT2], s_off[kNT2];
+  __shared__ float f_lj1[kNT2], f_lj2[kNT2], f_lj3[kNT2], f_lj4[kNT2], f_off[kNT2];
+  const int nt2 = nt1 * nt1;
+  for (int q = threadIdx.x; q < nt2; q += blockDim.x) {
+    s_cutsq[q] = tab->cutsq[q];
+    s_lj1[q] = tab->lj1[q];
+    s_lj2[q] = tab->lj2[q];
+    s_lj3[q] = tab->lj3[q];
+    s_lj4[q] = tab->lj4[q];
+    s_off[q] = tab->offset[q];
+    f_lj1[q] = (float)tab->lj1[q];
+    f_lj2[q] = (float)tab->lj2[q];
+    f_lj3[q] = (float)tab->lj3[q];
+    f_lj4[q] = (float)tab->lj4[q];
+    f_off[q] = (float)tab->offset[q];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+  if (i < n) {
+    const double4 xi = x[i];
+    const int row = static_cast<int>(xi.w) * nt1;
+    const int jn = numneigh[i];
+    const int *nb = nbr + i;
+    for (int k = 0; k < jn; k++) {
+      const int jc = nb[(size_t)k * npad];
+      const int j = jc & MDX_IDX_MASK;
+      const int code = (unsigned)jc >> MDX_IDX_BITS;
+      const double4 xj = x[j];
+      double gx = xj.x, gy = xj.y, gz = xj.z;
+      if (code != MDX_NOSHIFT) {
+        double sx, sy, sz;
+        image_shift(code, px, py, pz, sx, sy, sz);
+        gx += sx;
+        gy += sy;
+        gz += sz;
+      }
+      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      const int t = row + static_cast<int>(xj.w);
+      if (rsq < s_cutsq[t]) {
+        double fpair, evdwl = 0.0;
+        if (MIXED) {
+          const float r2inv = 1.0f / (float)rsq;
+          const float r6inv = r2inv * r2inv * r2inv;
+          const float forcelj = r6inv * (f_lj1[t] * r6inv - f_lj2[t]);
+          fpair = (double)(forcelj * r2inv);
+          if (EV) evdwl = (double)(r6inv * (f_lj3[t] * r6inv - f_lj4[t]) - f_off[t]);
+        } else {
+          const double r2inv = 1.0 / rsq;
+          const double r6inv = r2inv * r2inv * r2inv;
+          const double forcelj = r6inv * (s_lj1[t] * r6inv - s_lj2[t]);
+          fpair = forcelj * r2inv;
+          if (EV) evdwl = r6inv * (s_lj3[t] * r6inv - s_lj4[t]) - s_off[t];
+        }
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        if (EV) {
+          e += 0.5 * evdwl;
+          const double hf = 0.5 * fpair;
+          w0 += delx * delx * hf;
+          w1 += dely * dely * hf;
+          w2 += delz * delz * hf;
+          w3 += delx * dely * hf;
+          w4 += delx * delz * hf;
+          w5 += dely * delz * hf;
+        }
+      }
+    }
+    f[i] = make_double4(fx, fy, fz, e);
+    if (EV) {
+      vatom[i] = w0;
+      vatom[(size_t)npad + i] = w1;
+      vatom[(size_t)2 * npad + i] = w2;
+      vatom[(size_t)3 * npad + i] = w3;
+      vatom[(size_t)4 * npad + i] = w4;
+      vatom[(size_t)5 * npad + i] = w5;
+    }
+  }
+  if (EV) {
+    double v[7] = {e, w0, w1, w2, w3, w4, w5};
+    block_reduce_store<7>(v, partials);
+  }
+}
+
+// ------------------------------------------------------------------ EAM
+template <typename T>
+__device__ __forceinline__ void spline_index(T r, T rdr, int nr, int &m, T &p) {
+  // pair_eam.cpp:188-192: p = r*rdr + 1; m = min(int(p), nr-1); p = min(p - m, 1)
+  p = r * rdr + T(1);
+  m = static_cast<int>(p);
+  m = min(max(m, 1), nr - 1);  // m >= 1 for any r >= 0; the max only guards NaN
+  p -= m;
+  p = p < T(1) ? p : T(1);
+}
+
+template <bool MIXED, bool EV>
+__global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, double px, double py, double pz, EAMArgs a,
+                                                const double4 *__restrict__ x, const int *__restrict__ nbr,
+                                                const int *__restrict__ numneigh, double *__restrict__ rho_out,
+                                                double *__restrict__ fp_out, double *__restrict__ eemb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double4 xi = x[i];
+    const int ti = static_cast<int>(xi.w);
+    const int jn = numneigh[i];
+    const int *nb = nbr + i;
+    const int stride = (a.nr + 1) * 7;
+    double rho = 0.0;
+    for (int k = 0; k < jn; k++) {
+      const int jc = nb[(size_t)k * npad];
+      const int j = jc & MDX_IDX_MASK;
+      const int code = (unsigned)jc >> MDX_IDX_BITS;
+      const double4 xj = x[j];
+      double gx = xj.x, gy = xj.y, gz = xj.z;
+      if (code != MDX_NOSHIFT) {
+        double sx, sy, sz;
+        image_shift(code, px, py, pz, sx, sy, sz);
+        gx += sx;
+        gy += sy;
+        gz += sz;
+      }
+      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < a.cutforcesq) {
+        const int tj = static_cast<int>(xj.w);
+        const int tab = a.type2rhor[tj * a.ntypes1 + ti];
+        int m;
+        if (MIXED) {
+          float p;
+          spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
+          const float *c = a.rhor_f + (size_t)tab * stride + m * 7;
+          rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
+        } else {
+          double p;
+          spline_index<double>(sqrt(rsq), a.rdr, a.nr, m, p);
+          const double *c = a.rhor + (size_t)tab * stride + m * 7;
+          rho += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+        }
+      }
+    }
+    // fp pass (pair_eam.cpp:213-228), FP64 in both modes: one per atom
+    double p = rho * a.rdrho + 1.0;
+    int m = static_cast<int>(p);
+    m = max(1, min(m, a.nrho - 1));
+    p -= m;
+    p = fmin(p, 1.0);
+    const double *c = a.frho + ((size_t)a.type2frho[ti] * (a.nrho + 1) + m) * 7;
+    const double fpi = (c[0] * p + c[1]) * p + c[2];
+    rho_out[i] = rho;
+    fp_out[i] = fpi;
+    if (EV) {
+      double phi = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+      if (rho > a.rhomax) phi += fpi * (rho - a.rhomax);
+      eemb[i] = phi;
+    }
+  }
+}
+
+template <bool MIXED, bool EV>
+__global__ __launch_bounds__(256) void k_eam_force(int n, int npad, double px, double py, double pz, EAMArgs a,
+                                                  const double4 *__restrict__ x, const int *__restrict__ nbr,
+                                                  const int *__restrict__ numneigh, const double *__restrict__ fp,
+                                                  const double *__restrict__ eemb, double4 *__restrict__ f,
+                                                  double *__restrict__ vatom, double *__restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+  if (i < n) {
+    const double4 xi = x[i];
+    const int ti = static_cast<int>(xi.w);
+    const double fpi = fp[i];
+    const int jn = numneigh[i];
+    const int *nb = nbr + i;
+    const int stride = (a.nr + 1) * 7;
+    for (int k = 0; k < jn; k++) {
+      const int jc = nb[(size_t)k * npad];
+      const int j = jc & MDX_IDX_MASK;
+      const int code = (unsigned)jc >> MDX_IDX_BITS;
+      const double4 xj = x[j];
+      double gx = xj.x, gy = xj.y, gz = xj.z;
+      if (code != MDX_NOSHIFT) {
+        double sx, sy, sz;
+        image_shift(code, px, py, pz, sx, sy, sz);
+        gx += sx;
+        gy += sy;
+        gz += sz;
+      }
+      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < a.cutforcesq) {
+        const int tj = static_cast<int>(xj.w);
+        const int tij = ti * a.ntypes1 + tj, tji = tj * a.ntypes1 + ti;
+        double fpair, phi;
+        if (MIXED) {
+          const float r = sqrtf((float)rsq);
+          int m;
+          float p;
+          spline_index<float>(r, (float)a.rdr, a.nr, m, p);
+          const float *ci = a.rhor_f + (size_t)a.type2rhor[tij] * stride + m * 7;
+          const float *cj = a.rhor_f + (size_t)a.type2rhor[tji] * stride + m * 7;
+          const float *cz = a.z2r_f + (size_t)a.type2z2r[tij] * stride + m * 7;
+          const float rhoip = (ci[0] * p + ci[1]) * p + ci[2];
+          const float rhojp = (cj[0] * p + cj[1]) * p + cj[2];
+          const float z2p = (cz[0] * p + cz[1]) * p + cz[2];
+          const float z2 = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
+          const float recip = 1.0f / r;
+          const float phif = z2 * recip;
+          const float phip = z2p * recip - phif * recip;
+          const float psip = (float)fpi * rhojp + (float)fp[j] * rhoip + phip;
+          fpair = (double)(-psip * recip);
+          phi = (double)phif;
+        } else {
+          const double r = sqrt(rsq);
+          int m;
+          double p;
+          spline_index<double>(r, a.rdr, a.nr, m, p);
+          const double *ci = a.rhor + (size_t)a.type2rhor[tij] * stride + m * 7;
+          const double *cj = a.rhor + (size_t)a.type2rhor[tji] * stride + m * 7;
+          const double *cz = a.z2r + (size_t)a.type2z2r[tij] * stride + m * 7;
+          const double rhoip = (ci[0] * p + ci[1]) * p + ci[2];
+          const double rhojp = (cj[0] * p + cj[1]) * p + cj[2];
+          const double z2p = (cz[0] * p + cz[1]) * p + cz[2];
+          const double z2 = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
+          const double recip = 1.0 / r;
+          phi = z2 * recip;
+          const double phip = z2p * recip - phi * recip;
+          const double psip = fpi * rhojp + fp[j] * rhoip + phip;
+          fpair = -psip * recip;
+        }
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        if (EV) {
+          e += 0.5 * phi;
+          const double hf = 0.5 * fpair;
+          w0 += delx * delx * hf;
+          w1 += dely * dely * hf;
+          w2 += delz * delz * hf;
+          w3 += delx * dely * hf;
+          w4 += delx * delz * hf;
+          w5 += dely * delz * hf;
+        }
+      }
+    }
+    if (EV) e += eemb[i];
+    f[i] = make_double4(fx, fy, fz, e);
+    if (EV) {
+      vatom[i] = w0;
+      vatom[(size_t)npad + i] = w1;
+      vatom[(size_t)2 * npad + i] = w2;
+      vatom[(size_t)3 * npad + i] = w3;
+      vatom[(size_t)4 * npad + i] = w4;
+      vatom[(size_t)5 * npad + i] = w5;
+    }
+  }
+  if (EV) {
+    double v[7] = {e, w0, w1, w2, w3, w4, w5};
+    block_reduce_store<7>(v, partials);
+  }
+}
+
+template <bool MIXED, bool EV>
+void launch_lj(mdx_ctx *c, int G, int B) {
+  hipLaunchKernelGGL((k_lj<MIXED, EV>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->ntypes + 1, c->prd[0],
+                     c->prd[1], c->prd[2], c->x, c->nbr, c->numneigh, c->lj_dev, c->f, c->vatom, c->partials);
+}
+
+template <bool MIXED, bool EV>
+void launch_eam(mdx_ctx *c, int G, int B) {
+  hipLaunchKernelGGL((k_eam_rho<MIXED, EV>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->prd[0], c->prd[1],
+                     c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
+  hipLaunchKernelGGL((k_eam_force<MIXED, EV>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->prd[0],
+                     c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
+                     c->partials);
+}
+
+}  // namespace
+
+int pair_compute(mdx_ctx *c, int ev) {
+  if (c->n == 0) return MDX_OK;
+  const int B = c->launch.block;
+  const int G = (c->n + B - 1) / B;
+  if (c->profile) hipEventRecord(c->ev[2], c->stream);
+  const bool mixed = c->precision == MDX_MIXED;
+  if (c->pair_style == MDX_PAIR_LJ) {
+    if (mixed) {
+      if (ev) launch_lj<true, true>(c, G, B);
+      else launch_lj<true, false>(c, G, B);
+    } else {
+      if (ev) launch_lj<false, true>(c, G, B);
+      else launch_lj<false, false>(c, G, B);
+    }
+  } else if (c->pair_style == MDX_PAIR_EAM) {
+    if (mixed) {
+      if (ev) launch_eam<true, true>(c, G, B);
+      else launch_eam<true, false>(c, G, B);
+    } else {
+      if (ev) launch_eam<false, true>(c, G, B);
+      else launch_eam<false, false>(c, G, B);
+    }
+  } else {
+    set_error("no pair style set");
+    return MDX_ERR_ARG;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
+  if (c->profile) {
+    hipEventRecord(c->ev[3], c->stream);
+    hipEventSynchronize(c->ev[3]);
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+    c->ms_pair += ms;
+  }
+  // the per-atom energy in f.w already includes the EAM embedding term
+  if (ev) return finish_reduction(c, G, 7);
+  return MDX_OK;
+}
+
+}  // namespace mdx

@@ -1,0 +1,263 @@
+"""ctypes binding of libmdx (include/mdx.h) for tests, bench.py and scripts.
+
+Loads the in-tree lammps_amd/libmdx.so and fails loudly when it is missing:
+there is no CPU fallback for the device path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdx.so")
+
+FP64, MIXED = 0, 1
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_OVERFLOW = 0, -1, -2, -3, -4
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+THERMO_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(ctypes.c_double))
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ncalls", ctypes.c_long), ("ndanger", ctypes.c_long), ("total_neighbors", ctypes.c_long),
+                ("max_neighbors", ctypes.c_int), ("capacity", ctypes.c_int), ("nbins", ctypes.c_int * 3),
+                ("ms_neigh", ctypes.c_double), ("ms_pair", ctypes.c_double), ("ms_integrate", ctypes.c_double)]
+
+
+class MdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mdx error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmdx.so is not built (run __graft_entry__.build() or python -m lammps_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "mdx_version": [],
+            "mdx_create": [_i, ctypes.POINTER(_p)],
+            "mdx_destroy": [_p],
+            "mdx_set_box": [_p, _dp, _dp, _ip],
+            "mdx_set_atoms": [_p, _i, _i, _dp, _dp, _ip, _ip, _dp],
+            "mdx_set_positions": [_p, _dp],
+            "mdx_get_atoms": [_p, _dp, _dp, _dp, _ip],
+            "mdx_pair_lj_init": [_p, _i, _dp, _dp, _dp, _dp, _dp, _dp],
+            "mdx_pair_eam_init": [_p, _i, _i, _d, _i, _d, _d, _d, _i, _dp, _i, _dp, _i, _dp, _ip, _ip, _ip],
+            "mdx_set_precision": [_p, _i],
+            "mdx_neigh_modify": [_p, _d, _i, _i, _i, _i],
+            "mdx_neigh_build": [_p],
+            "mdx_neigh_pairs": [_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long)],
+            "mdx_compute": [_p, _i, _i, _dp, _dp, _dp, _dp, _dp],
+            "mdx_eam_rho_fp": [_p, _dp, _dp],
+            "mdx_setup": [_p, THERMO_FN, _p],
+            "mdx_run": [_p, _i, _d, _d, _i, THERMO_FN, _p],
+            "mdx_get_stats": [_p, ctypes.POINTER(Stats)],
+            "mdx_set_launch": [_p, _i, _i],
+            "mdx_set_profiling": [_p, _i],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _i
+        L.mdx_last_error.restype = ctypes.c_char_p
+        L.mdx_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names every entry point that include/mdx.h declares (for load tests)."""
+    return ["mdx_last_error", "mdx_version", "mdx_create", "mdx_destroy", "mdx_set_box", "mdx_set_atoms",
+            "mdx_set_positions", "mdx_get_atoms", "mdx_pair_lj_init", "mdx_pair_eam_init", "mdx_set_precision",
+            "mdx_neigh_modify", "mdx_neigh_build", "mdx_neigh_pairs", "mdx_compute", "mdx_eam_rho_fp",
+            "mdx_setup", "mdx_run", "mdx_get_stats", "mdx_set_profiling", "mdx_set_launch"]
+
+
+def _ptr(a, kind=_dp):
+    return None if a is None else a.ctypes.data_as(kind)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """One device context (all device memory of one simulated system)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = _p()
+        self._ck(self._L.mdx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.n = 0
+        self._thermo_rows = []
+        self._cb = THERMO_FN(self._on_thermo)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.mdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise MdxError(rc, self._L.mdx_last_error().decode())
+        return rc
+
+    def _on_thermo(self, user, step, row):
+        self._thermo_rows.append((int(step), [row[k] for k in range(8)]))
+
+    # --- state ---
+    def set_box(self, lo, hi, periodic=(1, 1, 1)):
+        lo, hi, per = _f64(lo), _f64(hi), _i32(periodic)
+        self._ck(self._L.mdx_set_box(self.h, _ptr(lo), _ptr(hi), _ptr(per, _ip)))
+
+    def set_atoms(self, x, types, mass, v=None, tags=None):
+        x = _f64(x)
+        t = _i32(types)
+        n = len(t)
+        mass = _f64(mass)
+        ntypes = len(mass) - 1
+        v = None if v is None else _f64(v)
+        tags = None if tags is None else _i32(tags)
+        self._ck(self._L.mdx_set_atoms(self.h, n, ntypes, _ptr(x), _ptr(v), _ptr(t, _ip), _ptr(tags, _ip),
+                                       _ptr(mass)))
+        self.n = n
+
+    def set_positions(self, x):
+        x = _f64(x)
+        self._ck(self._L.mdx_set_positions(self.h, _ptr(x)))
+
+    def get_atoms(self):
+        n = self.n
+        x = np.zeros((n, 3)); v = np.zeros((n, 3)); f = np.zeros((n, 3)); im = np.zeros((n, 3), dtype=np.int32)
+        self._ck(self._L.mdx_get_atoms(self.h, _ptr(x), _ptr(v), _ptr(f), _ptr(im, _ip)))
+        return x, v, f, im
+
+    # --- styles ---
+    def pair_lj(self, ntypes, cutsq, lj1, lj2, lj3, lj4, offset):
+        a = [_f64(q).ravel() for q in (cutsq, lj1, lj2, lj3, lj4, offset)]
+        self._ck(self._L.mdx_pair_lj_init(self.h, ntypes, *[_ptr(q) for q in a]))
+
+    def pair_lj_from_coeffs(self, ntypes, coeffs, shift=False):
+        """coeffs {(i,j): (eps, sigma, cut)} for i <= j with every (i,i) set;
+        mixing (geometric) and init_one as src/pair_lj_cut.cpp:545-602."""
+        nt1 = ntypes + 1
+        eps = np.zeros((nt1, nt1)); sig = np.zeros((nt1, nt1)); cut = np.zeros((nt1, nt1))
+        for (i, j), (e, s, c) in coeffs.items():
+            i, j = min(i, j), max(i, j)
+            eps[i, j], sig[i, j], cut[i, j] = e, s, c
+        tabs = {k: np.zeros((nt1, nt1)) for k in ("cutsq", "lj1", "lj2", "lj3", "lj4", "offset")}
+        for i in range(1, nt1):
+            for j in range(i, nt1):
+                if (i, j) not in coeffs and (j, i) not in coeffs:
+                    eps[i, j] = np.sqrt(eps[i, i] * eps[j, j])
+                    sig[i, j] = np.sqrt(sig[i, i] * sig[j, j])
+                    cut[i, j] = np.sqrt(cut[i, i] * cut[j, j])
+                e, s, c = eps[i, j], sig[i, j], cut[i, j]
+                vals = {"cutsq": c * c, "lj1": 48.0 * e * s ** 12.0, "lj2": 24.0 * e * s ** 6.0,
+                        "lj3": 4.0 * e * s ** 12.0, "lj4": 4.0 * e * s ** 6.0,
+                        "offset": 4.0 * e * ((s / c) ** 12.0 - (s / c) ** 6.0) if shift else 0.0}
+                for k, val in vals.items():
+                    tabs[k][i, j] = tabs[k][j, i] = val
+        self.pair_lj(ntypes, tabs["cutsq"], tabs["lj1"], tabs["lj2"], tabs["lj3"], tabs["lj4"], tabs["offset"])
+        return tabs
+
+    def pair_eam(self, ntypes, nrho, drho, nr, dr, rhomax, cutforcesq, frho_spline, rhor_spline, z2r_spline,
+                 type2frho, type2rhor, type2z2r):
+        fr, rr, zr = _f64(frho_spline), _f64(rhor_spline), _f64(z2r_spline)
+        self._ck(self._L.mdx_pair_eam_init(
+            self.h, ntypes, nrho, drho, nr, dr, rhomax, cutforcesq, fr.shape[0], _ptr(fr), rr.shape[0], _ptr(rr),
+            zr.shape[0], _ptr(zr), _ptr(_i32(type2frho), _ip), _ptr(_i32(type2rhor), _ip),
+            _ptr(_i32(type2z2r), _ip)))
+
+    def precision(self, p):
+        self._ck(self._L.mdx_set_precision(self.h, p))
+
+    def neigh_modify(self, skin, every=1, delay=10, check=True, one=2000):
+        self._ck(self._L.mdx_neigh_modify(self.h, skin, every, delay, 1 if check else 0, one))
+
+    def set_launch(self, block=0, tpa=0):
+        self._ck(self._L.mdx_set_launch(self.h, block, tpa))
+
+    def profiling(self, on=True):
+        self._ck(self._L.mdx_set_profiling(self.h, 1 if on else 0))
+
+    # --- work ---
+    def neigh_build(self):
+        self._ck(self._L.mdx_neigh_build(self.h))
+
+    def neigh_pairs(self):
+        cnt = ctypes.c_long(0)
+        self._ck(self._L.mdx_neigh_pairs(self.h, None, 0, ctypes.byref(cnt)))
+        buf = np.zeros((cnt.value, 2), dtype=np.int32)
+        self._ck(self._L.mdx_neigh_pairs(self.h, _ptr(buf, _ip), cnt.value, ctypes.byref(cnt)))
+        return buf
+
+    def compute(self, eflag=1, vflag=1, peratom=False):
+        n = self.n
+        f = np.zeros((n, 3))
+        e = ctypes.c_double(0.0)
+        w = np.zeros(6)
+        ea = np.zeros(n) if peratom else None
+        va = np.zeros((n, 6)) if peratom else None
+        self._ck(self._L.mdx_compute(self.h, eflag, vflag, _ptr(f), ctypes.byref(e), _ptr(w), _ptr(ea), _ptr(va)))
+        return {"f": f, "eng": e.value, "virial": w, "eatom": ea, "vatom": va}
+
+    def eam_rho_fp(self):
+        r = np.zeros(self.n); fp = np.zeros(self.n)
+        self._ck(self._L.mdx_eam_rho_fp(self.h, _ptr(r), _ptr(fp)))
+        return r, fp
+
+    def setup(self):
+        self._thermo_rows = []
+        self._ck(self._L.mdx_setup(self.h, self._cb, None))
+        return list(self._thermo_rows)
+
+    def run(self, nsteps, dt, ftm2v=1.0, thermo_every=0):
+        self._thermo_rows = []
+        self._ck(self._L.mdx_run(self.h, nsteps, dt, ftm2v, thermo_every, self._cb, None))
+        return list(self._thermo_rows)
+
+    def stats(self):
+        s = Stats()
+        self._ck(self._L.mdx_get_stats(self.h, ctypes.byref(s)))
+        return {k: (list(getattr(s, k)) if k == "nbins" else getattr(s, k)) for k, _ in Stats._fields_}
+
+
+def thermo_columns(raw, natoms, prd, units="lj"):
+    """Thermo row (Temp, E_pair, E_mol, TotEng, Press) from the raw sums the
+    library reports, with thermo.cpp's normalisation (lj: per atom) and
+    compute_temp/compute_pressure's formulas (dof = 3N-3)."""
+    if units == "lj":
+        boltz, mvv2e, nktv2p, norm = 1.0, 1.0, 1.0, True
+    else:
+        boltz, mvv2e, nktv2p, norm = 8.617343e-5, 1.0364269e-4, 1.6021765e6, False
+    mv2, pe = raw[0], raw[1]
+    dof = 3.0 * natoms - 3.0
+    t = mv2 * mvv2e / (dof * boltz)
+    ke = t * 0.5 * dof * boltz
+    epair = pe / natoms if norm else pe
+    if norm:
+        ke /= natoms
+    vol = prd[0] * prd[1] * prd[2]
+    press = (dof * boltz * t + raw[2] + raw[3] + raw[4]) / 3.0 / vol * nktv2p
+    return [t, epair, 0.0, epair + ke, press]

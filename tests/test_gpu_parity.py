@@ -1,0 +1,161 @@
+"""GPU parity: the HIP path through the C ABI against the CPU FP64 oracle.
+
+Bars (stated here, in the test):
+  * neighbour pair sets: identical to the oracle's half_bin_newton set
+    (bit-exact integer work), full-list size exactly twice the half list;
+  * FP64 path: per-atom forces max_rel <= 1e-11, energy/virial rel <= 1e-11;
+  * mixed path: max_rel <= 1e-5 (the force tolerance of the target), energy
+    and virial rel <= 1e-6;
+  * NVE runs: thermo rows equal to the reference log's printed digits, and
+    the reference's 'Total # of neighbors' and build count.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from gpu_helpers import box_of, device_from_oracle, eam_oracle, force_errors, lj_oracle
+from oracle import mdref as M
+
+pytestmark = pytest.mark.gpu
+
+A_LJ = (4.0 / 0.8442) ** (1.0 / 3.0)
+
+with open(os.path.join(GOLDEN, "goldens.json")) as fh:
+    GOLD = json.load(fh)
+
+
+def fmt(row):
+    return ["%d" % int(row[0])] + [("%12.8g" % v).strip() for v in row[1:]]
+
+
+@pytest.fixture(scope="module")
+def lj_state():
+    r = lj_oracle(ncell=20, steps=37, every=20, check=False)
+    r.compute(3, 5)
+    return r
+
+
+@pytest.fixture(scope="module")
+def eam_state():
+    r = eam_oracle(ncell=20, steps=40)
+    r.compute(3, 5)
+    return r
+
+
+def test_lj_neighbor_pairs_identical(lj_state):
+    from lammps_amd import mdx
+    r = lj_state
+    lo, hi = box_of(r, 20, A_LJ)
+    ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+    ctx.neigh_build()
+    got = ctx.neigh_pairs()
+    ref = r.list_pairs()
+    ref_set = {(min(a, b), max(a, b)) for a, b in ref}
+    got_set = {(int(a), int(b)) for a, b in got}
+    assert len(got) == len(got_set)
+    assert got_set == ref_set
+    st = ctx.stats()
+    assert st["total_neighbors"] == 2 * len(ref_set)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lj_forces_energy_virial(lj_state, prec):
+    from lammps_amd import mdx
+    r = lj_state
+    lo, hi = box_of(r, 20, A_LJ)
+    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
+    ctx = device_from_oracle(r, lo, hi, "lj", p, 0.3, 20, 0, False)
+    out = ctx.compute(3, 5, peratom=True)
+    _, _, fref, _, _ = r.state()
+    eref, wref = r.energy_virial()
+    ea_ref, va_ref = r.peratom()
+    mx, rms = force_errors(out["f"], fref)
+    tol = 1e-11 if prec == "fp64" else 1e-5
+    etol = 1e-11 if prec == "fp64" else 1e-6
+    print("LJ %s force max_rel=%.3e rms_rel=%.3e" % (prec, mx, rms))
+    assert mx <= tol
+    assert abs(out["eng"] - eref) <= etol * abs(eref)
+    np.testing.assert_allclose(out["virial"], wref, rtol=etol, atol=etol * np.abs(wref).max())
+    np.testing.assert_allclose(out["eatom"], ea_ref, rtol=0, atol=100 * etol * np.abs(ea_ref).max())
+    np.testing.assert_allclose(out["vatom"], va_ref, rtol=0, atol=100 * etol * np.abs(va_ref).max())
+
+
+def test_eam_neighbor_pairs_identical(eam_state):
+    from lammps_amd import mdx
+    r = eam_state
+    lo, hi = box_of(r, 20, 3.615)
+    ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
+    ctx.neigh_build()
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
+    assert got == ref
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_eam_forces_energy_virial(eam_state, prec):
+    from lammps_amd import mdx
+    r = eam_state
+    lo, hi = box_of(r, 20, 3.615)
+    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
+    ctx = device_from_oracle(r, lo, hi, "eam", p, 1.0, 1, 5, True)
+    out = ctx.compute(3, 5, peratom=True)
+    _, _, fref, _, _ = r.state()
+    eref, wref = r.energy_virial()
+    rho_ref, fp_ref = r.eam_rho_fp()
+    rho, fp = ctx.eam_rho_fp()
+    mx, rms = force_errors(out["f"], fref)
+    tol = 1e-11 if prec == "fp64" else 1e-5
+    etol = 1e-11 if prec == "fp64" else 1e-6
+    print("EAM %s force max_rel=%.3e rms_rel=%.3e" % (prec, mx, rms))
+    assert mx <= tol
+    np.testing.assert_allclose(rho, rho_ref, rtol=etol)
+    np.testing.assert_allclose(fp, fp_ref, rtol=etol * 10, atol=etol)
+    assert abs(out["eng"] - eref) <= etol * abs(eref)
+    np.testing.assert_allclose(out["virial"], wref, rtol=etol, atol=etol * np.abs(wref).max())
+
+
+def test_lj_bench_run_matches_reference_log():
+    """bench/in.lj on the device (FP64): thermo rows at steps 0 and 100 equal
+    the log's printed digits, 1202833 neighbours, 5 builds."""
+    from lammps_amd import mdx
+    g = GOLD["lj_bench_32k"]
+    r = M.Ref(M.LJ)
+    r.lattice_box(M.FCC, 0.8442, (0, 0, 0), (20, 20, 20))
+    r.mass(1, 1.0)
+    r.velocity_create(1.44, 87287, dist="uniform", loop="geom")
+    lo, hi = box_of(r, 20, A_LJ)
+    r.pair_lj(2.5)
+    r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+    rows = ctx.setup() + ctx.run(100, 0.005, 1.0, 0)
+    prd = np.array(hi) - np.array(lo)
+    got = [[s] + mdx.thermo_columns(raw, 32000, prd, "lj") for s, raw in rows]
+    assert [fmt(x) for x in got] == g["thermo_text"]
+    st = ctx.stats()
+    assert st["total_neighbors"] == 2 * g["total_neighbors"]
+    assert st["ncalls"] == g["builds"]
+
+
+def test_eam_bench_run_matches_reference_log():
+    """bench/in.eam on the device (FP64): rows at 0, 50, 100, builds and neighbours."""
+    from lammps_amd import mdx
+    g = GOLD["eam_bench_32k"]
+    r = M.Ref(M.METAL)
+    r.lattice_box(M.FCC, 3.615, (0, 0, 0), (20, 20, 20))
+    r.pair_eam(os.path.join(GOLDEN, "Cu_u3.eam"))
+    r.velocity_create(1600.0, 376847, dist="uniform", loop="geom")
+    r.neighbor(1.0, every=1, delay=5, check=True)
+    r.compute(1, 0)  # initialises the spline tables
+    lo, hi = box_of(r, 20, 3.615)
+    ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
+    rows = ctx.setup() + ctx.run(100, 0.005, 1.0 / 1.0364269e-4, 50)
+    prd = np.array(hi) - np.array(lo)
+    got = [[s] + mdx.thermo_columns(raw, 32000, prd, "metal") for s, raw in rows]
+    assert [fmt(x) for x in got] == g["thermo_text"]
+    st = ctx.stats()
+    assert st["total_neighbors"] == 2 * g["total_neighbors"]
+    assert st["ncalls"] == g["builds"]
+    assert st["ndanger"] == g["dangerous"]
