@@ -47,6 +47,18 @@ typedef struct mdx_ctx mdx_ctx;
    row[1] = pair energy (eng_vdwl), row[2..7] = pair virial xx yy zz xy xz yz */
 typedef void (*mdx_thermo_fn)(void *user, long step, const double *row);
 
+/* kernels timed when profiling is on (index into mdx_stats.kernel_ms) */
+enum {
+  MDX_K_BUILD = 0,       /* whole list build: wrap/bin, sort, permute, list */
+  MDX_K_LJ = 1,          /* lj/cut force kernel */
+  MDX_K_EAM_RHO = 2,     /* eam density + embedding derivative kernel */
+  MDX_K_EAM_FORCE = 3,   /* eam force kernel */
+  MDX_K_NVE_INITIAL = 4, /* fix nve initial integrate (+ rebuild trigger) */
+  MDX_K_NVE_FINAL = 5,   /* fix nve final integrate */
+  MDX_K_FUSED = 6,       /* force + final + next initial integrate, fused */
+  MDX_NKERNEL = 8
+};
+
 typedef struct {
   long ncalls;          /* neighbour builds during mdx_run (Neighbor::ncalls) */
   long ndanger;         /* dangerous builds (neighbor.cpp:1370) */
@@ -54,9 +66,8 @@ typedef struct {
   int max_neighbors;    /* largest per-atom count at the last build */
   int capacity;         /* per-atom list capacity currently allocated */
   int nbins[3];         /* device bin grid */
-  double ms_neigh;      /* accumulated device time of list builds (ms) */
-  double ms_pair;       /* accumulated device time of pair kernels (ms) */
-  double ms_integrate;  /* accumulated device time of integrator kernels (ms) */
+  double kernel_ms[MDX_NKERNEL]; /* accumulated device time per kernel kind (ms) */
+  long kernel_launches[MDX_NKERNEL];
 } mdx_stats;
 
 const char *mdx_last_error(void);
@@ -144,13 +155,18 @@ int mdx_run(mdx_ctx *ctx, int nsteps, double dt, double ftm2v, int thermo_every,
 
 int mdx_get_stats(mdx_ctx *ctx, mdx_stats *out);
 
-/* Times every list build, pair launch and step with HIP events on the
-   context's stream (adds a sync per phase); resets the accumulated times. */
+/* Times every list build, pair launch and integrator launch with HIP event
+   pairs on the context's stream (no synchronisation is added; the events
+   are resolved by mdx_get_stats).  Resets the accumulated times. */
 int mdx_set_profiling(mdx_ctx *ctx, int on);
 
-/* Kernel shape overrides for the autotuner (0 = library default):
-   block = threads per workgroup, tpa = threads per atom in pair kernels. */
+/* Kernel shape of the pair kernels, set by the autotuner (0 = library
+   default): block = threads per workgroup (64..256, multiple of 64), tpa =
+   threads per atom (1, 2, 4, 8, 16 or 32; the `threads_per_atom` keyword of
+   the reference's `package gpu`, src/GPU/fix_gpu.cpp:112-150). */
 int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
+/* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default). */
+int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
 
 #ifdef __cplusplus
 }

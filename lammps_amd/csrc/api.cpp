@@ -21,6 +21,39 @@ int hip_fail(hipError_t e, const char *what) {
   return e == hipErrorOutOfMemory ? MDX_ERR_NOMEM : MDX_ERR_HIP;
 }
 
+void prof_begin(mdx_ctx *c, int kind) {
+  if (!c->profile) return;
+  if (c->evused == c->evpool.size()) {
+    mdx_ctx::EvRec r{};
+    hipEventCreate(&r.a);
+    hipEventCreate(&r.b);
+    c->evpool.push_back(r);
+  }
+  c->evpool[c->evused].kind = kind;
+  hipEventRecord(c->evpool[c->evused].a, c->stream);
+}
+
+void prof_end(mdx_ctx *c) {
+  if (!c->profile) return;
+  hipEventRecord(c->evpool[c->evused].b, c->stream);
+  c->evused++;
+}
+
+int prof_resolve(mdx_ctx *c) {
+  if (c->evused == 0) return MDX_OK;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(e, "profile sync");
+  for (size_t k = 0; k < c->evused; k++) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->evpool[k].a, c->evpool[k].b) == hipSuccess) {
+      c->kernel_ms[c->evpool[k].kind] += ms;
+      c->kernel_n[c->evpool[k].kind]++;
+    }
+  }
+  c->evused = 0;
+  return MDX_OK;
+}
+
 // device-order gathers into caller order (integrate.hip)
 int gather_w(mdx_ctx *c, const double4 *dev, double *host_out);
 int gather_soa(mdx_ctx *c, const double *dev, int ncols, double *host_out);
@@ -114,7 +147,8 @@ int prepare(mdx_ctx *c) {
     c->maxneigh = guess;
     c->nbr_elems = (long)guess * c->npad;
   }
-  const int G = (c->n + 63) / 64 + 1;  // enough for any block size >= 64
+  // enough blocks for any launch: block >= 64 threads, up to 32 threads per atom
+  const long G = ((long)c->n * 32 + 63) / 64 + 1;
   if (G * 8 > c->cap_partials) {
     if ((rc = dev_alloc(&c->partials, (size_t)G * 8))) return rc;
     c->cap_partials = G * 8;
@@ -160,7 +194,6 @@ int mdx_create(int device, mdx_ctx **out) {
     delete c;
     return hip_fail(e, "stream");
   }
-  for (auto &ev : c->ev) hipEventCreate(&ev);
   if (hipMalloc(&c->dflags, 64) != hipSuccess || hipMalloc(&c->red_out, sizeof(double) * MDX_RED_SLOTS) != hipSuccess ||
       hipHostMalloc(&c->red_host, sizeof(double) * MDX_RED_SLOTS, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&c->lj_dev, sizeof(LJTab)) != hipSuccess) {
@@ -182,8 +215,10 @@ int mdx_destroy(mdx_ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->red_host) hipHostFree(c->red_host);
-  for (auto &ev : c->ev)
-    if (ev) hipEventDestroy(ev);
+  for (auto &r : c->evpool) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return MDX_OK;
@@ -474,7 +509,7 @@ int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
   long cnt = 0;
   for (int i = 0; i < n; i++)
     for (int k = 0; k < num[i]; k++) {
-      const int j = nb[(size_t)k * c->npad + i] & MDX_IDX_MASK;
+      const int j = nb[(size_t)i * c->maxneigh + k] & MDX_IDX_MASK;
       if (tag[i] < tag[j]) {
         if (pairs && cnt < cap) {
           pairs[2 * cnt] = tag[i];
@@ -573,7 +608,6 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
     const int ago_next = c->ago + 1;
     const bool decide_step = ago_next >= c->delay && ago_next % c->every == 0;
     const int check_now = decide_step && c->check;
-    if (c->profile) hipEventRecord(c->ev[4], c->stream);
     if ((rc = nve_initial(c, dtv, dtf, check_now))) return rc;
     c->ago = ago_next;
     bool rebuild = false;
@@ -592,13 +626,8 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
     }
     if ((rc = force_step(c, out ? 1 : 0))) return rc;
     if ((rc = nve_final(c, dtf))) return rc;
-    if (c->profile) {
-      hipEventRecord(c->ev[5], c->stream);
-      hipEventSynchronize(c->ev[5]);
-      float ms = 0;
-      hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
-      c->ms_integrate += ms;
-    }
+    // bound the event pool: resolve on thermo steps and every 4096 records
+    if (c->evused > 4096 && (rc = prof_resolve(c))) return rc;
     if (out && (rc = thermo_out(c, cb, user))) return rc;
   }
   hipError_t e = hipStreamSynchronize(c->stream);
@@ -618,9 +647,11 @@ int mdx_get_stats(mdx_ctx *c, mdx_stats *out) {
   out->max_neighbors = c->max_neighbors;
   out->capacity = c->maxneigh;
   for (int d = 0; d < 3; d++) out->nbins[d] = c->na.nbin[d];
-  out->ms_neigh = c->ms_neigh;
-  out->ms_pair = c->ms_pair;
-  out->ms_integrate = c->ms_integrate;
+  if ((rc = prof_resolve(c))) return rc;
+  for (int k = 0; k < MDX_NKERNEL; k++) {
+    out->kernel_ms[k] = c->kernel_ms[k];
+    out->kernel_launches[k] = c->kernel_n[k];
+  }
   return MDX_OK;
 }
 
@@ -631,20 +662,35 @@ int mdx_set_launch(mdx_ctx *c, int block, int tpa) {
     set_error("block must be a multiple of 64 in [64, 256]");
     return MDX_ERR_ARG;
   }
-  if (tpa != 0 && tpa != 1) {
-    set_error("threads per atom must be 1 in this build");
+  if (tpa != 0 && tpa != 1 && tpa != 2 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32) {
+    set_error("threads per atom must be one of 1, 2, 4, 8, 16, 32");
     return MDX_ERR_ARG;
   }
   c->launch.block = block ? block : 256;
-  c->launch.tpa = tpa ? tpa : 1;
+  c->launch.tpa = tpa ? tpa : 8;
+  return MDX_OK;
+}
+
+int mdx_set_build_launch(mdx_ctx *c, int tpa) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (tpa != 0 && tpa != 1 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32) {
+    set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
+    return MDX_ERR_ARG;
+  }
+  c->launch.build_tpa = tpa ? tpa : 1;
   return MDX_OK;
 }
 
 int mdx_set_profiling(mdx_ctx *c, int on) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if ((rc = prof_resolve(c))) return rc;
   c->profile = on != 0;
-  c->ms_neigh = c->ms_pair = c->ms_integrate = 0.0;
+  for (int k = 0; k < MDX_NKERNEL; k++) {
+    c->kernel_ms[k] = 0.0;
+    c->kernel_n[k] = 0;
+  }
   return MDX_OK;
 }
 

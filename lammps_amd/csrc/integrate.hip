@@ -155,8 +155,10 @@ int nve_initial(mdx_ctx *c, double dtv, double dtf, int check) {
     hipError_t e = hipMemsetAsync(c->dflags + 1, 0, sizeof(int), c->stream);
     if (e != hipSuccess) return hip_fail(e, "clear trigger flag");
   }
+  prof_begin(c, MDX_K_NVE_INITIAL);
   hipLaunchKernelGGL(k_nve_initial, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtv, dtf, c->x, c->v, c->f,
                      c->xhold, check, c->triggersq, c->dflags);
+  prof_end(c);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDX_OK : hip_fail(e, "nve initial");
 }
@@ -164,7 +166,9 @@ int nve_initial(mdx_ctx *c, double dtv, double dtf, int check) {
 int nve_final(mdx_ctx *c, double dtf) {
   if (c->n == 0) return MDX_OK;
   const int B = 256, G = (c->n + B - 1) / B;
+  prof_begin(c, MDX_K_NVE_FINAL);
   hipLaunchKernelGGL(k_nve_final, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtf, c->x, c->v, c->f);
+  prof_end(c);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDX_OK : hip_fail(e, "nve final");
 }

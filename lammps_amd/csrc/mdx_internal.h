@@ -8,10 +8,11 @@
 //   f      double4[npad]  fx, fy, fz, w = per-atom energy of the last compute
 //   tag    int[npad]      atom ID;  orig int[npad]: caller index of each atom
 //   image  int[npad]      periodic images, 10 bits per dimension (IMGMAX 512)
-//   nbr    int[maxneigh][npad]  full list, column major (coalesced for
-//                                one-thread-per-atom reads); bits 0-26 hold
-//                                the neighbour index, bits 27-31 a periodic
-//                                image code (sx+1) + 3(sy+1) + 9(sz+1)
+//   nbr    int[npad][maxneigh]  full list, one row per atom (row major, so
+//                                the lanes of an atom read consecutive
+//                                entries); bits 0-26 hold the neighbour
+//                                index, bits 27-31 a periodic image code
+//                                (sx+1) + 3(sy+1) + 9(sz+1)
 //   vatom  double[6][npad] per-atom virial (SoA)
 #pragma once
 #include <hip/hip_runtime.h>
@@ -21,6 +22,7 @@
 #include <cstring>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/mdx.h"
 
@@ -69,8 +71,9 @@ struct EAMArgs {
 };
 
 struct Launch {
-  int block = 256;
-  int tpa = 1;
+  int block = 256;     // threads per workgroup of the pair kernels
+  int tpa = 8;         // threads per atom in the pair kernels
+  int build_tpa = 1;   // threads per atom in the list build
 };
 
 }  // namespace mdx
@@ -126,16 +129,24 @@ struct mdx_ctx {
   double *rho = nullptr, *fp = nullptr, *eemb = nullptr;
   // reductions
   double *partials = nullptr;
-  int cap_partials = 0;
+  long cap_partials = 0;
+  int last_grid = 0;  // blocks of the last pair launch (partials to reduce)
   double *red_out = nullptr;    // device, MDX_RED_SLOTS
   double *red_host = nullptr;   // pinned, MDX_RED_SLOTS
   // results of the last compute
   double eng_vdwl = 0.0, virial[6] = {0, 0, 0, 0, 0, 0};
-  // launch shape and timing
+  // launch shape and timing (event pairs recorded on `stream`, resolved
+  // without a per-launch sync when the statistics are read)
   mdx::Launch launch;
   bool profile = false;
-  hipEvent_t ev[8] = {nullptr};
-  double ms_neigh = 0, ms_pair = 0, ms_integrate = 0;
+  struct EvRec {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<EvRec> evpool;
+  size_t evused = 0;
+  double kernel_ms[MDX_NKERNEL] = {0};
+  long kernel_n[MDX_NKERNEL] = {0};
   long total_neighbors = 0;
   int max_neighbors = 0;
 };
@@ -144,6 +155,11 @@ namespace mdx {
 
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
+
+// kernel timing with HIP events (no-ops unless profiling is on)
+void prof_begin(mdx_ctx *c, int kind);
+void prof_end(mdx_ctx *c);
+int prof_resolve(mdx_ctx *c);
 
 // neighbour build (neigh.hip)
 int neigh_setup_grid(mdx_ctx *c);

@@ -8,13 +8,17 @@
 // box (Domain::pbc, src/domain.cpp:478), binned on the reference's own bin
 // grid (binsize = cutneigh/2, src/neighbor.cpp:1580-1620), and stably
 // radix-sorted by bin so that every x-row of bins is one contiguous atom
-// range.  One thread per atom then scans the 5x5 rows of its stencil as at
-// most 2 contiguous segments each and keeps j iff rsq <= cutneighsq, with rsq
-// evaluated in FP64 without contraction on the periodic image x_j + s*L, the
-// value the reference computes for a ghost atom (atom_vec_atomic.cpp:290-297).
-// The result is a FULL list (each pair at both atoms): forces become a
-// gather, deterministic and atomic-free.  The stored image code lets the
-// pair kernels rebuild the same image between builds.
+// range.  A group of TPA lanes per atom then scans the 5x5 rows of its
+// stencil (each row at most 3 contiguous segments: wrapped-low, main,
+// wrapped-high) and keeps j iff rsq <= cutneighsq, with rsq evaluated in
+// FP64 without contraction on the periodic image x_j + s*L, the value the
+// reference computes for a ghost atom (atom_vec_atomic.cpp:290-297).  The
+// lanes of a group test consecutive candidates (coalesced gathers); a
+// wavefront ballot and a popcount prefix give each kept candidate its slot,
+// so the list order is the sequential scan order whatever TPA is.
+// The result is a FULL list, one row of `cap` entries per atom (row major);
+// each entry holds the neighbour index (bits 0-26) and its periodic image
+// code (bits 27-31) so pair kernels rebuild the same image between builds.
 #include <hipcub/hipcub.hpp>
 
 #include "mdx_internal.h"
@@ -128,20 +132,26 @@ __global__ void k_permute(int n, const int *__restrict__ src, const double4 *__r
   inv[oi] = k;
 }
 
-// Full list, one thread per atom i (atoms are bin sorted, so the lanes of a
-// wavefront walk the same rows and their x_j gathers hit the same lines).
+// Full list, TPA lanes per atom.  All lanes of a group run the same loop
+// trip counts (bounds depend only on the atom), so the group's lanes are
+// active together at every ballot.
+template <int TPA>
 __global__ __launch_bounds__(256) void k_build_full(NeighArgs a, const double4 *__restrict__ x,
                                                    const unsigned *__restrict__ bkey,
                                                    const int *__restrict__ bin_start, int *__restrict__ nbr,
                                                    int *__restrict__ numneigh, int *__restrict__ dflags,
                                                    unsigned long long *__restrict__ nsum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  if (i >= a.n) return;  // whole groups leave together (TPA divides 64)
+  const int lane = threadIdx.x % TPA;
+  const int gshift = (threadIdx.x & 63) - lane;  // group's first lane within the wavefront
+  const unsigned long long below = (1ull << lane) - 1ull;
   const double4 xi = x[i];
   const int ti = static_cast<int>(xi.w);
   const unsigned b = bkey[i];
   const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
   const int ix = b % nbx, iy = (b / nbx) % nby, iz = b / (nbx * nby);
+  int *row = nbr + (size_t)i * a.maxneigh;
   int cnt = 0;
   for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
     int jz = iz + dz, sz = 0;
@@ -166,56 +176,70 @@ __global__ __launch_bounds__(256) void k_build_full(NeighArgs a, const double4 *
         sy = 1;
       }
       const int rowbase = (jz * nby + jy) * nbx;
-      int x0 = ix - a.S[0], x1 = ix + a.S[0];
-      // up to three segments: wrapped low part, main part, wrapped high part
-      int seg_lo[3], seg_hi[3], seg_s[3], nseg = 0;
-      if (x0 < 0 && a.periodic[0]) {
-        seg_lo[nseg] = max(x0 + nbx, 0);
-        seg_hi[nseg] = nbx - 1;
-        seg_s[nseg++] = -1;
-      }
-      seg_lo[nseg] = max(x0, 0);
-      seg_hi[nseg] = min(x1, nbx - 1);
-      seg_s[nseg++] = 0;
-      if (x1 >= nbx && a.periodic[0]) {
-        seg_lo[nseg] = 0;
-        seg_hi[nseg] = min(x1 - nbx, nbx - 1);
-        seg_s[nseg++] = 1;
-      }
+      const int x0 = ix - a.S[0], x1 = ix + a.S[0];
       const double shy = sy * a.prd[1], shz = sz * a.prd[2];
-      for (int s = 0; s < nseg; s++) {
-        if (seg_lo[s] > seg_hi[s]) continue;
-        const int sx = seg_s[s];
+      // segments in stencil order: wrapped low part, main part, wrapped high part
+      for (int s = 0; s < 3; s++) {
+        int lo, hi, sx;
+        if (s == 0) {
+          if (!(x0 < 0 && a.periodic[0])) continue;
+          lo = max(x0 + nbx, 0);
+          hi = nbx - 1;
+          sx = -1;
+        } else if (s == 1) {
+          lo = max(x0, 0);
+          hi = min(x1, nbx - 1);
+          sx = 0;
+        } else {
+          if (!(x1 >= nbx && a.periodic[0])) continue;
+          lo = 0;
+          hi = min(x1 - nbx, nbx - 1);
+          sx = 1;
+        }
+        if (lo > hi) continue;
         const double shx = sx * a.prd[0];
         const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-        const int j0 = bin_start[rowbase + seg_lo[s]];
-        const int j1 = bin_start[rowbase + seg_hi[s] + 1];
-        for (int j = j0; j < j1; j++) {
-          if (code == MDX_NOSHIFT && j == i) continue;
-          const double4 xj = x[j];
-          const int tj = static_cast<int>(xj.w);
-          // ghost image coordinate as CommBrick::borders forms it
-          const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
-          const double c = a.cutneighsq[ti * a.ntypes1 + tj];
-          const double rsq = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2);
-          bool keep = rsq <= c;
-          if (code != MDX_NOSHIFT) {
-            // the reference evaluates an owned-ghost pair once, from one side
-            const double rsq_b = rsq_exact(xj.x, xj.y, xj.z, xi.x - shx, xi.y - shy, xi.z - shz);
-            const bool keep_b = rsq_b <= c;
-            if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+        const int j0 = bin_start[rowbase + lo];
+        const int j1 = bin_start[rowbase + hi + 1];
+        for (int base = j0; base < j1; base += TPA) {
+          const int j = base + lane;
+          bool keep = false;
+          if (j < j1 && !(code == MDX_NOSHIFT && j == i)) {
+            const double4 xj = x[j];
+            const int tj = static_cast<int>(xj.w);
+            // ghost image coordinate as CommBrick::borders forms it
+            const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
+            const double c = a.cutneighsq[ti * a.ntypes1 + tj];
+            keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
+            if (code != MDX_NOSHIFT) {
+              // the reference evaluates an owned-ghost pair once, from one side
+              const bool keep_b = rsq_exact(xj.x, xj.y, xj.z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
+              if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+            }
           }
+          const unsigned long long m = (__ballot(keep) >> gshift) & ((TPA == 64) ? ~0ull : ((1ull << TPA) - 1ull));
           if (keep) {
-            if (cnt < a.maxneigh) nbr[(size_t)cnt * a.npad + i] = j | (code << MDX_IDX_BITS);
-            cnt++;
+            const int pos = cnt + __popcll(m & below);
+            if (pos < a.maxneigh) row[pos] = j | (code << MDX_IDX_BITS);
           }
+          cnt += __popcll(m);
         }
       }
     }
   }
-  numneigh[i] = cnt;
-  atomicMax(&dflags[0], cnt);
-  atomicAdd(nsum, (unsigned long long)cnt);
+  if (lane == 0) {
+    numneigh[i] = cnt;
+    atomicMax(&dflags[0], cnt);
+    atomicAdd(nsum, (unsigned long long)cnt);
+  }
+}
+
+template <int TPA>
+void launch_build(mdx_ctx *c) {
+  const int B = 256;
+  const int G = (int)(((long)c->n * TPA + B - 1) / B);
+  hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
+                     c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
 }
 
 }  // namespace
@@ -286,7 +310,7 @@ int neigh_build(mdx_ctx *c) {
     c->built = true;
     return MDX_OK;
   }
-  if (c->profile) hipEventRecord(c->ev[0], c->stream);
+  prof_begin(c, MDX_K_BUILD);
   const int B = 256;
   const int G = (c->n + B - 1) / B;
   hipError_t e;
@@ -335,8 +359,16 @@ int neigh_build(mdx_ctx *c) {
     a.maxneigh = c->maxneigh;
     if ((e = hipMemsetAsync(c->dflags, 0, sizeof(int), c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
     if ((e = hipMemsetAsync(c->dflags + 8, 0, 8, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
-    hipLaunchKernelGGL(k_build_full, dim3(G), dim3(B), 0, c->stream, a, c->x, c->bkey_s, c->bin_start, c->nbr,
-                       c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
+    switch (c->launch.build_tpa) {
+      case 1: launch_build<1>(c); break;
+      case 4: launch_build<4>(c); break;
+      case 8: launch_build<8>(c); break;
+      case 16: launch_build<16>(c); break;
+      case 32: launch_build<32>(c); break;
+      default:
+        set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
+        return MDX_ERR_ARG;
+    }
     int flags[16];
     if ((e = hipMemcpyAsync(flags, c->dflags, 64, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
       return hip_fail(e, "read build flags");
@@ -372,13 +404,7 @@ int neigh_build(mdx_ctx *c) {
     set_error("neighbour list capacity could not be grown to fit");
     return MDX_ERR_OVERFLOW;
   }
-  if (c->profile) {
-    hipEventRecord(c->ev[1], c->stream);
-    hipEventSynchronize(c->ev[1]);
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-    c->ms_neigh += ms;
-  }
+  prof_end(c);
   c->built = true;
   c->ago = 0;
   return MDX_OK;

@@ -8,6 +8,14 @@
 // Energy and virial per atom are half of each pair term (pair.cpp:862-901,
 // ev_tally_full), and the global sums come from a fixed-order block tree.
 //
+// Threads per atom (TPA, a template parameter dividing 64): the TPA lanes of
+// an atom walk its row k = lane, lane+TPA, ...  Rows list neighbours in
+// bin-sorted order, so consecutive lanes gather consecutive atoms and one
+// 64-lane load touches a few cache lines instead of up to 64 (the vector
+// memory unit resolves about one line per clock, so this is what bounds a
+// thread-per-atom gather).  The TPA partial sums are combined with
+// cross-lane shuffles; lane 0 of the group writes the atom.
+//
 // Precision modes (template MIXED):
 //   FP64  : everything in double, as the reference.
 //   MIXED : positions, the image shift, del = x_i - x_j and rsq stay FP64
@@ -17,7 +25,7 @@
 //
 // EAM: the reference needs rho -> reverse_comm -> fp -> forward_comm ->
 // force (pair_eam.cpp:206,232).  With a full list rho_i is complete inside
-// one thread, so the fp pass is fused into the rho kernel and the only
+// one atom's lanes, so the fp pass is fused into the rho kernel and the only
 // grid-wide dependency left is fp[j] for the force kernel: two launches.
 #include "mdx_internal.h"
 
@@ -46,6 +54,14 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *__re
   }
 }
 
+// sum over the TPA lanes of a group; the result is valid in lane 0
+template <int TPA>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = TPA / 2; off > 0; off >>= 1) v += __shfl_down(v, off, TPA);
+  return v;
+}
+
 __device__ __forceinline__ void image_shift(int code, double px, double py, double pz, double &sx, double &sy,
                                             double &sz) {
   sx = ((code % 3) - 1) * px;
@@ -53,9 +69,51 @@ __device__ __forceinline__ void image_shift(int code, double px, double py, doub
   sz = ((code / 9) - 1) * pz;
 }
 
+// del = x_i - (x_j + image shift), the ghost-image geometry of the reference
+__device__ __forceinline__ void pair_delta(const double4 &xi, const double4 &xj, int code, double px, double py,
+                                           double pz, double &delx, double &dely, double &delz) {
+  double gx = xj.x, gy = xj.y, gz = xj.z;
+  if (code != MDX_NOSHIFT) {
+    double sx, sy, sz;
+    image_shift(code, px, py, pz, sx, sy, sz);
+    gx += sx;
+    gy += sy;
+    gz += sz;
+  }
+  delx = xi.x - gx;
+  dely = xi.y - gy;
+  delz = xi.z - gz;
+}
+
+// per-atom results of one group: lane 0 stores, all lanes join the block sum
+template <int TPA, bool EV>
+__device__ __forceinline__ void store_atom(int i, bool owner, int npad, double fx, double fy, double fz, double e,
+                                           double (&w)[6], double4 *__restrict__ f, double *__restrict__ vatom,
+                                           double *__restrict__ partials) {
+  fx = group_sum<TPA>(fx);
+  fy = group_sum<TPA>(fy);
+  fz = group_sum<TPA>(fz);
+  if (EV) {
+    e = group_sum<TPA>(e);
+#pragma unroll
+    for (int q = 0; q < 6; q++) w[q] = group_sum<TPA>(w[q]);
+  }
+  if (owner) {
+    f[i] = make_double4(fx, fy, fz, e);
+    if (EV)
+#pragma unroll
+      for (int q = 0; q < 6; q++) vatom[(size_t)q * npad + i] = w[q];
+  }
+  if (EV) {
+    double v[7] = {owner ? e : 0.0, owner ? w[0] : 0.0, owner ? w[1] : 0.0, owner ? w[2] : 0.0,
+                   owner ? w[3] : 0.0,   owner ? w[4] : 0.0, owner ? w[5] : 0.0};
+    block_reduce_store<7>(v, partials);
+  }
+}
+
 // ------------------------------------------------------------------ LJ
-template <bool MIXED, bool EV>
-__global__ __launch_bounds__(256) void k_lj(int n, int npad, int nt1, double px, double py, double pz,
+template <bool MIXED, bool EV, int TPA>
+__global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
                                            const double4 *__restrict__ x, const int *__restrict__ nbr,
                                            const int *__restrict__ numneigh, const LJTab *__restrict__ tab,
                                            double4 *__restrict__ f, double *__restrict__ vatom,
@@ -77,28 +135,22 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int nt1, double px,
     f_off[q] = (float)tab->offset[q];
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  const int lane = threadIdx.x % TPA;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+  double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     const double4 xi = x[i];
     const int row = static_cast<int>(xi.w) * nt1;
     const int jn = numneigh[i];
-    const int *nb = nbr + i;
-    for (int k = 0; k < jn; k++) {
-      const int jc = nb[(size_t)k * npad];
+    const int *nb = nbr + (size_t)i * cap;
+    for (int k = lane; k < jn; k += TPA) {
+      const int jc = nb[k];
       const int j = jc & MDX_IDX_MASK;
       const int code = (unsigned)jc >> MDX_IDX_BITS;
       const double4 xj = x[j];
-      double gx = xj.x, gy = xj.y, gz = xj.z;
-      if (code != MDX_NOSHIFT) {
-        double sx, sy, sz;
-        image_shift(code, px, py, pz, sx, sy, sz);
-        gx += sx;
-        gy += sy;
-        gz += sz;
-      }
-      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      double delx, dely, delz;
+      pair_delta(xi, xj, code, px, py, pz, delx, dely, delz);
       const double rsq = delx * delx + dely * dely + delz * delz;
       const int t = row + static_cast<int>(xj.w);
       if (rsq < s_cutsq[t]) {
@@ -122,29 +174,17 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int nt1, double px,
         if (EV) {
           e += 0.5 * evdwl;
           const double hf = 0.5 * fpair;
-          w0 += delx * delx * hf;
-          w1 += dely * dely * hf;
-          w2 += delz * delz * hf;
-          w3 += delx * dely * hf;
-          w4 += delx * delz * hf;
-          w5 += dely * delz * hf;
+          w[0] += delx * delx * hf;
+          w[1] += dely * dely * hf;
+          w[2] += delz * delz * hf;
+          w[3] += delx * dely * hf;
+          w[4] += delx * delz * hf;
+          w[5] += dely * delz * hf;
         }
       }
     }
-    f[i] = make_double4(fx, fy, fz, e);
-    if (EV) {
-      vatom[i] = w0;
-      vatom[(size_t)npad + i] = w1;
-      vatom[(size_t)2 * npad + i] = w2;
-      vatom[(size_t)3 * npad + i] = w3;
-      vatom[(size_t)4 * npad + i] = w4;
-      vatom[(size_t)5 * npad + i] = w5;
-    }
   }
-  if (EV) {
-    double v[7] = {e, w0, w1, w2, w3, w4, w5};
-    block_reduce_store<7>(v, partials);
-  }
+  store_atom<TPA, EV>(i, i < n && lane == 0, npad, fx, fy, fz, e, w, f, vatom, partials);
 }
 
 // ------------------------------------------------------------------ EAM
@@ -158,33 +198,28 @@ __device__ __forceinline__ void spline_index(T r, T rdr, int nr, int &m, T &p) {
   p = p < T(1) ? p : T(1);
 }
 
-template <bool MIXED, bool EV>
-__global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, double px, double py, double pz, EAMArgs a,
+template <bool MIXED, bool EV, int TPA>
+__global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, double px, double py, double pz, EAMArgs a,
                                                 const double4 *__restrict__ x, const int *__restrict__ nbr,
                                                 const int *__restrict__ numneigh, double *__restrict__ rho_out,
                                                 double *__restrict__ fp_out, double *__restrict__ eemb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  const int lane = threadIdx.x % TPA;
+  double rho = 0.0;
+  int ti = 0;
   if (i < n) {
     const double4 xi = x[i];
-    const int ti = static_cast<int>(xi.w);
+    ti = static_cast<int>(xi.w);
     const int jn = numneigh[i];
-    const int *nb = nbr + i;
+    const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
-    double rho = 0.0;
-    for (int k = 0; k < jn; k++) {
-      const int jc = nb[(size_t)k * npad];
+    for (int k = lane; k < jn; k += TPA) {
+      const int jc = nb[k];
       const int j = jc & MDX_IDX_MASK;
       const int code = (unsigned)jc >> MDX_IDX_BITS;
       const double4 xj = x[j];
-      double gx = xj.x, gy = xj.y, gz = xj.z;
-      if (code != MDX_NOSHIFT) {
-        double sx, sy, sz;
-        image_shift(code, px, py, pz, sx, sy, sz);
-        gx += sx;
-        gy += sy;
-        gz += sz;
-      }
-      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      double delx, dely, delz;
+      pair_delta(xi, xj, code, px, py, pz, delx, dely, delz);
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < a.cutforcesq) {
         const int tj = static_cast<int>(xj.w);
@@ -203,6 +238,9 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, double px, dou
         }
       }
     }
+  }
+  rho = group_sum<TPA>(rho);
+  if (i < n && lane == 0) {
     // fp pass (pair_eam.cpp:213-228), FP64 in both modes: one per atom
     double p = rho * a.rdrho + 1.0;
     int m = static_cast<int>(p);
@@ -221,36 +259,31 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, double px, dou
   }
 }
 
-template <bool MIXED, bool EV>
-__global__ __launch_bounds__(256) void k_eam_force(int n, int npad, double px, double py, double pz, EAMArgs a,
-                                                  const double4 *__restrict__ x, const int *__restrict__ nbr,
-                                                  const int *__restrict__ numneigh, const double *__restrict__ fp,
-                                                  const double *__restrict__ eemb, double4 *__restrict__ f,
-                                                  double *__restrict__ vatom, double *__restrict__ partials) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+template <bool MIXED, bool EV, int TPA>
+__global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
+                                                  EAMArgs a, const double4 *__restrict__ x,
+                                                  const int *__restrict__ nbr, const int *__restrict__ numneigh,
+                                                  const double *__restrict__ fp, const double *__restrict__ eemb,
+                                                  double4 *__restrict__ f, double *__restrict__ vatom,
+                                                  double *__restrict__ partials) {
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  const int lane = threadIdx.x % TPA;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
+  double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     const double4 xi = x[i];
     const int ti = static_cast<int>(xi.w);
     const double fpi = fp[i];
     const int jn = numneigh[i];
-    const int *nb = nbr + i;
+    const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
-    for (int k = 0; k < jn; k++) {
-      const int jc = nb[(size_t)k * npad];
+    for (int k = lane; k < jn; k += TPA) {
+      const int jc = nb[k];
       const int j = jc & MDX_IDX_MASK;
       const int code = (unsigned)jc >> MDX_IDX_BITS;
       const double4 xj = x[j];
-      double gx = xj.x, gy = xj.y, gz = xj.z;
-      if (code != MDX_NOSHIFT) {
-        double sx, sy, sz;
-        image_shift(code, px, py, pz, sx, sy, sz);
-        gx += sx;
-        gy += sy;
-        gz += sz;
-      }
-      const double delx = xi.x - gx, dely = xi.y - gy, delz = xi.z - gz;
+      double delx, dely, delz;
+      pair_delta(xi, xj, code, px, py, pz, delx, dely, delz);
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < a.cutforcesq) {
         const int tj = static_cast<int>(xj.w);
@@ -298,86 +331,81 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, double px, d
         if (EV) {
           e += 0.5 * phi;
           const double hf = 0.5 * fpair;
-          w0 += delx * delx * hf;
-          w1 += dely * dely * hf;
-          w2 += delz * delz * hf;
-          w3 += delx * dely * hf;
-          w4 += delx * delz * hf;
-          w5 += dely * delz * hf;
+          w[0] += delx * delx * hf;
+          w[1] += dely * dely * hf;
+          w[2] += delz * delz * hf;
+          w[3] += delx * dely * hf;
+          w[4] += delx * delz * hf;
+          w[5] += dely * delz * hf;
         }
       }
     }
-    if (EV) e += eemb[i];
-    f[i] = make_double4(fx, fy, fz, e);
-    if (EV) {
-      vatom[i] = w0;
-      vatom[(size_t)npad + i] = w1;
-      vatom[(size_t)2 * npad + i] = w2;
-      vatom[(size_t)3 * npad + i] = w3;
-      vatom[(size_t)4 * npad + i] = w4;
-      vatom[(size_t)5 * npad + i] = w5;
-    }
+    if (EV && lane == 0) e += eemb[i];
   }
-  if (EV) {
-    double v[7] = {e, w0, w1, w2, w3, w4, w5};
-    block_reduce_store<7>(v, partials);
-  }
+  store_atom<TPA, EV>(i, i < n && lane == 0, npad, fx, fy, fz, e, w, f, vatom, partials);
+}
+
+template <bool MIXED, bool EV, int TPA>
+void launch_lj(mdx_ctx *c, int B) {
+  const int G = (int)(((long)c->n * TPA + B - 1) / B);
+  prof_begin(c, MDX_K_LJ);
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+                     c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->nbr, c->numneigh, c->lj_dev, c->f,
+                     c->vatom, c->partials);
+  prof_end(c);
+  c->last_grid = G;
+}
+
+template <bool MIXED, bool EV, int TPA>
+void launch_eam(mdx_ctx *c, int B) {
+  const int G = (int)(((long)c->n * TPA + B - 1) / B);
+  prof_begin(c, MDX_K_EAM_RHO);
+  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+                     c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
+  prof_end(c);
+  prof_begin(c, MDX_K_EAM_FORCE);
+  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+                     c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->fp, c->eemb, c->f,
+                     c->vatom, c->partials);
+  prof_end(c);
+  c->last_grid = G;
 }
 
 template <bool MIXED, bool EV>
-void launch_lj(mdx_ctx *c, int G, int B) {
-  hipLaunchKernelGGL((k_lj<MIXED, EV>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->ntypes + 1, c->prd[0],
-                     c->prd[1], c->prd[2], c->x, c->nbr, c->numneigh, c->lj_dev, c->f, c->vatom, c->partials);
-}
-
-template <bool MIXED, bool EV>
-void launch_eam(mdx_ctx *c, int G, int B) {
-  hipLaunchKernelGGL((k_eam_rho<MIXED, EV>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->prd[0], c->prd[1],
-                     c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
-  hipLaunchKernelGGL((k_eam_force<MIXED, EV>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->prd[0],
-                     c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
-                     c->partials);
+int dispatch_tpa(mdx_ctx *c, int B) {
+  const bool lj = c->pair_style == MDX_PAIR_LJ;
+  switch (c->launch.tpa) {
+    case 1: lj ? launch_lj<MIXED, EV, 1>(c, B) : launch_eam<MIXED, EV, 1>(c, B); break;
+    case 2: lj ? launch_lj<MIXED, EV, 2>(c, B) : launch_eam<MIXED, EV, 2>(c, B); break;
+    case 4: lj ? launch_lj<MIXED, EV, 4>(c, B) : launch_eam<MIXED, EV, 4>(c, B); break;
+    case 8: lj ? launch_lj<MIXED, EV, 8>(c, B) : launch_eam<MIXED, EV, 8>(c, B); break;
+    case 16: lj ? launch_lj<MIXED, EV, 16>(c, B) : launch_eam<MIXED, EV, 16>(c, B); break;
+    case 32: lj ? launch_lj<MIXED, EV, 32>(c, B) : launch_eam<MIXED, EV, 32>(c, B); break;
+    default:
+      set_error("threads per atom must be one of 1, 2, 4, 8, 16, 32");
+      return MDX_ERR_ARG;
+  }
+  return MDX_OK;
 }
 
 }  // namespace
 
 int pair_compute(mdx_ctx *c, int ev) {
   if (c->n == 0) return MDX_OK;
-  const int B = c->launch.block;
-  const int G = (c->n + B - 1) / B;
-  if (c->profile) hipEventRecord(c->ev[2], c->stream);
-  const bool mixed = c->precision == MDX_MIXED;
-  if (c->pair_style == MDX_PAIR_LJ) {
-    if (mixed) {
-      if (ev) launch_lj<true, true>(c, G, B);
-      else launch_lj<true, false>(c, G, B);
-    } else {
-      if (ev) launch_lj<false, true>(c, G, B);
-      else launch_lj<false, false>(c, G, B);
-    }
-  } else if (c->pair_style == MDX_PAIR_EAM) {
-    if (mixed) {
-      if (ev) launch_eam<true, true>(c, G, B);
-      else launch_eam<true, false>(c, G, B);
-    } else {
-      if (ev) launch_eam<false, true>(c, G, B);
-      else launch_eam<false, false>(c, G, B);
-    }
-  } else {
+  if (c->pair_style != MDX_PAIR_LJ && c->pair_style != MDX_PAIR_EAM) {
     set_error("no pair style set");
     return MDX_ERR_ARG;
   }
+  const int B = c->launch.block;
+  const bool mixed = c->precision == MDX_MIXED;
+  int rc;
+  if (mixed) rc = ev ? dispatch_tpa<true, true>(c, B) : dispatch_tpa<true, false>(c, B);
+  else rc = ev ? dispatch_tpa<false, true>(c, B) : dispatch_tpa<false, false>(c, B);
+  if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
-  if (c->profile) {
-    hipEventRecord(c->ev[3], c->stream);
-    hipEventSynchronize(c->ev[3]);
-    float ms = 0;
-    hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-    c->ms_pair += ms;
-  }
   // the per-atom energy in f.w already includes the EAM embedding term
-  if (ev) return finish_reduction(c, G, 7);
+  if (ev) return finish_reduction(c, c->last_grid, 7);
   return MDX_OK;
 }
 

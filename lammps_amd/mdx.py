@@ -27,7 +27,10 @@ THERMO_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_long, ctypes.POINTE
 class Stats(ctypes.Structure):
     _fields_ = [("ncalls", ctypes.c_long), ("ndanger", ctypes.c_long), ("total_neighbors", ctypes.c_long),
                 ("max_neighbors", ctypes.c_int), ("capacity", ctypes.c_int), ("nbins", ctypes.c_int * 3),
-                ("ms_neigh", ctypes.c_double), ("ms_pair", ctypes.c_double), ("ms_integrate", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double * 8), ("kernel_launches", ctypes.c_long * 8)]
+
+
+K_BUILD, K_LJ, K_EAM_RHO, K_EAM_FORCE, K_NVE_INITIAL, K_NVE_FINAL, K_FUSED = range(7)
 
 
 class MdxError(RuntimeError):
@@ -62,6 +65,7 @@ def lib():
             "mdx_run": [_p, _i, _d, _d, _i, THERMO_FN, _p],
             "mdx_get_stats": [_p, ctypes.POINTER(Stats)],
             "mdx_set_launch": [_p, _i, _i],
+            "mdx_set_build_launch": [_p, _i],
             "mdx_set_profiling": [_p, _i],
         }
         for name, args in sig.items():
@@ -198,6 +202,9 @@ class Context:
     def set_launch(self, block=0, tpa=0):
         self._ck(self._L.mdx_set_launch(self.h, block, tpa))
 
+    def set_build_launch(self, tpa=0):
+        self._ck(self._L.mdx_set_build_launch(self.h, tpa))
+
     def profiling(self, on=True):
         self._ck(self._L.mdx_set_profiling(self.h, 1 if on else 0))
 
@@ -240,7 +247,101 @@ class Context:
     def stats(self):
         s = Stats()
         self._ck(self._L.mdx_get_stats(self.h, ctypes.byref(s)))
-        return {k: (list(getattr(s, k)) if k == "nbins" else getattr(s, k)) for k, _ in Stats._fields_}
+        return {k: (list(getattr(s, k)) if k in ("nbins", "kernel_ms", "kernel_launches") else getattr(s, k))
+                for k, _ in Stats._fields_}
+
+
+LATTICE = {"sc": 0, "bcc": 1, "fcc": 2}
+UNITS = {"lj": 0, "metal": 1}
+# update.cpp:118-180
+UNIT_CONSTANTS = {"lj": dict(boltz=1.0, mvv2e=1.0, ftm2v=1.0, nktv2p=1.0),
+                  "metal": dict(boltz=8.617343e-5, mvv2e=1.0364269e-4, ftm2v=1.0 / 1.0364269e-4,
+                                nktv2p=1.6021765e6)}
+
+
+class EamInfo(ctypes.Structure):
+    _fields_ = [("nrho", ctypes.c_int), ("nr", ctypes.c_int), ("nfrho", ctypes.c_int), ("nrhor", ctypes.c_int),
+                ("nz2r", ctypes.c_int), ("drho", ctypes.c_double), ("dr", ctypes.c_double),
+                ("rhomax", ctypes.c_double), ("cutmax", ctypes.c_double), ("mass", ctypes.c_double)]
+
+
+def _host_lib():
+    L = lib()
+    if not getattr(L, "_host_ready", False):
+        L.mdx_host_lattice.argtypes = [_i, _d, _i, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_long,
+                                       ctypes.POINTER(ctypes.c_long)]
+        L.mdx_host_lattice.restype = _i
+        L.mdx_host_velocity.argtypes = [ctypes.c_long, _dp, _ip, _dp, _d, _i, _i, _i, _i, _d, _d, _dp]
+        L.mdx_host_velocity.restype = _i
+        L.mdx_host_eam_funcfl.argtypes = [ctypes.c_char_p, ctypes.POINTER(EamInfo), _dp, _dp, _dp]
+        L.mdx_host_eam_funcfl.restype = _i
+        L.mdx_host_write_synthetic_funcfl.argtypes = [ctypes.c_char_p, _i, _d, _i, _i, _d, _d, _d, _dp]
+        L.mdx_host_write_synthetic_funcfl.restype = _i
+        L.mdx_host_ranpark_uniform.argtypes = [_ip]
+        L.mdx_host_ranpark_uniform.restype = _d
+        L._host_ready = True
+    return L
+
+
+def _host_ck(rc):
+    if rc != 0:
+        raise MdxError(rc, lib().mdx_last_error().decode())
+
+
+def lattice(style, arg, units, lo, hi, periodic=(1, 1, 1)):
+    """create_atoms on a lattice: returns (boxlo, boxhi, x[n][3]) in atom-ID order."""
+    L = _host_lib()
+    lo, hi, per = _i32(lo), _i32(hi), _i32(periodic)
+    blo, bhi = np.zeros(3), np.zeros(3)
+    n = ctypes.c_long(0)
+    _host_ck(L.mdx_host_lattice(LATTICE[style], arg, UNITS[units], _ptr(lo, _ip), _ptr(hi, _ip), _ptr(per, _ip),
+                                _ptr(blo), _ptr(bhi), None, 0, ctypes.byref(n)))
+    x = np.zeros((n.value, 3))
+    _host_ck(L.mdx_host_lattice(LATTICE[style], arg, UNITS[units], _ptr(lo, _ip), _ptr(hi, _ip), _ptr(per, _ip),
+                                _ptr(blo), _ptr(bhi), _ptr(x), n.value, ctypes.byref(n)))
+    return blo, bhi, x
+
+
+def velocity_create(x, types, mass, temp, seed, dist="uniform", loop="all", mom=True, units="lj"):
+    L = _host_lib()
+    x, t, mass = _f64(x), _i32(types), _f64(mass)
+    v = np.zeros_like(x)
+    u = UNIT_CONSTANTS[units]
+    _host_ck(L.mdx_host_velocity(len(t), _ptr(x), _ptr(t, _ip), _ptr(mass), temp, seed,
+                                 1 if dist == "gaussian" else 0, 1 if loop == "geom" else 0, 1 if mom else 0,
+                                 u["boltz"], u["mvv2e"], _ptr(v)))
+    return v
+
+
+def eam_funcfl(path):
+    """PairEAM tables of one funcfl file: dict with the sizes and the
+    frho/rhor/z2r spline arrays ([k][n+1][7])."""
+    L = _host_lib()
+    info = EamInfo()
+    _host_ck(L.mdx_host_eam_funcfl(path.encode(), ctypes.byref(info), None, None, None))
+    fr = np.zeros((info.nfrho, info.nrho + 1, 7))
+    rr = np.zeros((info.nrhor, info.nr + 1, 7))
+    zr = np.zeros((info.nz2r, info.nr + 1, 7))
+    _host_ck(L.mdx_host_eam_funcfl(path.encode(), ctypes.byref(info), _ptr(fr), _ptr(rr), _ptr(zr)))
+    return dict(nrho=info.nrho, nr=info.nr, drho=info.drho, dr=info.dr, rhomax=info.rhomax, cutmax=info.cutmax,
+                mass=info.mass, frho_spline=fr, rhor_spline=rr, z2r_spline=zr)
+
+
+def write_synthetic_funcfl(path, seed=4951, a=3.615, nrho=500, nr=500, dr=0.01, cut=4.95, mass=63.55):
+    L = _host_lib()
+    params = np.zeros(6)
+    _host_ck(L.mdx_host_write_synthetic_funcfl(path.encode(), seed, a, nrho, nr, dr, cut, mass, _ptr(params)))
+    return dict(zip(("Z0", "beta", "chi", "E0", "re", "rho_e"), params.tolist()))
+
+
+def eam_context_tables(ctx, path, ntypes=1):
+    """pair_style eam + pair_coeff * * path on a context; returns the tables."""
+    t = eam_funcfl(path)
+    nt1 = ntypes + 1
+    ctx.pair_eam(ntypes, t["nrho"], t["drho"], t["nr"], t["dr"], t["rhomax"], t["cutmax"] ** 2, t["frho_spline"],
+                 t["rhor_spline"], t["z2r_spline"], np.zeros(nt1, dtype=np.int32),
+                 np.zeros(nt1 * nt1, dtype=np.int32), np.zeros(nt1 * nt1, dtype=np.int32))
+    return t
 
 
 def thermo_columns(raw, natoms, prd, units="lj"):

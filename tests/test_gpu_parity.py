@@ -83,6 +83,27 @@ def test_lj_forces_energy_virial(lj_state, prec):
     np.testing.assert_allclose(out["vatom"], va_ref, rtol=0, atol=100 * etol * np.abs(va_ref).max())
 
 
+@pytest.mark.parametrize("tpa", [1, 2, 4, 8, 16, 32])
+def test_lj_threads_per_atom_variants_agree(lj_state, tpa):
+    """Every pair-kernel shape gives the oracle's forces, and every build
+    shape gives the identical list (the ballot compaction keeps scan order)."""
+    from lammps_amd import mdx
+    r = lj_state
+    lo, hi = box_of(r, 20, A_LJ)
+    ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+    ctx.set_launch(128 if tpa in (2, 16) else 256, tpa)
+    ctx.set_build_launch(tpa if tpa != 2 else 1)
+    out = ctx.compute(1, 1)
+    _, _, fref, _, _ = r.state()
+    eref, _ = r.energy_virial()
+    mx, _ = force_errors(out["f"], fref)
+    assert mx <= 1e-11
+    assert abs(out["eng"] - eref) <= 1e-11 * abs(eref)
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
+    assert got == ref
+
+
 def test_eam_neighbor_pairs_identical(eam_state):
     from lammps_amd import mdx
     r = eam_state
