@@ -100,6 +100,22 @@ def run_case(kind, inp, precision, steps, warmup):
     return res
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "lj2m_eam256k_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(kernel + " ") or line.startswith("void " + kernel + " "):
+                try:
+                    return float(line.split()[-1]) * 1e6
+                except ValueError:
+                    return None
+    return None
+
+
 def energy_per_atom(raw, n, units):
     from lammps_amd import mdx
     mvv2e = mdx.UNIT_CONSTANTS[units]["mvv2e"]
@@ -229,6 +245,12 @@ def main():
                              builds=eres["fp64"]["builds"]),
                    mixed_over_fp64=eres["mixed"]["value"] / eres["fp64"]["value"])
 
+    # measured streaming-copy rate of this device (next to the 8 TB/s spec)
+    probe = mdx.Context(0)
+    copy_gbs = probe.copy_bandwidth(1 << 30, 20)
+    probe.close()
+    log("HBM copy probe: %.0f GB/s" % copy_gbs)
+
     # roofline of the dominant kernel (LJ force, mixed): algorithmic bytes per
     # launch = N * (4*(n_h+1) half-list indices + 32 B x_i + 24 B f_i)
     km = res["mixed"]["kernel_ms"]
@@ -238,9 +260,13 @@ def main():
     bytes_per_launch = n * (4.0 * (n_h + 1.0) + 32.0 + 24.0)
     achieved = bytes_per_launch / (lj_ms * 1e-3) / 1e9
     total_ms = res["mixed"]["seconds"] * 1e3
+    kname = "k_lj<true, false, %d>" % 8
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_lj<mixed>", avg_launch_ms=lj_ms,
-                    bytes_per_launch=bytes_per_launch, n_half_neighbors_per_atom=n_h)
+                    traffic=profiled_traffic(kname), kernel=kname, avg_launch_ms=lj_ms,
+                    bytes_per_launch=bytes_per_launch, n_half_neighbors_per_atom=n_h,
+                    measured_copy_gbs=copy_gbs,
+                    step_algorithmic_frac=(n * (4.0 * (n_h + 1) + 32 + 24 + 200 + 16 + (4.0 * n_h + 32) / 20.0)
+                                           / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9) / HBM_PEAK_GBS)
     shares = {name: km[k] / total_ms for name, k in
               (("neigh_build", mdx.K_BUILD), ("pair", mdx.K_LJ), ("nve_initial", mdx.K_NVE_INITIAL),
                ("nve_final", mdx.K_NVE_FINAL))}

@@ -38,6 +38,9 @@ enum {
 };
 
 enum { MDX_FP64 = 0, MDX_MIXED = 1 };        /* pair arithmetic precision */
+/* neighbour-list layout: global 32-bit indices read by per-neighbour gathers,
+   or 16-bit slots into LDS-staged bricks of bins (the default) */
+enum { MDX_LIST_GATHER = 0, MDX_LIST_BRICK = 1 };
 enum { MDX_PAIR_NONE = 0, MDX_PAIR_LJ = 1, MDX_PAIR_EAM = 2 };
 
 typedef struct mdx_ctx mdx_ctx;
@@ -160,6 +163,11 @@ int mdx_get_stats(mdx_ctx *ctx, mdx_stats *out);
    are resolved by mdx_get_stats).  Resets the accumulated times. */
 int mdx_set_profiling(mdx_ctx *ctx, int on);
 
+/* HBM calibration probe (no counterpart in the reference): copies nbytes
+   device-to-device `iters` times with a streaming kernel (k_copy) on the
+   context's stream and returns the achieved read+write rate in GB/s. */
+int mdx_copy_bandwidth(mdx_ctx *ctx, long nbytes, int iters, double *gbs);
+
 /* Kernel shape of the pair kernels, set by the autotuner (0 = library
    default): block = threads per workgroup (64..256, multiple of 64), tpa =
    threads per atom (1, 2, 4, 8, 16 or 32; the `threads_per_atom` keyword of
@@ -167,6 +175,10 @@ int mdx_set_profiling(mdx_ctx *ctx, int on);
 int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
 /* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default). */
 int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
+/* List layout (MDX_LIST_GATHER or MDX_LIST_BRICK) and, for bricks, the
+   brick size in bins (0 = default 8x4x4; shrunk automatically until the
+   staged halo fits the LDS).  Takes effect at the next list build. */
+int mdx_set_list_mode(mdx_ctx *ctx, int mode, int rx, int ry, int rz);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdx.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["api.cpp", "host_setup.cpp", "neigh.hip", "pair.hip", "integrate.hip"]
+SOURCES = ["api.cpp", "host_setup.cpp", "neigh.hip", "brick.hip", "pair.hip", "integrate.hip"]
 ARCH = "gfx950"
 
 

@@ -90,7 +90,7 @@ int check_ctx(mdx_ctx *c) {
 void free_atoms(mdx_ctx *c) {
   void *ptrs[] = {c->x, c->xb, c->v, c->vb, c->f, c->xhold, c->tag, c->tagb, c->orig, c->origb, c->image,
                   c->imageb, c->inv, c->vatom, c->dscratch, c->bkey, c->bkey_s, c->bidx, c->bidx_s,
-                  c->numneigh, c->nbr, c->rho, c->fp, c->eemb, c->partials};
+                  c->numneigh, c->nbr, c->nbr16, c->rho, c->fp, c->eemb, c->partials};
   for (void *p : ptrs)
     if (p) hipFree(p);
   c->x = c->xb = c->v = c->vb = c->f = c->xhold = nullptr;
@@ -98,6 +98,7 @@ void free_atoms(mdx_ctx *c) {
   c->vatom = c->dscratch = nullptr;
   c->bkey = c->bkey_s = nullptr;
   c->bidx = c->bidx_s = c->numneigh = c->nbr = nullptr;
+  c->nbr16 = nullptr;
   c->rho = c->fp = c->eemb = c->partials = nullptr;
   if (c->host_stage) hipHostFree(c->host_stage);
   if (c->host_istage) hipHostFree(c->host_istage);
@@ -138,14 +139,13 @@ int prepare(mdx_ctx *c) {
   int rc = neigh_setup_grid(c);
   if (rc) return rc;
   // first-guess list capacity from the mean density; neigh_build regrows
-  if (!c->nbr && c->n > 0) {
+  const bool have_list = c->list_mode == MDX_LIST_BRICK ? c->nbr16 != nullptr : c->nbr != nullptr;
+  if (!have_list && c->n > 0) {
     const double vol = c->prd[0] * c->prd[1] * c->prd[2];
     const double rc3 = c->cutneighmax * c->cutneighmax * c->cutneighmax;
     int guess = (int)(1.5 * 4.18879 * rc3 * c->n / vol) + 16;
     guess = std::min(((guess + 15) / 16) * 16, c->one);
-    if ((rc = dev_alloc(&c->nbr, (size_t)guess * c->npad))) return rc;
-    c->maxneigh = guess;
-    c->nbr_elems = (long)guess * c->npad;
+    if ((rc = alloc_list(c, guess))) return rc;
   }
   // enough blocks for any launch: block >= 64 threads, up to 32 threads per atom
   const long G = ((long)c->n * 32 + 63) / 64 + 1;
@@ -201,6 +201,11 @@ int mdx_create(int device, mdx_ctx **out) {
     return MDX_ERR_NOMEM;
   }
   hipMemset(c->dflags, 0, 64);
+  // LDS available to the brick kernels' dynamic allocation, minus their
+  // static tables and reductions
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 16384)
+    c->lds_budget = prop.sharedMemPerBlock - 8192;
   *out = c;
   return MDX_OK;
 }
@@ -499,17 +504,39 @@ int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
   }
   const int n = c->n;
   std::vector<int> num(n), tag(n);
-  std::vector<int> nb((size_t)c->maxneigh * c->npad);
   hipError_t e;
   if (n > 0 &&
       ((e = hipMemcpy(num.data(), c->numneigh, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
-       (e = hipMemcpy(tag.data(), c->tag, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
-       (e = hipMemcpy(nb.data(), c->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost)) != hipSuccess))
+       (e = hipMemcpy(tag.data(), c->tag, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess))
     return hip_fail(e, "list copy");
+  // global neighbour index of entry k of atom i
+  std::vector<int> nb;
+  std::vector<unsigned short> nb16;
+  std::vector<std::vector<int>> slotmap;  // brick mode: per brick, LDS slot -> global index
+  std::vector<int> brick_of;              // brick mode: brick of each atom
+  if (c->list_mode == MDX_LIST_BRICK) {
+    nb16.resize((size_t)c->maxneigh * c->npad);
+    std::vector<unsigned> bk(n);
+    std::vector<int> bs(c->nbins + 1);
+    if ((e = hipMemcpy(nb16.data(), c->nbr16, sizeof(unsigned short) * nb16.size(), hipMemcpyDeviceToHost)) !=
+            hipSuccess ||
+        (e = hipMemcpy(bk.data(), c->bkey_s, sizeof(unsigned) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(bs.data(), c->bin_start, sizeof(int) * bs.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(e, "list copy");
+    slotmap.resize(c->br.nbricks);
+    brick_of.resize(n);
+    for (int i = 0; i < n; i++) brick_of[i] = brick_of_bin(c->br, c->na, (int)bk[i]);
+    for (int b = 0; b < c->br.nbricks; b++) brick_slot_map(c->na, c->br, b, bs.data(), slotmap[b]);
+  } else {
+    nb.resize((size_t)c->maxneigh * c->npad);
+    if ((e = hipMemcpy(nb.data(), c->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(e, "list copy");
+  }
   long cnt = 0;
   for (int i = 0; i < n; i++)
     for (int k = 0; k < num[i]; k++) {
-      const int j = nb[(size_t)i * c->maxneigh + k] & MDX_IDX_MASK;
+      const size_t q = (size_t)i * c->maxneigh + k;
+      const int j = c->list_mode == MDX_LIST_BRICK ? slotmap[brick_of[i]].at(nb16[q]) : (nb[q] & MDX_IDX_MASK);
       if (tag[i] < tag[j]) {
         if (pairs && cnt < cap) {
           pairs[2 * cnt] = tag[i];
@@ -671,6 +698,33 @@ int mdx_set_launch(mdx_ctx *c, int block, int tpa) {
   return MDX_OK;
 }
 
+int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (mode != MDX_LIST_GATHER && mode != MDX_LIST_BRICK) {
+    set_error("list mode must be MDX_LIST_GATHER or MDX_LIST_BRICK");
+    return MDX_ERR_ARG;
+  }
+  if (rx < 0 || ry < 0 || rz < 0 || rx > 64 || ry > 64 || rz > 64) {
+    set_error("brick dimensions must be 0 (default) or 1..64 bins");
+    return MDX_ERR_ARG;
+  }
+  if (mode != c->list_mode) {
+    hipFree(c->nbr);
+    hipFree(c->nbr16);
+    c->nbr = nullptr;
+    c->nbr16 = nullptr;
+    c->maxneigh = 0;
+  }
+  c->list_mode = mode;
+  const int def[3] = {8, 4, 4};
+  c->brick_req[0] = rx ? rx : def[0];
+  c->brick_req[1] = ry ? ry : def[1];
+  c->brick_req[2] = rz ? rz : def[2];
+  c->built = false;
+  return MDX_OK;
+}
+
 int mdx_set_build_launch(mdx_ctx *c, int tpa) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -680,6 +734,16 @@ int mdx_set_build_launch(mdx_ctx *c, int tpa) {
   }
   c->launch.build_tpa = tpa ? tpa : 1;
   return MDX_OK;
+}
+
+int mdx_copy_bandwidth(mdx_ctx *c, long nbytes, int iters, double *gbs) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!gbs) {
+    set_error("null output pointer");
+    return MDX_ERR_ARG;
+  }
+  return copy_probe(c, nbytes, iters, gbs);
 }
 
 int mdx_set_profiling(mdx_ctx *c, int on) {

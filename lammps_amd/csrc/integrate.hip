@@ -140,6 +140,13 @@ __global__ void k_gather_soa(int n, int npad, int ncols, const int *__restrict__
   for (int c = 0; c < ncols; c++) out[(size_t)o * ncols + c] = src[(size_t)c * npad + i];
 }
 
+// streaming copy of known size (HBM calibration: bytes moved = 2 * bytes)
+__global__ __launch_bounds__(256) void k_copy(long n4, const double4 *__restrict__ src,
+                                             double4 *__restrict__ dst) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n4; k += stride) dst[k] = src[k];
+}
+
 MassTab mass_tab(const mdx_ctx *c) {
   MassTab t;
   for (int k = 0; k < MDX_NT1; k++) t.m[k] = c->mass[k];
@@ -222,6 +229,39 @@ int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out) {
   if (e != hipSuccess) return hip_fail(e, "gather copy");
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "gather sync");
   memcpy(host_out, c->host_stage, sizeof(double) * 3 * c->n);
+  return MDX_OK;
+}
+
+int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs) {
+  const long n4 = nbytes / (long)sizeof(double4);
+  if (n4 <= 0 || iters <= 0) {
+    set_error("copy probe needs nbytes >= 32 and iters >= 1");
+    return MDX_ERR_ARG;
+  }
+  double4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, sizeof(double4) * n4) != hipSuccess || hipMalloc(&b, sizeof(double4) * n4) != hipSuccess) {
+    if (a) hipFree(a);
+    set_error("Insufficient memory on accelerator (copy probe)");
+    return MDX_ERR_NOMEM;
+  }
+  hipMemsetAsync(a, 0, sizeof(double4) * n4, c->stream);
+  const int B = 256, G = 256 * 32;  // 32 workgroups per CU on 256 CUs
+  hipLaunchKernelGGL(k_copy, dim3(G), dim3(B), 0, c->stream, n4, a, b);  // warm-up
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, c->stream);
+  for (int it = 0; it < iters; it++) hipLaunchKernelGGL(k_copy, dim3(G), dim3(B), 0, c->stream, n4, a, b);
+  hipEventRecord(e1, c->stream);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return hip_fail(e, "copy probe");
+  *gbs = 2.0 * sizeof(double4) * (double)n4 * iters / (ms * 1e-3) / 1e9;
   return MDX_OK;
 }
 

@@ -55,6 +55,15 @@ struct NeighArgs {
   RefBins ref;
 };
 
+// LDS-staged bricks of bins (brick.hip)
+struct BrickArgs {
+  int R[3];        // brick size in bins
+  int nbrick[3];   // bricks per dimension
+  int nbricks;
+  int max_slots;   // largest number of staged (halo) atoms over all bricks
+  int cap;         // neighbour-list row capacity (entries)
+};
+
 struct LJTab {  // (ntypes+1)^2 row-major, as Pair::init_one fills them
   double cutsq[MDX_NT1 * MDX_NT1], lj1[MDX_NT1 * MDX_NT1], lj2[MDX_NT1 * MDX_NT1],
       lj3[MDX_NT1 * MDX_NT1], lj4[MDX_NT1 * MDX_NT1], offset[MDX_NT1 * MDX_NT1];
@@ -108,7 +117,14 @@ struct mdx_ctx {
   double skin = 0.3;
   int every = 1, delay = 10, check = 1, one = 2000;
   double triggersq = 0.0225, cutneighmax = 0.0;
-  int *nbr = nullptr, *numneigh = nullptr, *dflags = nullptr;  // dflags: [0] max count, [1] displaced, [2] lost
+  // dflags: [0] max count, [1] displaced, [4] lost atom, [5] brick staging
+  // overflow, [6] brick count scratch, [8..9] 64-bit neighbour total
+  int *nbr = nullptr, *numneigh = nullptr, *dflags = nullptr;
+  unsigned short *nbr16 = nullptr;  // brick mode: LDS slot numbers, [npad][cap]
+  int list_mode = MDX_LIST_GATHER;
+  int brick_req[3] = {8, 4, 4};
+  mdx::BrickArgs br{};
+  size_t lds_budget = 60 * 1024;
   int maxneigh = 0;
   long nbr_elems = 0;
   bool built = false;
@@ -164,7 +180,15 @@ int prof_resolve(mdx_ctx *c);
 // neighbour build (neigh.hip)
 int neigh_setup_grid(mdx_ctx *c);
 int neigh_build(mdx_ctx *c);
+int alloc_list(mdx_ctx *c, int cap);
 int neigh_check_displacement(mdx_ctx *c, int *flag);
+
+// LDS-staged bricks (brick.hip)
+int brick_prepare(mdx_ctx *c);
+int brick_build_list(mdx_ctx *c);
+int brick_pair(mdx_ctx *c, int ev);
+void brick_slot_map(const NeighArgs &a, const BrickArgs &br, int brick, const int *bin_start, std::vector<int> &gidx);
+int brick_of_bin(const BrickArgs &br, const NeighArgs &a, int bin);
 
 // pair kernels (pair.hip); eflag/vflag as Pair::compute.  Writes f (device,
 // sorted order), per-atom energy in f.w, per-atom virial in vatom, and the
@@ -177,6 +201,7 @@ int nve_final(mdx_ctx *c, double dtf);
 int reduce_ke(mdx_ctx *c);
 int finish_reduction(mdx_ctx *c, int nblocks, int nslots);
 int scatter_positions(mdx_ctx *c, const double *host_x);
+int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs);
 int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out);
 
 }  // namespace mdx

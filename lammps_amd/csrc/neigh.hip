@@ -295,6 +295,27 @@ int neigh_setup_grid(mdx_ctx *c) {
   return MDX_OK;
 }
 
+// (re)allocates the list of the current layout with `cap` entries per atom
+int alloc_list(mdx_ctx *c, int cap) {
+  cap = std::max(cap, 16);
+  hipFree(c->nbr);
+  hipFree(c->nbr16);
+  c->nbr = nullptr;
+  c->nbr16 = nullptr;
+  hipError_t e;
+  if (c->list_mode == MDX_LIST_BRICK)
+    e = hipMalloc(&c->nbr16, sizeof(unsigned short) * (size_t)cap * c->npad);
+  else
+    e = hipMalloc(&c->nbr, sizeof(int) * (size_t)cap * c->npad);
+  if (e != hipSuccess) {
+    set_error("Insufficient memory on accelerator (neighbour list)");
+    return MDX_ERR_NOMEM;
+  }
+  c->maxneigh = cap;
+  c->nbr_elems = (long)cap * c->npad;
+  return MDX_OK;
+}
+
 static int end_bit_for(int nbins) {
   int bits = 1;
   while ((1L << bits) < nbins) bits++;
@@ -354,20 +375,34 @@ int neigh_build(mdx_ctx *c) {
   std::swap(c->tag, c->tagb);
   std::swap(c->orig, c->origb);
   std::swap(c->image, c->imageb);
+  const bool brick = c->list_mode == MDX_LIST_BRICK;
+  if (brick) {
+    int rc = brick_prepare(c);
+    if (rc) return rc;
+  }
+  if (!(brick ? (void *)c->nbr16 : (void *)c->nbr)) {
+    int rc = alloc_list(c, c->maxneigh);
+    if (rc) return rc;
+  }
   // list build; regrow the per-atom capacity and rebuild on overflow
   for (int attempt = 0; attempt < 4; attempt++) {
     a.maxneigh = c->maxneigh;
     if ((e = hipMemsetAsync(c->dflags, 0, sizeof(int), c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
     if ((e = hipMemsetAsync(c->dflags + 8, 0, 8, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
-    switch (c->launch.build_tpa) {
-      case 1: launch_build<1>(c); break;
-      case 4: launch_build<4>(c); break;
-      case 8: launch_build<8>(c); break;
-      case 16: launch_build<16>(c); break;
-      case 32: launch_build<32>(c); break;
-      default:
-        set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
-        return MDX_ERR_ARG;
+    if (brick) {
+      int rc = brick_build_list(c);
+      if (rc) return rc;
+    } else {
+      switch (c->launch.build_tpa) {
+        case 1: launch_build<1>(c); break;
+        case 4: launch_build<4>(c); break;
+        case 8: launch_build<8>(c); break;
+        case 16: launch_build<16>(c); break;
+        case 32: launch_build<32>(c); break;
+        default:
+          set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
+          return MDX_ERR_ARG;
+      }
     }
     int flags[16];
     if ((e = hipMemcpyAsync(flags, c->dflags, 64, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
@@ -375,6 +410,10 @@ int neigh_build(mdx_ctx *c) {
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "neighbour build");
     if (flags[4]) {
       set_error("Lost atoms: an atom left a non-periodic box");
+      return MDX_ERR_ARG;
+    }
+    if (flags[5]) {
+      set_error("brick staging exceeded its LDS sizing (internal error)");
       return MDX_ERR_ARG;
     }
     const int mx = flags[0];
@@ -390,15 +429,8 @@ int neigh_build(mdx_ctx *c) {
     // regrow: the accelerator library's max_nbors resize (lal_neighbor.cpp:582-595)
     int cap = ((mx + mx / 4 + 15) / 16) * 16;
     cap = std::min(cap, c->one);
-    hipFree(c->nbr);
-    c->nbr = nullptr;
-    size_t bytes = sizeof(int) * (size_t)cap * c->npad;
-    if (hipMalloc(&c->nbr, bytes) != hipSuccess) {
-      set_error("Insufficient memory on accelerator (neighbour list)");
-      return MDX_ERR_NOMEM;
-    }
-    c->maxneigh = cap;
-    c->nbr_elems = (long)cap * c->npad;
+    int rc = alloc_list(c, cap);
+    if (rc) return rc;
   }
   if (c->max_neighbors > c->maxneigh) {
     set_error("neighbour list capacity could not be grown to fit");

@@ -34,6 +34,7 @@ namespace mdx {
 namespace {
 
 constexpr int kNT2 = MDX_NT1 * MDX_NT1;
+constexpr int kUnroll = 4;  // neighbour entries in flight per lane
 
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *__restrict__ out) {
@@ -144,42 +145,51 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
     const int row = static_cast<int>(xi.w) * nt1;
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
-    for (int k = lane; k < jn; k += TPA) {
-      const int jc = nb[k];
-      const int j = jc & MDX_IDX_MASK;
-      const int code = (unsigned)jc >> MDX_IDX_BITS;
-      const double4 xj = x[j];
-      double delx, dely, delz;
-      pair_delta(xi, xj, code, px, py, pz, delx, dely, delz);
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      const int t = row + static_cast<int>(xj.w);
-      if (rsq < s_cutsq[t]) {
-        double fpair, evdwl = 0.0;
-        if (MIXED) {
-          const float r2inv = 1.0f / (float)rsq;
-          const float r6inv = r2inv * r2inv * r2inv;
-          const float forcelj = r6inv * (f_lj1[t] * r6inv - f_lj2[t]);
-          fpair = (double)(forcelj * r2inv);
-          if (EV) evdwl = (double)(r6inv * (f_lj3[t] * r6inv - f_lj4[t]) - f_off[t]);
-        } else {
-          const double r2inv = 1.0 / rsq;
-          const double r6inv = r2inv * r2inv * r2inv;
-          const double forcelj = r6inv * (s_lj1[t] * r6inv - s_lj2[t]);
-          fpair = forcelj * r2inv;
-          if (EV) evdwl = r6inv * (s_lj3[t] * r6inv - s_lj4[t]) - s_off[t];
-        }
-        fx += delx * fpair;
-        fy += dely * fpair;
-        fz += delz * fpair;
-        if (EV) {
-          e += 0.5 * evdwl;
-          const double hf = 0.5 * fpair;
-          w[0] += delx * delx * hf;
-          w[1] += dely * dely * hf;
-          w[2] += delz * delz * hf;
-          w[3] += delx * dely * hf;
-          w[4] += delx * delz * hf;
-          w[5] += dely * delz * hf;
+    // UNROLL entries per lane per trip: all index loads, then all gathers,
+    // are issued before the first use, so a lane has UNROLL gathers in flight
+    for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
+      int jc[kUnroll];
+      double4 xj[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) xj[u] = x[jc[u] & MDX_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
+        double delx, dely, delz;
+        pair_delta(xi, xj[u], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        const int t = row + static_cast<int>(xj[u].w);
+        if (rsq < s_cutsq[t]) {
+          double fpair, evdwl = 0.0;
+          if (MIXED) {
+            // hardware reciprocal (1 ulp): well inside the FP32 pair-math budget
+            const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
+            const float r6inv = r2inv * r2inv * r2inv;
+            const float forcelj = r6inv * (f_lj1[t] * r6inv - f_lj2[t]);
+            fpair = (double)(forcelj * r2inv);
+            if (EV) evdwl = (double)(r6inv * (f_lj3[t] * r6inv - f_lj4[t]) - f_off[t]);
+          } else {
+            const double r2inv = 1.0 / rsq;
+            const double r6inv = r2inv * r2inv * r2inv;
+            const double forcelj = r6inv * (s_lj1[t] * r6inv - s_lj2[t]);
+            fpair = forcelj * r2inv;
+            if (EV) evdwl = r6inv * (s_lj3[t] * r6inv - s_lj4[t]) - s_off[t];
+          }
+          fx += delx * fpair;
+          fy += dely * fpair;
+          fz += delz * fpair;
+          if (EV) {
+            e += 0.5 * evdwl;
+            const double hf = 0.5 * fpair;
+            w[0] += delx * delx * hf;
+            w[1] += dely * dely * hf;
+            w[2] += delz * delz * hf;
+            w[3] += delx * dely * hf;
+            w[4] += delx * delz * hf;
+            w[5] += dely * delz * hf;
+          }
         }
       }
     }
@@ -213,28 +223,33 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
-    for (int k = lane; k < jn; k += TPA) {
-      const int jc = nb[k];
-      const int j = jc & MDX_IDX_MASK;
-      const int code = (unsigned)jc >> MDX_IDX_BITS;
-      const double4 xj = x[j];
-      double delx, dely, delz;
-      pair_delta(xi, xj, code, px, py, pz, delx, dely, delz);
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < a.cutforcesq) {
-        const int tj = static_cast<int>(xj.w);
-        const int tab = a.type2rhor[tj * a.ntypes1 + ti];
-        int m;
-        if (MIXED) {
-          float p;
-          spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
-          const float *c = a.rhor_f + (size_t)tab * stride + m * 7;
-          rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
-        } else {
-          double p;
-          spline_index<double>(sqrt(rsq), a.rdr, a.nr, m, p);
-          const double *c = a.rhor + (size_t)tab * stride + m * 7;
-          rho += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+    for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
+      int jc[kUnroll];
+      double4 xj[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) xj[u] = x[jc[u] & MDX_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
+        double delx, dely, delz;
+        pair_delta(xi, xj[u], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < a.cutforcesq) {
+          const int tab = a.type2rhor[static_cast<int>(xj[u].w) * a.ntypes1 + ti];
+          int m;
+          if (MIXED) {
+            float p;
+            spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
+            const float *c = a.rhor_f + (size_t)tab * stride + m * 7;
+            rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
+          } else {
+            double p;
+            spline_index<double>(sqrt(rsq), a.rdr, a.nr, m, p);
+            const double *c = a.rhor + (size_t)tab * stride + m * 7;
+            rho += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+          }
         }
       }
     }
@@ -277,13 +292,24 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
-    for (int k = lane; k < jn; k += TPA) {
-      const int jc = nb[k];
-      const int j = jc & MDX_IDX_MASK;
-      const int code = (unsigned)jc >> MDX_IDX_BITS;
-      const double4 xj = x[j];
+    for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
+      int jc[kUnroll];
+      double4 xjv[kUnroll];
+      double fpjv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int jj = jc[u] & MDX_IDX_MASK;
+        xjv[u] = x[jj];
+        fpjv[u] = fp[jj];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+      if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
+      const double4 xj = xjv[u];
       double delx, dely, delz;
-      pair_delta(xi, xj, code, px, py, pz, delx, dely, delz);
+      pair_delta(xi, xj, (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < a.cutforcesq) {
         const int tj = static_cast<int>(xj.w);
@@ -301,10 +327,10 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
           const float rhojp = (cj[0] * p + cj[1]) * p + cj[2];
           const float z2p = (cz[0] * p + cz[1]) * p + cz[2];
           const float z2 = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
-          const float recip = 1.0f / r;
+          const float recip = __builtin_amdgcn_rcpf(r);  // hardware reciprocal, 1 ulp
           const float phif = z2 * recip;
           const float phip = z2p * recip - phif * recip;
-          const float psip = (float)fpi * rhojp + (float)fp[j] * rhoip + phip;
+          const float psip = (float)fpi * rhojp + (float)fpjv[u] * rhoip + phip;
           fpair = (double)(-psip * recip);
           phi = (double)phif;
         } else {
@@ -322,7 +348,7 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
           const double recip = 1.0 / r;
           phi = z2 * recip;
           const double phip = z2p * recip - phi * recip;
-          const double psip = fpi * rhojp + fp[j] * rhoip + phip;
+          const double psip = fpi * rhojp + fpjv[u] * rhoip + phip;
           fpair = -psip * recip;
         }
         fx += delx * fpair;
@@ -338,6 +364,7 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
           w[4] += delx * delz * hf;
           w[5] += dely * delz * hf;
         }
+      }
       }
     }
     if (EV && lane == 0) e += eemb[i];
@@ -399,7 +426,8 @@ int pair_compute(mdx_ctx *c, int ev) {
   const int B = c->launch.block;
   const bool mixed = c->precision == MDX_MIXED;
   int rc;
-  if (mixed) rc = ev ? dispatch_tpa<true, true>(c, B) : dispatch_tpa<true, false>(c, B);
+  if (c->list_mode == MDX_LIST_BRICK) rc = brick_pair(c, ev);
+  else if (mixed) rc = ev ? dispatch_tpa<true, true>(c, B) : dispatch_tpa<true, false>(c, B);
   else rc = ev ? dispatch_tpa<false, true>(c, B) : dispatch_tpa<false, false>(c, B);
   if (rc) return rc;
   hipError_t e = hipGetLastError();

@@ -66,6 +66,8 @@ def lib():
             "mdx_get_stats": [_p, ctypes.POINTER(Stats)],
             "mdx_set_launch": [_p, _i, _i],
             "mdx_set_build_launch": [_p, _i],
+            "mdx_copy_bandwidth": [_p, ctypes.c_long, _i, _dp],
+            "mdx_set_list_mode": [_p, _i, _i, _i, _i],
             "mdx_set_profiling": [_p, _i],
         }
         for name, args in sig.items():
@@ -204,6 +206,16 @@ class Context:
 
     def set_build_launch(self, tpa=0):
         self._ck(self._L.mdx_set_build_launch(self.h, tpa))
+
+    def list_mode(self, mode, brick=(0, 0, 0)):
+        """mode 'gather' or 'brick' (LDS-staged bricks of brick[0]x[1]x[2] bins)."""
+        m = {"gather": 0, "brick": 1}[mode]
+        self._ck(self._L.mdx_set_list_mode(self.h, m, int(brick[0]), int(brick[1]), int(brick[2])))
+
+    def copy_bandwidth(self, nbytes=1 << 30, iters=20):
+        g = ctypes.c_double(0.0)
+        self._ck(self._L.mdx_copy_bandwidth(self.h, nbytes, iters, ctypes.byref(g)))
+        return g.value
 
     def profiling(self, on=True):
         self._ck(self._L.mdx_set_profiling(self.h, 1 if on else 0))
