@@ -38,8 +38,8 @@ enum {
 };
 
 enum { MDX_FP64 = 0, MDX_MIXED = 1 };        /* pair arithmetic precision */
-/* neighbour-list layout: global 32-bit indices read by per-neighbour gathers,
-   or 16-bit slots into LDS-staged bricks of bins (the default) */
+/* neighbour-list layout: global 32-bit indices read by per-neighbour gathers
+   (the default), or 16-bit slots into LDS-staged bricks of bins */
 enum { MDX_LIST_GATHER = 0, MDX_LIST_BRICK = 1 };
 enum { MDX_PAIR_NONE = 0, MDX_PAIR_LJ = 1, MDX_PAIR_EAM = 2 };
 
@@ -157,6 +157,13 @@ int mdx_run(mdx_ctx *ctx, int nsteps, double dt, double ftm2v, int thermo_every,
             void *user);
 
 int mdx_get_stats(mdx_ctx *ctx, mdx_stats *out);
+
+/* mdx_run fuses fix nve into the force kernel by default (final integrate of
+   step n and initial integrate of step n+1 run inside step n's force
+   kernel; same arithmetic, bit-identical trajectory).  0 runs the three
+   kernels separately, as FixNVE::initial_integrate / Pair::compute /
+   FixNVE::final_integrate are separate calls in the reference. */
+int mdx_set_fused(mdx_ctx *ctx, int on);
 
 /* Times every list build, pair launch and integrator launch with HIP event
    pairs on the context's stream (no synchronisation is added; the events

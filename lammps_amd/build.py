@@ -54,8 +54,17 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # the LAMMPS-style script runner: C++ plugin classes over the C ABI
+    exe = os.path.join(HERE, "lmp_mdx")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(HERE, "host", "plugin.cpp"),
+           os.path.join(HERE, "host", "lmp_mdx.cpp"), "-o", exe + ".tmp", "-L" + HERE, "-l:libmdx.so",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", exe)
     return LIB
 
 

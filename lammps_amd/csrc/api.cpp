@@ -148,7 +148,7 @@ int prepare(mdx_ctx *c) {
     if ((rc = alloc_list(c, guess))) return rc;
   }
   // enough blocks for any launch: block >= 64 threads, up to 32 threads per atom
-  const long G = ((long)c->n * 32 + 63) / 64 + 1;
+  const long G = ((long)c->n * 32 + 63) / 64 + 1 + kXCD;
   if (G * 8 > c->cap_partials) {
     if ((rc = dev_alloc(&c->partials, (size_t)G * 8))) return rc;
     c->cap_partials = G * 8;
@@ -597,7 +597,7 @@ static int thermo_out(mdx_ctx *c, mdx_thermo_fn cb, void *user) {
 }
 
 static int force_step(mdx_ctx *c, int ev) {
-  int rc = pair_compute(c, ev);
+  int rc = pair_compute(c, ev, nullptr);
   if (rc) return rc;
   if (ev) {
     c->eng_vdwl = c->red_host[0];
@@ -628,14 +628,22 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
   if (!c->run_setup_done && (rc = mdx_setup(c, cb, user))) return rc;
   const double dtv = dt, dtf = 0.5 * dt * ftm2v;
   const long last = c->ntimestep + nsteps;
+  // Fused path: the force kernel of step n also runs fix nve's final
+  // integrate of n and the initial integrate of n+1 (writing x(n+1) to the
+  // spare position buffer), so only the first step of a call runs a
+  // standalone initial integrate and no step runs a standalone final one.
+  const bool fused = c->fuse && c->list_mode == MDX_LIST_GATHER;
+  bool pending_initial = true;
+  auto is_decide = [c](int ago) { return ago >= c->delay && ago % c->every == 0; };
   for (int s = 0; s < nsteps; s++) {
     const long step = ++c->ntimestep;
+    const bool is_last = s == nsteps - 1;
     const bool out = step == last || (thermo_every > 0 && step % thermo_every == 0);
     // Neighbor::decide: ago++ then test (neighbor.cpp:1294-1309)
     const int ago_next = c->ago + 1;
-    const bool decide_step = ago_next >= c->delay && ago_next % c->every == 0;
+    const bool decide_step = is_decide(ago_next);
     const int check_now = decide_step && c->check;
-    if ((rc = nve_initial(c, dtv, dtf, check_now))) return rc;
+    if (pending_initial && (rc = nve_initial(c, dtv, dtf, check_now))) return rc;
     c->ago = ago_next;
     bool rebuild = false;
     if (decide_step) {
@@ -651,11 +659,42 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
       if ((rc = neigh_build(c))) return rc;
       c->ncalls++;
     }
-    if ((rc = force_step(c, out ? 1 : 0))) return rc;
-    if ((rc = nve_final(c, dtf))) return rc;
-    // bound the event pool: resolve on thermo steps and every 4096 records
+    if (fused) {
+      IntegArgs ig{};
+      for (int t = 0; t < MDX_NT1; t++) ig.mass[t] = c->mass[t];
+      ig.dtf = dtf;
+      ig.dtv = dtv;
+      ig.triggersq = c->triggersq;
+      ig.next = is_last ? 0 : 1;
+      ig.check = (!is_last && c->check && is_decide(c->ago + 1)) ? 1 : 0;
+      ig.write_f = is_last ? 1 : 0;
+      ig.v = c->v;
+      ig.xn = c->xb;
+      ig.xhold = c->xhold;
+      ig.dflags = c->dflags;
+      if (ig.check) {
+        hipError_t e = hipMemsetAsync(c->dflags + 1, 0, sizeof(int), c->stream);
+        if (e != hipSuccess) return hip_fail(e, "clear trigger flag");
+      }
+      if ((rc = pair_compute(c, out ? 1 : 0, &ig))) return rc;
+      if (out) {
+        c->eng_vdwl = c->red_host[0];
+        for (int k = 0; k < 6; k++) c->virial[k] = c->red_host[1 + k];
+        double row[8];
+        row[0] = c->red_host[7];
+        row[1] = c->eng_vdwl;
+        for (int k = 0; k < 6; k++) row[2 + k] = c->virial[k];
+        if (cb) cb(user, c->ntimestep, row);
+      }
+      if (ig.next) std::swap(c->x, c->xb);
+      pending_initial = false;
+    } else {
+      if ((rc = force_step(c, out ? 1 : 0))) return rc;
+      if ((rc = nve_final(c, dtf))) return rc;
+      if (out && (rc = thermo_out(c, cb, user))) return rc;
+    }
+    // bound the event pool: resolve every 4096 records
     if (c->evused > 4096 && (rc = prof_resolve(c))) return rc;
-    if (out && (rc = thermo_out(c, cb, user))) return rc;
   }
   hipError_t e = hipStreamSynchronize(c->stream);
   return e == hipSuccess ? MDX_OK : hip_fail(e, "run");
@@ -744,6 +783,13 @@ int mdx_copy_bandwidth(mdx_ctx *c, long nbytes, int iters, double *gbs) {
     return MDX_ERR_ARG;
   }
   return copy_probe(c, nbytes, iters, gbs);
+}
+
+int mdx_set_fused(mdx_ctx *c, int on) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  c->fuse = on != 0;
+  return MDX_OK;
 }
 
 int mdx_set_profiling(mdx_ctx *c, int on) {

@@ -79,6 +79,20 @@ struct EAMArgs {
   const float *frho_f, *rhor_f, *z2r_f;
 };
 
+// Velocity-Verlet work fused into the force kernel (fix nve final integrate
+// of this step, then initial integrate of the next, src/fix_nve.cpp:65-142)
+struct IntegArgs {
+  double mass[MDX_NT1];
+  double dtf, dtv, triggersq;
+  int next;      // also do the next step's initial integrate (x -> xn)
+  int check;     // test |xn - xhold|^2 > triggersq (next step is a decide step)
+  int write_f;   // store f (last step of a run, energy steps)
+  double4 *v;
+  double4 *xn;
+  const double4 *xhold;
+  int *dflags;
+};
+
 struct Launch {
   int block = 256;     // threads per workgroup of the pair kernels
   int tpa = 8;         // threads per atom in the pair kernels
@@ -132,6 +146,7 @@ struct mdx_ctx {
   long ncalls = 0, ndanger = 0;
   long ntimestep = 0;
   bool run_setup_done = false;
+  bool fuse = true;  // fuse fix nve into the force kernel in mdx_run (gather layout)
   // pair
   int pair_style = MDX_PAIR_NONE;
   int precision = MDX_FP64;
@@ -169,6 +184,17 @@ struct mdx_ctx {
 
 namespace mdx {
 
+// MI355X has 8 XCDs, each with its own L2; workgroup k is dispatched to XCD
+// k % 8.  Kernels over bin-sorted atoms use this logical block index so
+// that consecutive (spatially adjacent) tiles run on the same XCD and share
+// its L2 for the neighbour gathers.  Grids are launched as a multiple of 8.
+constexpr int kXCD = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int per = gridDim.x / kXCD;
+  return (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
+}
+inline int xcd_grid(long blocks) { return (int)(((blocks + kXCD - 1) / kXCD) * kXCD); }
+
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 
@@ -193,7 +219,7 @@ int brick_of_bin(const BrickArgs &br, const NeighArgs &a, int bin);
 // pair kernels (pair.hip); eflag/vflag as Pair::compute.  Writes f (device,
 // sorted order), per-atom energy in f.w, per-atom virial in vatom, and the
 // global energy/virial into c->red_host (after a sync) when ev is set.
-int pair_compute(mdx_ctx *c, int ev);
+int pair_compute(mdx_ctx *c, int ev, const IntegArgs *ig = nullptr);
 
 // integrator (integrate.hip)
 int nve_initial(mdx_ctx *c, double dtv, double dtf, int check);

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_build_full(NeighArgs a, const double4 *
                                                    const int *__restrict__ bin_start, int *__restrict__ nbr,
                                                    int *__restrict__ numneigh, int *__restrict__ dflags,
                                                    unsigned long long *__restrict__ nsum) {
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   if (i >= a.n) return;  // whole groups leave together (TPA divides 64)
   const int lane = threadIdx.x % TPA;
   const int gshift = (threadIdx.x & 63) - lane;  // group's first lane within the wavefront
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_build_full(NeighArgs a, const double4 *
 template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
-  const int G = (int)(((long)c->n * TPA + B - 1) / B);
+  const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
                      c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
 }

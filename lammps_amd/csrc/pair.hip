@@ -86,11 +86,47 @@ __device__ __forceinline__ void pair_delta(const double4 &xi, const double4 &xj,
   delz = xi.z - gz;
 }
 
-// per-atom results of one group: lane 0 stores, all lanes join the block sum
-template <int TPA, bool EV>
-__device__ __forceinline__ void store_atom(int i, bool owner, int npad, double fx, double fy, double fz, double e,
-                                           double (&w)[6], double4 *__restrict__ f, double *__restrict__ vatom,
-                                           double *__restrict__ partials) {
+// fix nve for one atom right after its force is complete: final integrate
+// of this step, then (ig.next) the initial integrate of the next step into
+// ig.xn, with the rebuild trigger.  Same operations and order as
+// k_nve_final + k_nve_initial, so fused and unfused trajectories are
+// bit-identical.  Returns m*v^2 after the final half-kick (thermo).
+__device__ __forceinline__ double integrate_atom(int i, const double4 &xi, double fx, double fy, double fz,
+                                                 const IntegArgs &ig) {
+#pragma clang fp contract(off)
+  double4 q = ig.v[i];
+  const double m = ig.mass[static_cast<int>(xi.w)];
+  const double dtfm = ig.dtf / m;
+  q.x += dtfm * fx;
+  q.y += dtfm * fy;
+  q.z += dtfm * fz;
+  const double mv2 = (q.x * q.x + q.y * q.y + q.z * q.z) * m;
+  if (ig.next) {
+    q.x += dtfm * fx;
+    q.y += dtfm * fy;
+    q.z += dtfm * fz;
+    double4 p = xi;
+    p.x += ig.dtv * q.x;
+    p.y += ig.dtv * q.y;
+    p.z += ig.dtv * q.z;
+    ig.xn[i] = p;
+    if (ig.check) {
+      const double4 h = ig.xhold[i];
+      const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
+      if (dx * dx + dy * dy + dz * dz > ig.triggersq) atomicOr(&ig.dflags[1], 1);
+    }
+  }
+  ig.v[i] = q;
+  return mv2;
+}
+
+// per-atom results of one group: lane 0 stores (and integrates when fused),
+// all lanes join the block sum of {e, virial[6], m v^2}
+template <int TPA, bool EV, bool INTEG>
+__device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const double4 &xi, double fx, double fy,
+                                            double fz, double e, double (&w)[6], const IntegArgs &ig,
+                                            double4 *__restrict__ f, double *__restrict__ vatom,
+                                            double *__restrict__ partials) {
   fx = group_sum<TPA>(fx);
   fy = group_sum<TPA>(fy);
   fz = group_sum<TPA>(fz);
@@ -99,26 +135,28 @@ __device__ __forceinline__ void store_atom(int i, bool owner, int npad, double f
 #pragma unroll
     for (int q = 0; q < 6; q++) w[q] = group_sum<TPA>(w[q]);
   }
+  double mv2 = 0.0;
   if (owner) {
-    f[i] = make_double4(fx, fy, fz, e);
+    if (!INTEG || EV || ig.write_f) f[i] = make_double4(fx, fy, fz, e);
     if (EV)
 #pragma unroll
       for (int q = 0; q < 6; q++) vatom[(size_t)q * npad + i] = w[q];
+    if (INTEG) mv2 = integrate_atom(i, xi, fx, fy, fz, ig);
   }
   if (EV) {
-    double v[7] = {owner ? e : 0.0, owner ? w[0] : 0.0, owner ? w[1] : 0.0, owner ? w[2] : 0.0,
-                   owner ? w[3] : 0.0,   owner ? w[4] : 0.0, owner ? w[5] : 0.0};
-    block_reduce_store<7>(v, partials);
+    double v[8] = {owner ? e : 0.0,    owner ? w[0] : 0.0, owner ? w[1] : 0.0, owner ? w[2] : 0.0,
+                   owner ? w[3] : 0.0, owner ? w[4] : 0.0, owner ? w[5] : 0.0, owner ? mv2 : 0.0};
+    block_reduce_store<8>(v, partials);
   }
 }
 
 // ------------------------------------------------------------------ LJ
-template <bool MIXED, bool EV, int TPA>
+template <bool MIXED, bool EV, int TPA, bool INTEG>
 __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
                                            const double4 *__restrict__ x, const int *__restrict__ nbr,
                                            const int *__restrict__ numneigh, const LJTab *__restrict__ tab,
                                            double4 *__restrict__ f, double *__restrict__ vatom,
-                                           double *__restrict__ partials) {
+                                           double *__restrict__ partials, IntegArgs ig) {
   __shared__ double s_cutsq[kNT2], s_lj1[kNT2], s_lj2[kNT2], s_lj3[kNT2], s_lj4[kNT2], s_off[kNT2];
   __shared__ float f_lj1[kNT2], f_lj2[kNT2], f_lj3[kNT2], f_lj4[kNT2], f_off[kNT2];
   const int nt2 = nt1 * nt1;
@@ -136,12 +174,13 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
     f_off[q] = (float)tab->offset[q];
   }
   __syncthreads();
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   const int lane = threadIdx.x % TPA;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
   if (i < n) {
-    const double4 xi = x[i];
+    xi = x[i];
     const int row = static_cast<int>(xi.w) * nt1;
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
@@ -194,7 +233,7 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
       }
     }
   }
-  store_atom<TPA, EV>(i, i < n && lane == 0, npad, fx, fy, fz, e, w, f, vatom, partials);
+  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, f, vatom, partials);
 }
 
 // ------------------------------------------------------------------ EAM
@@ -213,7 +252,7 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
                                                 const double4 *__restrict__ x, const int *__restrict__ nbr,
                                                 const int *__restrict__ numneigh, double *__restrict__ rho_out,
                                                 double *__restrict__ fp_out, double *__restrict__ eemb) {
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   const int lane = threadIdx.x % TPA;
   double rho = 0.0;
   int ti = 0;
@@ -274,19 +313,20 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
   }
 }
 
-template <bool MIXED, bool EV, int TPA>
+template <bool MIXED, bool EV, int TPA, bool INTEG>
 __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
                                                   EAMArgs a, const double4 *__restrict__ x,
                                                   const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                                   const double *__restrict__ fp, const double *__restrict__ eemb,
                                                   double4 *__restrict__ f, double *__restrict__ vatom,
-                                                  double *__restrict__ partials) {
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / TPA;
+                                                  double *__restrict__ partials, IntegArgs ig) {
+  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   const int lane = threadIdx.x % TPA;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
   if (i < n) {
-    const double4 xi = x[i];
+    xi = x[i];
     const int ti = static_cast<int>(xi.w);
     const double fpi = fp[i];
     const int jn = numneigh[i];
@@ -369,45 +409,45 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
     }
     if (EV && lane == 0) e += eemb[i];
   }
-  store_atom<TPA, EV>(i, i < n && lane == 0, npad, fx, fy, fz, e, w, f, vatom, partials);
+  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, f, vatom, partials);
 }
 
-template <bool MIXED, bool EV, int TPA>
-void launch_lj(mdx_ctx *c, int B) {
-  const int G = (int)(((long)c->n * TPA + B - 1) / B);
-  prof_begin(c, MDX_K_LJ);
-  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+template <bool MIXED, bool EV, int TPA, bool INTEG>
+void launch_lj(mdx_ctx *c, int B, const IntegArgs &ig) {
+  const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
+  prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
                      c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->nbr, c->numneigh, c->lj_dev, c->f,
-                     c->vatom, c->partials);
+                     c->vatom, c->partials, ig);
   prof_end(c);
   c->last_grid = G;
 }
 
-template <bool MIXED, bool EV, int TPA>
-void launch_eam(mdx_ctx *c, int B) {
-  const int G = (int)(((long)c->n * TPA + B - 1) / B);
+template <bool MIXED, bool EV, int TPA, bool INTEG>
+void launch_eam(mdx_ctx *c, int B, const IntegArgs &ig) {
+  const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, MDX_K_EAM_RHO);
   hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
                      c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
   prof_end(c);
-  prof_begin(c, MDX_K_EAM_FORCE);
-  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
-                     c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->fp, c->eemb, c->f,
-                     c->vatom, c->partials);
+  prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
+  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
+                     c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->fp, c->eemb,
+                     c->f, c->vatom, c->partials, ig);
   prof_end(c);
   c->last_grid = G;
 }
 
-template <bool MIXED, bool EV>
-int dispatch_tpa(mdx_ctx *c, int B) {
+template <bool MIXED, bool EV, bool INTEG>
+int dispatch_tpa(mdx_ctx *c, int B, const IntegArgs &ig) {
   const bool lj = c->pair_style == MDX_PAIR_LJ;
   switch (c->launch.tpa) {
-    case 1: lj ? launch_lj<MIXED, EV, 1>(c, B) : launch_eam<MIXED, EV, 1>(c, B); break;
-    case 2: lj ? launch_lj<MIXED, EV, 2>(c, B) : launch_eam<MIXED, EV, 2>(c, B); break;
-    case 4: lj ? launch_lj<MIXED, EV, 4>(c, B) : launch_eam<MIXED, EV, 4>(c, B); break;
-    case 8: lj ? launch_lj<MIXED, EV, 8>(c, B) : launch_eam<MIXED, EV, 8>(c, B); break;
-    case 16: lj ? launch_lj<MIXED, EV, 16>(c, B) : launch_eam<MIXED, EV, 16>(c, B); break;
-    case 32: lj ? launch_lj<MIXED, EV, 32>(c, B) : launch_eam<MIXED, EV, 32>(c, B); break;
+    case 1: lj ? launch_lj<MIXED, EV, 1, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 1, INTEG>(c, B, ig); break;
+    case 2: lj ? launch_lj<MIXED, EV, 2, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 2, INTEG>(c, B, ig); break;
+    case 4: lj ? launch_lj<MIXED, EV, 4, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 4, INTEG>(c, B, ig); break;
+    case 8: lj ? launch_lj<MIXED, EV, 8, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 8, INTEG>(c, B, ig); break;
+    case 16: lj ? launch_lj<MIXED, EV, 16, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 16, INTEG>(c, B, ig); break;
+    case 32: lj ? launch_lj<MIXED, EV, 32, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 32, INTEG>(c, B, ig); break;
     default:
       set_error("threads per atom must be one of 1, 2, 4, 8, 16, 32");
       return MDX_ERR_ARG;
@@ -415,25 +455,45 @@ int dispatch_tpa(mdx_ctx *c, int B) {
   return MDX_OK;
 }
 
+template <bool INTEG>
+int dispatch_prec(mdx_ctx *c, int ev, const IntegArgs &ig) {
+  const int B = c->launch.block;
+  if (c->precision == MDX_MIXED)
+    return ev ? dispatch_tpa<true, true, INTEG>(c, B, ig) : dispatch_tpa<true, false, INTEG>(c, B, ig);
+  return ev ? dispatch_tpa<false, true, INTEG>(c, B, ig) : dispatch_tpa<false, false, INTEG>(c, B, ig);
+}
+
 }  // namespace
 
-int pair_compute(mdx_ctx *c, int ev) {
+// Pair::compute on the device.  With `ig` (gather layout only) the force
+// kernel also runs fix nve's final integrate of this step and, if ig->next,
+// the initial integrate of the next one (positions go to ig->xn).  With ev
+// the reduction's slots are {eng_vdwl, virial[6], sum m v^2 (fused only)}.
+int pair_compute(mdx_ctx *c, int ev, const IntegArgs *ig) {
   if (c->n == 0) return MDX_OK;
   if (c->pair_style != MDX_PAIR_LJ && c->pair_style != MDX_PAIR_EAM) {
     set_error("no pair style set");
     return MDX_ERR_ARG;
   }
-  const int B = c->launch.block;
-  const bool mixed = c->precision == MDX_MIXED;
-  int rc;
-  if (c->list_mode == MDX_LIST_BRICK) rc = brick_pair(c, ev);
-  else if (mixed) rc = ev ? dispatch_tpa<true, true>(c, B) : dispatch_tpa<true, false>(c, B);
-  else rc = ev ? dispatch_tpa<false, true>(c, B) : dispatch_tpa<false, false>(c, B);
+  int rc, nslots = 8;
+  if (c->list_mode == MDX_LIST_BRICK) {
+    if (ig) {
+      set_error("fused integration needs the gather list layout");
+      return MDX_ERR_ARG;
+    }
+    rc = brick_pair(c, ev);
+    nslots = 7;
+  } else if (ig) {
+    rc = dispatch_prec<true>(c, ev, *ig);
+  } else {
+    IntegArgs none{};
+    rc = dispatch_prec<false>(c, ev, none);
+  }
   if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
   // the per-atom energy in f.w already includes the EAM embedding term
-  if (ev) return finish_reduction(c, c->last_grid, 7);
+  if (ev) return finish_reduction(c, c->last_grid, nslots);
   return MDX_OK;
 }
 

@@ -68,6 +68,7 @@ def lib():
             "mdx_set_build_launch": [_p, _i],
             "mdx_copy_bandwidth": [_p, ctypes.c_long, _i, _dp],
             "mdx_set_list_mode": [_p, _i, _i, _i, _i],
+            "mdx_set_fused": [_p, _i],
             "mdx_set_profiling": [_p, _i],
         }
         for name, args in sig.items():
@@ -211,6 +212,9 @@ class Context:
         """mode 'gather' or 'brick' (LDS-staged bricks of brick[0]x[1]x[2] bins)."""
         m = {"gather": 0, "brick": 1}[mode]
         self._ck(self._L.mdx_set_list_mode(self.h, m, int(brick[0]), int(brick[1]), int(brick[2])))
+
+    def fused(self, on=True):
+        self._ck(self._L.mdx_set_fused(self.h, 1 if on else 0))
 
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         g = ctypes.c_double(0.0)

@@ -177,6 +177,37 @@ def test_eam_forces_energy_virial(eam_state, prec):
     np.testing.assert_allclose(out["virial"], wref, rtol=etol, atol=etol * np.abs(wref).max())
 
 
+@pytest.mark.parametrize("style", ["lj", "eam"])
+def test_fused_integration_is_bit_identical_to_separate_kernels(lj_state, eam_state, style):
+    """mdx_run with fix nve fused into the force kernel reproduces the
+    separate initial/force/final kernels bit for bit over 60 steps with
+    rebuilds, and thermo rows agree."""
+    from lammps_amd import mdx
+    out = {}
+    for fused in (True, False):
+        if style == "lj":
+            r = lj_state
+            lo, hi = box_of(r, 20, A_LJ)
+            ctx = device_from_oracle(r, lo, hi, "lj", mdx.MIXED, 0.3, 5, 0, True)
+            ftm2v = 1.0
+        else:
+            r = eam_state
+            lo, hi = box_of(r, 20, 3.615)
+            ctx = device_from_oracle(r, lo, hi, "eam", mdx.MIXED, 1.0, 1, 2, True)
+            ftm2v = 1.0 / 1.0364269e-4
+        ctx.fused(fused)
+        rows = ctx.setup() + ctx.run(37, 0.005, ftm2v, 10) + ctx.run(23, 0.005, ftm2v, 10)
+        x, v, f, im = ctx.get_atoms()
+        out[fused] = dict(x=x, v=v, f=f, im=im, rows=rows, st=ctx.stats())
+        ctx.close()
+    for k in ("x", "v", "f", "im"):
+        assert np.array_equal(out[True][k], out[False][k]), k
+    assert out[True]["st"]["ncalls"] == out[False]["st"]["ncalls"] > 0
+    for (s1, r1), (s2, r2) in zip(out[True]["rows"], out[False]["rows"]):
+        assert s1 == s2
+        np.testing.assert_allclose(r1, r2, rtol=1e-13, atol=1e-13 * max(abs(q) for q in r2))
+
+
 def test_lj_bench_run_matches_reference_log():
     """bench/in.lj on the device (FP64): thermo rows at steps 0 and 100 equal
     the log's printed digits, 1202833 neighbours, 5 builds."""
