@@ -97,6 +97,10 @@ def run_case(kind, inp, precision, steps, warmup):
                builds=st["ncalls"] - st0["ncalls"], total_neighbors=st["total_neighbors"],
                max_neighbors=st["max_neighbors"], kernel_ms=st["kernel_ms"], kernel_launches=st["kernel_launches"],
                end_row=rows[-1][1] if rows else None, ctx=ctx)
+    names = ("build", "lj", "eam_rho", "eam_force", "nve_initial", "nve_final", "fused")
+    log("  kernels (ms/step, launches): " + ", ".join(
+        "%s %.4f (%d)" % (nm, st["kernel_ms"][k] / steps, st["kernel_launches"][k])
+        for k, nm in enumerate(names) if st["kernel_launches"][k]))
     return res
 
 
@@ -251,25 +255,31 @@ def main():
     probe.close()
     log("HBM copy probe: %.0f GB/s" % copy_gbs)
 
-    # roofline of the dominant kernel (LJ force, mixed): algorithmic bytes per
-    # launch = N * (4*(n_h+1) half-list indices + 32 B x_i + 24 B f_i)
+    # roofline of the dominant kernel: the LJ force kernel, which in the fused
+    # path (the default) also runs fix nve's final integrate of step n and the
+    # initial integrate of n+1.  Algorithmic bytes per atom and launch:
+    # 4*(n_h+1) half-list indices + 32 B x_i (+ 72 B when fused: v read and
+    # write, x(n+1) write; 24 B f_i write otherwise).
     km = res["mixed"]["kernel_ms"]
     kl = res["mixed"]["kernel_launches"]
-    lj_ms = km[mdx.K_LJ] / max(1, kl[mdx.K_LJ])
+    fused = kl[mdx.K_FUSED] > kl[mdx.K_LJ]
+    kk = mdx.K_FUSED if fused else mdx.K_LJ
+    lj_ms = km[kk] / max(1, kl[kk])
     n_h = res["mixed"]["total_neighbors"] / 2.0 / n
-    bytes_per_launch = n * (4.0 * (n_h + 1.0) + 32.0 + 24.0)
-    achieved = bytes_per_launch / (lj_ms * 1e-3) / 1e9
+    bytes_per_launch = n * (4.0 * (n_h + 1.0) + 32.0 + (72.0 if fused else 24.0))
+    achieved = bytes_per_launch / (lj_ms * 1e-3) / 1e9 if lj_ms > 0 else None
     total_ms = res["mixed"]["seconds"] * 1e3
-    kname = "k_lj<true, false, %d>" % 8
-    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+    kname = "k_lj<true, false, 8, %s>" % ("true" if fused else "false")
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=None if achieved is None else achieved / HBM_PEAK_GBS,
                     traffic=profiled_traffic(kname), kernel=kname, avg_launch_ms=lj_ms,
                     bytes_per_launch=bytes_per_launch, n_half_neighbors_per_atom=n_h,
                     measured_copy_gbs=copy_gbs,
                     step_algorithmic_frac=(n * (4.0 * (n_h + 1) + 32 + 24 + 200 + 16 + (4.0 * n_h + 32) / 20.0)
                                            / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9) / HBM_PEAK_GBS)
     shares = {name: km[k] / total_ms for name, k in
-              (("neigh_build", mdx.K_BUILD), ("pair", mdx.K_LJ), ("nve_initial", mdx.K_NVE_INITIAL),
-               ("nve_final", mdx.K_NVE_FINAL))}
+              (("neigh_build", mdx.K_BUILD), ("pair", mdx.K_LJ), ("pair_fused_nve", mdx.K_FUSED),
+               ("nve_initial", mdx.K_NVE_INITIAL), ("nve_final", mdx.K_NVE_FINAL))}
 
     errors, cpu, t_force = None, None, None
     if not args.no_cpu_baseline:

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdx.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["api.cpp", "host_setup.cpp", "neigh.hip", "brick.hip", "pair.hip", "integrate.hip"]
+SOURCES = ["api.cpp", "host_setup.cpp", "neigh.hip", "brick.hip", "pair.hip", "integrate.hip", "batch.hip"]
 ARCH = "gfx950"
 
 
@@ -63,9 +63,36 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("host build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(exe + ".tmp", exe)
+    ext = build_torch_ext()
     if verbose:
-        print("built", LIB, "and", exe)
+        print("built", LIB, exe, "and", ext)
     return LIB
+
+
+TORCH_EXT = os.path.join(HERE, "_mdx_torch.so")
+
+
+def build_torch_ext():
+    """The autograd Functions over the batch ABI (csrc/torch_ext.cpp) as an
+    in-tree Python extension module linked against libmdx.so and libtorch."""
+    import sysconfig
+
+    import torch
+    import torch.utils.cpp_extension as ce
+
+    libdir = ce.library_paths()[0]
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-std=c++17",
+           "-D_GLIBCXX_USE_CXX11_ABI=%d" % int(torch._C._GLIBCXX_USE_CXX11_ABI),
+           "-DTORCH_EXTENSION_NAME=_mdx_torch", "-DTORCH_API_INCLUDE_EXTENSION_H", "-Wno-deprecated-declarations"]
+    cmd += ["-I" + p for p in ce.include_paths() + [sysconfig.get_paths()["include"], INCLUDE]]
+    cmd += [os.path.join(CSRC, "torch_ext.cpp"), "-o", TORCH_EXT + ".tmp", "-L" + libdir, "-lc10", "-ltorch",
+            "-ltorch_cpu", "-ltorch_python", "-L" + HERE, "-l:libmdx.so", "-Wl,-rpath," + libdir,
+            "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("torch extension build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(TORCH_EXT + ".tmp", TORCH_EXT)
+    return TORCH_EXT
 
 
 if __name__ == "__main__":

@@ -91,10 +91,11 @@ __device__ __forceinline__ void pair_delta(const double4 &xi, const double4 &xj,
 // ig.xn, with the rebuild trigger.  Same operations and order as
 // k_nve_final + k_nve_initial, so fused and unfused trajectories are
 // bit-identical.  Returns m*v^2 after the final half-kick (thermo).
+// v and xhold (vpre, hpre) are loaded at the start of the force kernel, so
+// their latency hides behind the neighbour loop instead of extending it.
 __device__ __forceinline__ double integrate_atom(int i, const double4 &xi, double fx, double fy, double fz,
-                                                 const IntegArgs &ig) {
+                                                 const IntegArgs &ig, double4 q, const double4 &h) {
 #pragma clang fp contract(off)
-  double4 q = ig.v[i];
   const double m = ig.mass[static_cast<int>(xi.w)];
   const double dtfm = ig.dtf / m;
   q.x += dtfm * fx;
@@ -111,7 +112,6 @@ __device__ __forceinline__ double integrate_atom(int i, const double4 &xi, doubl
     p.z += ig.dtv * q.z;
     ig.xn[i] = p;
     if (ig.check) {
-      const double4 h = ig.xhold[i];
       const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
       if (dx * dx + dy * dy + dz * dz > ig.triggersq) atomicOr(&ig.dflags[1], 1);
     }
@@ -125,8 +125,8 @@ __device__ __forceinline__ double integrate_atom(int i, const double4 &xi, doubl
 template <int TPA, bool EV, bool INTEG>
 __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const double4 &xi, double fx, double fy,
                                             double fz, double e, double (&w)[6], const IntegArgs &ig,
-                                            double4 *__restrict__ f, double *__restrict__ vatom,
-                                            double *__restrict__ partials) {
+                                            const double4 &vpre, const double4 &hpre, double4 *__restrict__ f,
+                                            double *__restrict__ vatom, double *__restrict__ partials) {
   fx = group_sum<TPA>(fx);
   fy = group_sum<TPA>(fy);
   fz = group_sum<TPA>(fz);
@@ -141,7 +141,7 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
     if (EV)
 #pragma unroll
       for (int q = 0; q < 6; q++) vatom[(size_t)q * npad + i] = w[q];
-    if (INTEG) mv2 = integrate_atom(i, xi, fx, fy, fz, ig);
+    if (INTEG) mv2 = integrate_atom(i, xi, fx, fy, fz, ig, vpre, hpre);
   }
   if (EV) {
     double v[8] = {owner ? e : 0.0,    owner ? w[0] : 0.0, owner ? w[1] : 0.0, owner ? w[2] : 0.0,
@@ -179,6 +179,11 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 vpre = make_double4(0.0, 0.0, 0.0, 0.0), hpre = vpre;
+  if (INTEG && i < n && lane == 0) {
+    vpre = ig.v[i];
+    if (ig.check) hpre = ig.xhold[i];
+  }
   if (i < n) {
     xi = x[i];
     const int row = static_cast<int>(xi.w) * nt1;
@@ -233,7 +238,8 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
       }
     }
   }
-  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, f, vatom, partials);
+  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
+                              partials);
 }
 
 // ------------------------------------------------------------------ EAM
@@ -325,6 +331,11 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 vpre = make_double4(0.0, 0.0, 0.0, 0.0), hpre = vpre;
+  if (INTEG && i < n && lane == 0) {
+    vpre = ig.v[i];
+    if (ig.check) hpre = ig.xhold[i];
+  }
   if (i < n) {
     xi = x[i];
     const int ti = static_cast<int>(xi.w);
@@ -409,7 +420,8 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
     }
     if (EV && lane == 0) e += eemb[i];
   }
-  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, f, vatom, partials);
+  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
+                              partials);
 }
 
 template <bool MIXED, bool EV, int TPA, bool INTEG>

@@ -33,3 +33,26 @@ def test_version_and_error_string_without_gpu():
     L = mdx.lib()
     assert L.mdx_version() == 1
     assert isinstance(L.mdx_last_error(), bytes)
+
+
+def test_batch_header_is_covered_and_create_fails_cleanly_without_gpu():
+    names = declared_symbols()
+    for n in ("mdx_batch_create", "mdx_batch_lj_vjp", "mdx_batch_eam_vjp", "mdx_batch_autotune"):
+        assert n in names
+    from lammps_amd import batch
+    from lammps_amd.mdx import ERR_HIP, MdxError
+    import numpy as np
+    try:
+        b = batch.Batch([2], [[0, 0, 0]], [[5, 5, 5]], [[1, 1, 1]], 1, np.ones(2, np.int32))
+    except MdxError as e:
+        # without a HIP device: a clean error, not a crash
+        assert e.code == ERR_HIP
+    else:
+        b.close()
+
+
+def test_torch_extension_loads_and_exposes_the_autograd_functions():
+    from lammps_amd import fit
+    m = fit.ext()
+    assert m.version() == 1
+    assert hasattr(m, "lj_forces") and hasattr(m, "eam_forces")
