@@ -204,6 +204,185 @@ def cpu_baseline_and_errors(kind, inp, ctxs, sample_steps):
     return out, base, t_force
 
 
+def fit_lj_ka(reps=10, check_oracle=True):
+    """Differentiable lj/cut force matching on the KA mixture: 64 snapshots of
+    8,000 atoms (device MD preparation), forward forces in mixed precision,
+    loss = mean |F - F_t|^2 / 3, backward through the autograd Function to
+    dL/d(eps, sigma).  Reports evaluations/s and the gradient against the
+    FP64 oracle gradient and central differences of the device FP64 loss."""
+    import torch
+    from lammps_amd import fit, workloads as WL
+    from lammps_amd.batch import FP64, MIXED
+    t0 = time.perf_counter()
+    snaps, types, L = WL.ka_snapshots()
+    prep = time.perf_counter() - t0
+    S, N = snaps.shape[0], snaps.shape[1]
+    natoms = [N] * S
+    lo, hi, per = [[0, 0, 0]] * S, [[L, L, L]] * S, [[1, 1, 1]] * S
+    t_all = np.tile(types, S)
+    x = snaps.reshape(-1, 3)
+    eps, sig, cut = WL.ka_pair_vectors()
+    e_t, s_t = WL.ka_target_vectors()
+    dev = torch.device("cuda", 0)
+    out = {"workload": "KA 80:20 LJ, 8,000 atoms x %d snapshots (T*=0.8 device MD), cut 2.5 sigma_ab" % S,
+           "prep_s": prep}
+    fm64 = fit.LJForceMatch(natoms, lo, hi, per, 2, t_all, x, cut, precision=FP64)
+    with torch.no_grad():
+        Ft = fm64(torch.tensor(e_t), torch.tensor(s_t)).clone()
+
+    def loss_and_grad(fm):
+        pe = torch.tensor(eps, requires_grad=True)
+        ps = torch.tensor(sig, requires_grad=True)
+        F = fm(pe, ps)
+        loss = ((F - Ft) ** 2).sum() / (3.0 * F.shape[0])
+        loss.backward()
+        return loss.item(), pe.grad.numpy().copy(), ps.grad.numpy().copy(), F
+
+    res = {}
+    for name, fm in (("mixed", fit.LJForceMatch(natoms, lo, hi, per, 2, t_all, x, cut, precision=MIXED)),
+                     ("fp64", fm64)):
+        fm.batch.autotune(True)
+        loss_and_grad(fm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            loss, ge, gs, F = loss_and_grad(fm)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = dict(loss=loss, grad_eps=ge.tolist(), grad_sigma=gs.tolist(), ms_per_loss_and_grad=dt * 1e3,
+                         configs_per_s=S / dt, atom_forces_per_s=S * N / dt)
+    # central differences of the device FP64 loss (h = 1e-6 relative)
+    fd = []
+    for which, base in ((0, eps), (1, sig)):
+        for k in range(3):
+            vals = []
+            for sgn in (1, -1):
+                pe, ps = eps.copy(), sig.copy()
+                (pe, ps)[which][k] = base[k] * (1 + sgn * 1e-6)
+                with torch.no_grad():
+                    F = fm64(torch.tensor(pe), torch.tensor(ps))
+                    vals.append((((F - Ft) ** 2).sum() / (3.0 * F.shape[0])).item())
+            fd.append((vals[0] - vals[1]) / (2e-6 * base[k]))
+    g64 = np.concatenate([res["fp64"]["grad_eps"], res["fp64"]["grad_sigma"]])
+    gmx = np.concatenate([res["mixed"]["grad_eps"], res["mixed"]["grad_sigma"]])
+    fd = np.array(fd)
+    out["central_fd"] = fd.tolist()
+    out["fp64_grad_vs_fd_max_rel"] = float(np.max(np.abs(g64 - fd) / np.abs(fd)))
+    out["mixed_grad_vs_fd_max_rel"] = float(np.max(np.abs(gmx - fd) / np.abs(fd)))
+    if check_oracle:
+        from oracle import fitref as FR
+        ref = FR.Batch(natoms, lo, hi, per, 2, t_all, x)
+        from lammps_amd.fit import pair_table
+        Fth, _ = ref.lj_forces(pair_table(e_t, 2), pair_table(s_t, 2), pair_table(cut, 2))
+        F0, _ = ref.lj_forces(pair_table(eps, 2), pair_table(sig, 2), pair_table(cut, 2))
+        G = 2.0 * (F0 - Fth) / (3.0 * F0.shape[0])
+        t0 = time.perf_counter()
+        ge_r, gs_r = ref.lj_vjp(pair_table(eps, 2), pair_table(sig, 2), pair_table(cut, 2), G)
+        out["oracle_vjp_s"] = time.perf_counter() - t0
+        gr = np.array([ge_r[1, 1], ge_r[1, 2], ge_r[2, 2], gs_r[1, 1], gs_r[1, 2], gs_r[2, 2]])
+        out["fp64_grad_vs_oracle_max_rel"] = float(np.max(np.abs(g64 - gr) / np.abs(gr)))
+        out["mixed_grad_vs_oracle_max_rel"] = float(np.max(np.abs(gmx - gr) / np.abs(gr)))
+    out.update(res)
+    fm64.batch.close()
+    return out
+
+
+def eam_replicas(reps=10, check_oracle=True, fd_knots=6):
+    """Batched EAM clusters: 512 non-periodic fcc Cu-like clusters of
+    200-5000 atoms (seed 7), thermalised on the device, forces + energies +
+    the force-matching adjoint w.r.t. the 3x64 knot values in one persistent
+    launch per pass.  Reports replica-forces/s and the knot gradient against
+    the FP64 oracle and central differences."""
+    from lammps_amd import batch as B, workloads as WL
+    t0 = time.perf_counter()
+    d = WL.eam_clusters(nrep=512, thermal_steps=100)
+    prep = time.perf_counter() - t0
+    natoms = [len(q) for q in d["x"]]
+    x = np.concatenate(d["x"])
+    n = len(x)
+    p = d["pot"]
+    target = WL.perturbed_knots(p)
+    per = [[0, 0, 0]] * len(natoms)
+    out = {"workload": "512 fcc Cu-like clusters, %d atoms (200-5000 each, seed 7), 64-knot synthetic eam, "
+                       "100 NVE steps thermalisation" % n, "prep_s": prep, "atoms": n}
+
+    def make(prec):
+        b = B.Batch(natoms, d["lo"], d["hi"], per, 1, np.ones(n, np.int32))
+        b.set_positions(x)
+        b.precision(prec)
+        return b
+
+    b64 = make(B.FP64)
+    b64.eam_init(p["drho"], p["dr"], p["cut"], *target)
+    Ft, _ = b64.eam_forces(energy=False)
+    b64.eam_init(p["drho"], p["dr"], p["cut"], p["frho"], p["rhor"], p["z2r"])
+    res = {}
+    for name, prec in (("mixed", B.MIXED), ("fp64", B.FP64)):
+        b = make(prec)
+        b.eam_init(p["drho"], p["dr"], p["cut"], p["frho"], p["rhor"], p["z2r"])
+        b.autotune(True)
+        F, _ = b.eam_forces()
+        b.eam_vjp(2.0 * (F - Ft) / (3.0 * n))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            F, e = b.eam_forces()
+        t_fwd = (time.perf_counter() - t0) / reps
+        G = 2.0 * (F - Ft) / (3.0 * n)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g = b.eam_vjp(G)
+        t_adj = (time.perf_counter() - t0) / reps
+        res[name] = dict(ms_forward=t_fwd * 1e3, ms_adjoint=t_adj * 1e3, replica_forces_per_s=len(natoms) / t_fwd,
+                         replica_force_and_grad_per_s=len(natoms) / (t_fwd + t_adj),
+                         launch={B.KIND_NAMES[k]: b.get_launch(k) for k in (3, 4, 5, 6)},
+                         loss=float(np.sum((F - Ft) ** 2) / (3.0 * n)))
+        res[name]["grad"] = g
+        b.close()
+    g64 = res["fp64"].pop("grad")
+    gmx = res["mixed"].pop("grad")
+    # central differences of the device FP64 loss on a sample of knots
+    F0, _ = b64.eam_forces(energy=False)
+    knots = [p["frho"], p["rhor"], p["z2r"]]
+    errs = []
+    rng = np.random.default_rng(0)
+    for which in range(3):
+        nz = np.nonzero(np.abs(g64[which]) > 1e-3 * np.max(np.abs(g64[which])))[0]
+        for k in rng.choice(nz, size=min(fd_knots, len(nz)), replace=False):
+            h = 1e-6 * max(abs(knots[which][k]), 1e-3)
+            vals = []
+            for sgn in (1, -1):
+                kk = [a.copy() for a in knots]
+                kk[which][k] += sgn * h
+                b64.eam_init(p["drho"], p["dr"], p["cut"], *kk)
+                F, _ = b64.eam_forces(energy=False)
+                vals.append(np.sum((F - Ft) ** 2) / (3.0 * n))
+            fdv = (vals[0] - vals[1]) / (2 * h)
+            scale = np.max(np.abs(g64[which]))
+            errs.append(dict(fn=("frho", "rhor", "z2r")[which], knot=int(k), fd=float(fdv),
+                             fp64_rel=float(abs(g64[which][k] - fdv) / scale),
+                             mixed_rel=float(abs(gmx[which][k] - fdv) / scale)))
+    b64.close()
+    out["fd_check"] = errs
+    out["fp64_grad_vs_fd_max_rel"] = max(e["fp64_rel"] for e in errs)
+    out["mixed_grad_vs_fd_max_rel"] = max(e["mixed_rel"] for e in errs)
+    if check_oracle:
+        from oracle import fitref as FR
+        ref = FR.Batch(natoms, d["lo"], d["hi"], per, 1, np.ones(n), x)
+        ref.eam_set(p["drho"], p["dr"], p["cut"], *target)
+        Ftr, _, _, _ = ref.eam_forces()
+        ref.eam_set(p["drho"], p["dr"], p["cut"], p["frho"], p["rhor"], p["z2r"])
+        F0r, _, _, _ = ref.eam_forces()
+        t0 = time.perf_counter()
+        gr = ref.eam_vjp(2.0 * (F0r - Ftr) / (3.0 * n))
+        out["oracle_vjp_s"] = time.perf_counter() - t0
+        out["fp64_grad_vs_oracle_max_rel"] = max(float(np.max(np.abs(a - r_)) / np.max(np.abs(r_)))
+                                                  for a, r_ in zip(g64, gr))
+        out["mixed_grad_vs_oracle_max_rel"] = max(float(np.max(np.abs(a - r_)) / np.max(np.abs(r_)))
+                                                   for a, r_ in zip(gmx, gr))
+    out.update(res)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +391,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-eam", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=1000)
+    ap.add_argument("--no-fit", action="store_true", help="skip the force-matching workloads (KA, EAM clusters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (spatial decomposition over "
@@ -293,6 +473,17 @@ def main():
         drift = {prec: drift_run("lj", lj, p, args.drift_steps) for prec, p in (("mixed", mdx.MIXED),
                                                                                ("fp64", mdx.FP64))}
 
+    fit_ka, replicas = None, None
+    if not args.no_fit:
+        log("force matching: KA lj/cut 64 x 8,000 atoms")
+        fit_ka = fit_lj_ka(check_oracle=not args.no_cpu_baseline)
+        log("  KA loss+grad %.2f ms (mixed), grad vs FD %.2e" % (fit_ka["mixed"]["ms_per_loss_and_grad"],
+                                                              fit_ka["mixed_grad_vs_fd_max_rel"]))
+        log("force matching: 512 eam clusters")
+        replicas = eam_replicas(check_oracle=not args.no_cpu_baseline)
+        log("  clusters forward %.2f ms, adjoint %.2f ms (mixed), grad vs FD %.2e"
+            % (replicas["mixed"]["ms_forward"], replicas["mixed"]["ms_adjoint"], replicas["mixed_grad_vs_fd_max_rel"]))
+
     out = {
         "metric": METRIC,
         "value": res["mixed"]["value"],
@@ -316,6 +507,8 @@ def main():
         "step_time_share": shares,
         "nve_drift": drift,
         "eam_256k": eam,
+        "force_matching_ka": fit_ka,
+        "eam_cluster_replicas": replicas,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -42,6 +42,9 @@ enum { MDX_FP64 = 0, MDX_MIXED = 1 };        /* pair arithmetic precision */
    (the default), or 16-bit slots into LDS-staged bricks of bins */
 enum { MDX_LIST_GATHER = 0, MDX_LIST_BRICK = 1 };
 enum { MDX_PAIR_NONE = 0, MDX_PAIR_LJ = 1, MDX_PAIR_EAM = 2 };
+/* neighbour positions read by the mixed-precision gather kernels: 32-bit
+   fixed point (16 B, the default) or the FP64 positions (32 B) */
+enum { MDX_POS_FP64 = 0, MDX_POS_Q32 = 1 };
 
 typedef struct mdx_ctx mdx_ctx;
 
@@ -186,6 +189,14 @@ int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
    brick size in bins (0 = default 8x4x4; shrunk automatically until the
    staged halo fits the LDS).  Takes effect at the next list build. */
 int mdx_set_list_mode(mdx_ctx *ctx, int mode, int rx, int ry, int rz);
+/* Position format of the mixed-precision gathers (gather list layout).
+   MDX_POS_Q32: each neighbour is read as 32-bit fixed point per dimension
+   (resolution prd * 2^-32 periodic, prd * 2^-30 otherwise) and its
+   separation rebuilt in FP64; pairs within 2e-6 relative of the cutoff are
+   re-tested on the FP64 positions, so the cutoff decision is the FP64 one.
+   MDX_POS_FP64: neighbours are read as FP64 positions.  FP64 precision
+   always reads FP64 positions. */
+int mdx_set_position_format(mdx_ctx *ctx, int format);
 
 #ifdef __cplusplus
 }

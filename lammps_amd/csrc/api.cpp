@@ -21,22 +21,27 @@ int hip_fail(hipError_t e, const char *what) {
   return e == hipErrorOutOfMemory ? MDX_ERR_NOMEM : MDX_ERR_HIP;
 }
 
+// Timing is best effort: if an event cannot be created or recorded the
+// pair is dropped (profiling switches itself off), never the launch.
 void prof_begin(mdx_ctx *c, int kind) {
+  c->prof_open = false;
   if (!c->profile) return;
   if (c->evused == c->evpool.size()) {
     mdx_ctx::EvRec r{};
-    hipEventCreate(&r.a);
-    hipEventCreate(&r.b);
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) {
+      c->profile = false;
+      return;
+    }
     c->evpool.push_back(r);
   }
   c->evpool[c->evused].kind = kind;
-  hipEventRecord(c->evpool[c->evused].a, c->stream);
+  c->prof_open = hipEventRecord(c->evpool[c->evused].a, c->stream) == hipSuccess;
 }
 
 void prof_end(mdx_ctx *c) {
-  if (!c->profile) return;
-  hipEventRecord(c->evpool[c->evused].b, c->stream);
-  c->evused++;
+  if (!c->profile || !c->prof_open) return;
+  if (hipEventRecord(c->evpool[c->evused].b, c->stream) == hipSuccess) c->evused++;
+  c->prof_open = false;
 }
 
 int prof_resolve(mdx_ctx *c) {
@@ -66,7 +71,7 @@ namespace {
 
 template <typename T>
 int dev_alloc(T **p, size_t count) {
-  if (*p) hipFree(*p);
+  if (*p) (void)hipFree(*p);
   *p = nullptr;
   if (count == 0) count = 1;
   hipError_t e = hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count);
@@ -90,9 +95,9 @@ int check_ctx(mdx_ctx *c) {
 void free_atoms(mdx_ctx *c) {
   void *ptrs[] = {c->x, c->xb, c->v, c->vb, c->f, c->xhold, c->tag, c->tagb, c->orig, c->origb, c->image,
                   c->imageb, c->inv, c->vatom, c->dscratch, c->bkey, c->bkey_s, c->bidx, c->bidx_s,
-                  c->numneigh, c->nbr, c->nbr16, c->rho, c->fp, c->eemb, c->partials};
+                  c->numneigh, c->nbr, c->nbr16, c->rho, c->fp, c->eemb, c->partials, c->xq, c->xqb};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   c->x = c->xb = c->v = c->vb = c->f = c->xhold = nullptr;
   c->tag = c->tagb = c->orig = c->origb = c->image = c->imageb = c->inv = nullptr;
   c->vatom = c->dscratch = nullptr;
@@ -100,8 +105,10 @@ void free_atoms(mdx_ctx *c) {
   c->bidx = c->bidx_s = c->numneigh = c->nbr = nullptr;
   c->nbr16 = nullptr;
   c->rho = c->fp = c->eemb = c->partials = nullptr;
-  if (c->host_stage) hipHostFree(c->host_stage);
-  if (c->host_istage) hipHostFree(c->host_istage);
+  c->xq = c->xqb = nullptr;
+  c->xq_stale = true;
+  if (c->host_stage) (void)hipHostFree(c->host_stage);
+  if (c->host_istage) (void)hipHostFree(c->host_istage);
   c->host_stage = nullptr;
   c->host_istage = nullptr;
   c->cap_atoms = 0;
@@ -200,7 +207,10 @@ int mdx_create(int device, mdx_ctx **out) {
     set_error("Insufficient memory on accelerator");
     return MDX_ERR_NOMEM;
   }
-  hipMemset(c->dflags, 0, 64);
+  if (hipMemset(c->dflags, 0, 64) != hipSuccess) {
+    set_error("device flags initialisation failed");
+    return MDX_ERR_HIP;
+  }
   // LDS available to the brick kernels' dynamic allocation, minus their
   // static tables and reductions
   hipDeviceProp_t prop;
@@ -212,21 +222,28 @@ int mdx_create(int device, mdx_ctx **out) {
 
 int mdx_destroy(mdx_ctx *c) {
   if (!c) return MDX_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_atoms(c);
   void *ptrs[] = {c->dflags, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->bin_count, c->bin_start,
                   c->sort_tmp, c->scan_tmp};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  if (c->red_host) hipHostFree(c->red_host);
+    if (p) (void)hipFree(p);
+  if (c->red_host) (void)hipHostFree(c->red_host);
   for (auto &r : c->evpool) {
-    hipEventDestroy(r.a);
-    hipEventDestroy(r.b);
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
   }
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MDX_OK;
+}
+
+// the fixed-point mirror is kept while mixed precision gathers it
+static void update_qpos(mdx_ctx *c) {
+  const bool on = c->precision == MDX_MIXED && c->list_mode == MDX_LIST_GATHER && c->pos_format == MDX_POS_Q32;
+  if (on && !c->qpos) c->xq_stale = true;
+  c->qpos = on;
 }
 
 int mdx_set_box(mdx_ctx *c, const double lo[3], const double hi[3], const int periodic[3]) {
@@ -245,7 +262,14 @@ int mdx_set_box(mdx_ctx *c, const double lo[3], const double hi[3], const int pe
     c->hi[d] = hi[d];
     c->prd[d] = hi[d] - lo[d];
     c->periodic[d] = periodic[d] ? 1 : 0;
+    // fixed-point mirror: a periodic length maps onto 2^32 (wrapping
+    // differences give the minimum image), a non-periodic one onto 2^30
+    const double span = c->periodic[d] ? 4294967296.0 : 1073741824.0;
+    c->qs.lo[d] = lo[d];
+    c->qs.scale[d] = span / c->prd[d];
+    c->qs.step[d] = c->prd[d] / span;
   }
+  c->xq_stale = true;
   c->box_set = true;
   c->built = false;
   c->run_setup_done = false;
@@ -285,8 +309,10 @@ int mdx_set_atoms(mdx_ctx *c, int n, int ntypes, const double *x, const double *
       (rc = dev_alloc(&c->vatom, (size_t)6 * npad)) || (rc = dev_alloc(&c->dscratch, (size_t)6 * npad)) ||
       (rc = dev_alloc(&c->bkey, npad)) || (rc = dev_alloc(&c->bkey_s, npad)) || (rc = dev_alloc(&c->bidx, npad)) ||
       (rc = dev_alloc(&c->bidx_s, npad)) || (rc = dev_alloc(&c->numneigh, npad)) ||
-      (rc = dev_alloc(&c->rho, npad)) || (rc = dev_alloc(&c->fp, npad)) || (rc = dev_alloc(&c->eemb, npad)))
+      (rc = dev_alloc(&c->rho, npad)) || (rc = dev_alloc(&c->fp, npad)) || (rc = dev_alloc(&c->eemb, npad)) ||
+      (rc = dev_alloc(&c->xq, npad)) || (rc = dev_alloc(&c->xqb, npad)))
     return rc;
+  c->xq_stale = true;
   if (hipHostMalloc(&c->host_stage, sizeof(double) * 6 * npad, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->host_istage, sizeof(int) * 3 * npad, hipHostMallocDefault) != hipSuccess) {
     set_error("pinned host allocation failed");
@@ -469,6 +495,19 @@ int mdx_set_precision(mdx_ctx *c, int precision) {
     return MDX_ERR_ARG;
   }
   c->precision = precision;
+  update_qpos(c);
+  return MDX_OK;
+}
+
+int mdx_set_position_format(mdx_ctx *c, int format) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (format != MDX_POS_FP64 && format != MDX_POS_Q32) {
+    set_error("position format must be MDX_POS_FP64 or MDX_POS_Q32");
+    return MDX_ERR_ARG;
+  }
+  c->pos_format = format;
+  update_qpos(c);
   return MDX_OK;
 }
 
@@ -670,6 +709,8 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
       ig.write_f = is_last ? 1 : 0;
       ig.v = c->v;
       ig.xn = c->xb;
+      ig.xqn = c->qpos ? c->xqb : nullptr;
+      ig.qs = c->qs;
       ig.xhold = c->xhold;
       ig.dflags = c->dflags;
       if (ig.check) {
@@ -686,7 +727,10 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
         for (int k = 0; k < 6; k++) row[2 + k] = c->virial[k];
         if (cb) cb(user, c->ntimestep, row);
       }
-      if (ig.next) std::swap(c->x, c->xb);
+      if (ig.next) {
+        std::swap(c->x, c->xb);
+        if (c->qpos) std::swap(c->xq, c->xqb);
+      }
       pending_initial = false;
     } else {
       if ((rc = force_step(c, out ? 1 : 0))) return rc;
@@ -749,13 +793,14 @@ int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
     return MDX_ERR_ARG;
   }
   if (mode != c->list_mode) {
-    hipFree(c->nbr);
-    hipFree(c->nbr16);
+    (void)hipFree(c->nbr);
+    (void)hipFree(c->nbr16);
     c->nbr = nullptr;
     c->nbr16 = nullptr;
     c->maxneigh = 0;
   }
   c->list_mode = mode;
+  update_qpos(c);
   const int def[3] = {8, 4, 4};
   c->brick_req[0] = rx ? rx : def[0];
   c->brick_req[1] = ry ? ry : def[1];

@@ -22,7 +22,8 @@ __global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double d
                                                     double4 *__restrict__ x, double4 *__restrict__ v,
                                                     const double4 *__restrict__ f,
                                                     const double4 *__restrict__ xhold, int check,
-                                                    double triggersq, int *__restrict__ dflags) {
+                                                    double triggersq, int *__restrict__ dflags,
+                                                    int4 *__restrict__ xq, QScale qs) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -38,6 +39,7 @@ __global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double d
   p.z += dtv * q.z;
   x[i] = p;
   v[i] = q;
+  if (xq) xq[i] = quantize(qs, p);
   if (check) {
     const double4 h = xhold[i];
     const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__
 
 // caller-order host positions -> device (sorted) order, type kept in w
 __global__ void k_scatter_x(int n, const double *__restrict__ hx, const int *__restrict__ orig,
-                            double4 *__restrict__ x) {
+                            double4 *__restrict__ x, int4 *__restrict__ xq, QScale qs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int o = orig[i];
@@ -109,6 +111,14 @@ __global__ void k_scatter_x(int n, const double *__restrict__ hx, const int *__r
   p.y = hx[3 * o + 1];
   p.z = hx[3 * o + 2];
   x[i] = p;
+  if (xq) xq[i] = quantize(qs, p);
+}
+
+// fixed-point mirror of all positions (after a change of precision or
+// position format)
+__global__ void k_quantize(int n, const double4 *__restrict__ x, int4 *__restrict__ xq, QScale qs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xq[i] = quantize(qs, x[i]);
 }
 
 // device order -> caller order [n][3]
@@ -164,7 +174,7 @@ int nve_initial(mdx_ctx *c, double dtv, double dtf, int check) {
   }
   prof_begin(c, MDX_K_NVE_INITIAL);
   hipLaunchKernelGGL(k_nve_initial, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtv, dtf, c->x, c->v, c->f,
-                     c->xhold, check, c->triggersq, c->dflags);
+                     c->xhold, check, c->triggersq, c->dflags, c->qpos ? c->xq : nullptr, c->qs);
   prof_end(c);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDX_OK : hip_fail(e, "nve initial");
@@ -216,8 +226,19 @@ int scatter_positions(mdx_ctx *c, const double *host_x) {
   double *dtmp = c->dscratch;
   hipError_t e = hipMemcpyAsync(dtmp, stage, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream);
   if (e != hipSuccess) return hip_fail(e, "positions upload");
-  hipLaunchKernelGGL(k_scatter_x, dim3(G), dim3(B), 0, c->stream, c->n, dtmp, c->orig, c->x);
+  hipLaunchKernelGGL(k_scatter_x, dim3(G), dim3(B), 0, c->stream, c->n, dtmp, c->orig, c->x,
+                     c->qpos ? c->xq : nullptr, c->qs);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "positions scatter");
+  return MDX_OK;
+}
+
+int refresh_xq(mdx_ctx *c) {
+  if (!c->qpos || !c->xq_stale || c->n == 0) return MDX_OK;
+  const int B = 256, G = (c->n + B - 1) / B;
+  hipLaunchKernelGGL(k_quantize, dim3(G), dim3(B), 0, c->stream, c->n, c->x, c->xq, c->qs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "fixed-point positions");
+  c->xq_stale = false;
   return MDX_OK;
 }
 
@@ -240,26 +261,30 @@ int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs) {
   }
   double4 *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, sizeof(double4) * n4) != hipSuccess || hipMalloc(&b, sizeof(double4) * n4) != hipSuccess) {
-    if (a) hipFree(a);
+    if (a) (void)hipFree(a);
     set_error("Insufficient memory on accelerator (copy probe)");
     return MDX_ERR_NOMEM;
   }
-  hipMemsetAsync(a, 0, sizeof(double4) * n4, c->stream);
   const int B = 256, G = 256 * 32;  // 32 workgroups per CU on 256 CUs
-  hipLaunchKernelGGL(k_copy, dim3(G), dim3(B), 0, c->stream, n4, a, b);  // warm-up
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, c->stream);
-  for (int it = 0; it < iters; it++) hipLaunchKernelGGL(k_copy, dim3(G), dim3(B), 0, c->stream, n4, a, b);
-  hipEventRecord(e1, c->stream);
-  hipError_t e = hipEventSynchronize(e1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   float ms = 0.0f;
-  hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
+  hipError_t e = hipMemsetAsync(a, 0, sizeof(double4) * n4, c->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_copy, dim3(G), dim3(B), 0, c->stream, n4, a, b);  // warm-up
+    e = hipEventRecord(e0, c->stream);
+  }
+  if (e == hipSuccess) {
+    for (int it = 0; it < iters; it++) hipLaunchKernelGGL(k_copy, dim3(G), dim3(B), 0, c->stream, n4, a, b);
+    e = hipEventRecord(e1, c->stream);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   if (e != hipSuccess) return hip_fail(e, "copy probe");
   *gbs = 2.0 * sizeof(double4) * (double)n4 * iters / (ms * 1e-3) / 1e9;
   return MDX_OK;

@@ -79,6 +79,28 @@ struct EAMArgs {
   const float *frho_f, *rhor_f, *z2r_f;
 };
 
+// 32-bit fixed-point positions read by the mixed-precision gathers (16 B
+// per neighbour instead of 32).  Per dimension q = round((x - lo) * scale)
+// modulo 2^32, with scale = 2^32 / prd in a periodic dimension, so that the
+// wrapping int32 difference q_i - q_j is the minimum-image separation, and
+// scale = 2^30 / prd otherwise; step = 1 / scale = prd * 2^-32 (or 2^-30) is
+// exact.  w holds the type.
+struct QScale {
+  double lo[3], scale[3], step[3];
+};
+
+__device__ __forceinline__ int qcoord(double x, double lo, double scale) {
+  return (int)(unsigned)(long long)rint((x - lo) * scale);
+}
+__device__ __forceinline__ int4 quantize(const QScale &q, const double4 &p) {
+  return make_int4(qcoord(p.x, q.lo[0], q.scale[0]), qcoord(p.y, q.lo[1], q.scale[1]),
+                   qcoord(p.z, q.lo[2], q.scale[2]), static_cast<int>(p.w));
+}
+// separation q_i - q_j of one dimension in length units (FP64, exact product)
+__device__ __forceinline__ double qdelta(int qi, int qj, double step) {
+  return (double)(int)((unsigned)qi - (unsigned)qj) * step;
+}
+
 // Velocity-Verlet work fused into the force kernel (fix nve final integrate
 // of this step, then initial integrate of the next, src/fix_nve.cpp:65-142)
 struct IntegArgs {
@@ -89,6 +111,8 @@ struct IntegArgs {
   int write_f;   // store f (last step of a run, energy steps)
   double4 *v;
   double4 *xn;
+  int4 *xqn;  // fixed-point copy of xn (mixed gathers), or null
+  QScale qs;
   const double4 *xhold;
   int *dflags;
 };
@@ -112,6 +136,13 @@ struct mdx_ctx {
   int n = 0, npad = 0, ntypes = 0, cap_atoms = 0;
   double mass[MDX_NT1] = {0};
   double4 *x = nullptr, *xb = nullptr, *v = nullptr, *vb = nullptr, *f = nullptr, *xhold = nullptr;
+  // fixed-point positions for the mixed-precision gathers (xqb: the fused
+  // path's next-step buffer); maintained by every position writer while qpos
+  int4 *xq = nullptr, *xqb = nullptr;
+  mdx::QScale qs{};
+  int pos_format = MDX_POS_Q32;  // mixed precision gathers: MDX_POS_Q32 or MDX_POS_FP64
+  bool qpos = false;             // xq is maintained (mixed precision, gather layout, Q32)
+  bool xq_stale = true;          // xq must be rebuilt from x before the next force call
   int *tag = nullptr, *tagb = nullptr, *orig = nullptr, *origb = nullptr, *image = nullptr, *imageb = nullptr;
   int *inv = nullptr;  // caller index -> device index
   double *vatom = nullptr;
@@ -176,6 +207,7 @@ struct mdx_ctx {
   };
   std::vector<EvRec> evpool;
   size_t evused = 0;
+  bool prof_open = false;  // a begin event was recorded for the current launch
   double kernel_ms[MDX_NKERNEL] = {0};
   long kernel_n[MDX_NKERNEL] = {0};
   long total_neighbors = 0;
@@ -227,6 +259,7 @@ int nve_final(mdx_ctx *c, double dtf);
 int reduce_ke(mdx_ctx *c);
 int finish_reduction(mdx_ctx *c, int nblocks, int nslots);
 int scatter_positions(mdx_ctx *c, const double *host_x);
+int refresh_xq(mdx_ctx *c);  // rebuild the fixed-point positions if stale
 int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs);
 int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out);
 

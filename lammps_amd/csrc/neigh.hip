@@ -117,13 +117,15 @@ __global__ void k_permute(int n, const int *__restrict__ src, const double4 *__r
                           const double4 *__restrict__ v, const int *__restrict__ tag,
                           const int *__restrict__ orig, const int *__restrict__ image, double4 *__restrict__ xo,
                           double4 *__restrict__ vo, int *__restrict__ tago, int *__restrict__ origo,
-                          int *__restrict__ imageo, int *__restrict__ inv, double4 *__restrict__ xhold) {
+                          int *__restrict__ imageo, int *__restrict__ inv, double4 *__restrict__ xhold,
+                          int4 *__restrict__ xq, QScale qs) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int o = src[k];
   double4 p = x[o];
   xo[k] = p;
   xhold[k] = p;
+  if (xq) xq[k] = quantize(qs, p);
   vo[k] = v[o];
   tago[k] = tag[o];
   int oi = orig[o];
@@ -234,10 +236,124 @@ __global__ __launch_bounds__(256) void k_build_full(NeighArgs a, const double4 *
   }
 }
 
+// One lane per atom (the default build).  A lane walks ~300 candidates, so
+// one gather in flight at a time leaves it latency bound: candidates are
+// taken four at a time (four gathers in flight), and accepted entries are
+// stored four at a time as one 16-B store into the row.  Candidate order,
+// keep test and tie rule are those of k_build_full, so the rows are
+// identical entry for entry.
+__global__ __launch_bounds__(256) void k_build_lane(NeighArgs a, const double4 *__restrict__ x,
+                                                   const unsigned *__restrict__ bkey,
+                                                   const int *__restrict__ bin_start, int *__restrict__ nbr,
+                                                   int *__restrict__ numneigh, int *__restrict__ dflags,
+                                                   unsigned long long *__restrict__ nsum) {
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const double4 xi = x[i];
+  const int ti = static_cast<int>(xi.w);
+  const unsigned b = bkey[i];
+  const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
+  const int ix = b % nbx, iy = (b / nbx) % nby, iz = b / (nbx * nby);
+  int *row = nbr + (size_t)i * a.maxneigh;  // maxneigh is a multiple of 4: rows are 16-B aligned
+  int cnt = 0;
+  int q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // pending entries cnt&~3 .. cnt-1
+  for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
+    int jz = iz + dz, sz = 0;
+    if (jz < 0) {
+      if (!a.periodic[2]) continue;
+      jz += nbz;
+      sz = -1;
+    } else if (jz >= nbz) {
+      if (!a.periodic[2]) continue;
+      jz -= nbz;
+      sz = 1;
+    }
+    for (int dy = -a.S[1]; dy <= a.S[1]; dy++) {
+      int jy = iy + dy, sy = 0;
+      if (jy < 0) {
+        if (!a.periodic[1]) continue;
+        jy += nby;
+        sy = -1;
+      } else if (jy >= nby) {
+        if (!a.periodic[1]) continue;
+        jy -= nby;
+        sy = 1;
+      }
+      const int rowbase = (jz * nby + jy) * nbx;
+      const int x0 = ix - a.S[0], x1 = ix + a.S[0];
+      const double shy = sy * a.prd[1], shz = sz * a.prd[2];
+      for (int s = 0; s < 3; s++) {
+        int lo, hi, sx;
+        if (s == 0) {
+          if (!(x0 < 0 && a.periodic[0])) continue;
+          lo = max(x0 + nbx, 0);
+          hi = nbx - 1;
+          sx = -1;
+        } else if (s == 1) {
+          lo = max(x0, 0);
+          hi = min(x1, nbx - 1);
+          sx = 0;
+        } else {
+          if (!(x1 >= nbx && a.periodic[0])) continue;
+          lo = 0;
+          hi = min(x1 - nbx, nbx - 1);
+          sx = 1;
+        }
+        if (lo > hi) continue;
+        const double shx = sx * a.prd[0];
+        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        const int j0 = bin_start[rowbase + lo];
+        const int j1 = bin_start[rowbase + hi + 1];
+        for (int base = j0; base < j1; base += 4) {
+          double4 xj[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) xj[u] = x[min(base + u, j1 - 1)];  // in range; unused tail ignored
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int j = base + u;
+            if (j >= j1 || (code == MDX_NOSHIFT && j == i)) continue;
+            const int tj = static_cast<int>(xj[u].w);
+            const double g0 = xj[u].x + shx, g1 = xj[u].y + shy, g2 = xj[u].z + shz;
+            const double c = a.cutneighsq[ti * a.ntypes1 + tj];
+            bool keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
+            if (code != MDX_NOSHIFT) {
+              const bool keep_b = rsq_exact(xj[u].x, xj[u].y, xj[u].z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
+              if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+            }
+            if (keep) {
+              const int e = j | (code << MDX_IDX_BITS);
+              const int slot = cnt & 3;
+              q0 = slot == 0 ? e : q0;
+              q1 = slot == 1 ? e : q1;
+              q2 = slot == 2 ? e : q2;
+              q3 = slot == 3 ? e : q3;
+              cnt++;
+              if (slot == 3 && cnt <= a.maxneigh) *reinterpret_cast<int4 *>(row + cnt - 4) = make_int4(q0, q1, q2, q3);
+            }
+          }
+        }
+      }
+    }
+  }
+  // entries after the last full group of four
+  const int tail = cnt & 3, t0 = cnt - tail;
+  if (tail > 0 && t0 < a.maxneigh) row[t0] = q0;
+  if (tail > 1 && t0 + 1 < a.maxneigh) row[t0 + 1] = q1;
+  if (tail > 2 && t0 + 2 < a.maxneigh) row[t0 + 2] = q2;
+  numneigh[i] = cnt;
+  atomicMax(&dflags[0], cnt);
+  atomicAdd(nsum, (unsigned long long)cnt);
+}
+
 template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
+  if (TPA == 1) {
+    hipLaunchKernelGGL(k_build_lane, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start, c->nbr,
+                       c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
+    return;
+  }
   hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
                      c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
 }
@@ -279,15 +395,15 @@ int neigh_setup_grid(mdx_ctx *c) {
   }
   c->nbins = (int)nb;
   if (c->nbins + 1 > c->cap_bins) {
-    hipFree(c->bin_count);
-    hipFree(c->bin_start);
+    (void)hipFree(c->bin_count);
+    (void)hipFree(c->bin_start);
     c->cap_bins = c->nbins + 1;
     if (hipMalloc(&c->bin_count, sizeof(int) * c->cap_bins) != hipSuccess ||
         hipMalloc(&c->bin_start, sizeof(int) * c->cap_bins) != hipSuccess) {
       set_error("Insufficient memory on accelerator (bins)");
       return MDX_ERR_NOMEM;
     }
-    hipFree(c->scan_tmp);
+    (void)hipFree(c->scan_tmp);
     c->scan_tmp = nullptr;
     c->scan_tmp_bytes = 0;
   }
@@ -297,9 +413,10 @@ int neigh_setup_grid(mdx_ctx *c) {
 
 // (re)allocates the list of the current layout with `cap` entries per atom
 int alloc_list(mdx_ctx *c, int cap) {
-  cap = std::max(cap, 16);
-  hipFree(c->nbr);
-  hipFree(c->nbr16);
+  // a multiple of 4 entries: rows start 16-B aligned (k_build_lane's stores)
+  cap = (std::max(cap, 16) + 3) / 4 * 4;
+  (void)hipFree(c->nbr);
+  (void)hipFree(c->nbr16);
   c->nbr = nullptr;
   c->nbr16 = nullptr;
   hipError_t e;
@@ -343,10 +460,10 @@ int neigh_build(mdx_ctx *c) {
   // stable radix sort of (bin, current index): deterministic order in a bin
   const int eb = end_bit_for(c->nbins);
   size_t need = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->bkey, c->bkey_s, c->bidx, c->bidx_s, c->n, 0, eb,
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->bkey, c->bkey_s, c->bidx, c->bidx_s, c->n, 0, eb,
                                      c->stream);
   if (need > c->sort_tmp_bytes) {
-    hipFree(c->sort_tmp);
+    (void)hipFree(c->sort_tmp);
     c->sort_tmp_bytes = need;
     if (hipMalloc(&c->sort_tmp, need) != hipSuccess) {
       set_error("Insufficient memory on accelerator (sort)");
@@ -357,9 +474,9 @@ int neigh_build(mdx_ctx *c) {
                                          eb, c->stream);
   if (e != hipSuccess) return hip_fail(e, "radix sort");
   need = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
   if (need > c->scan_tmp_bytes) {
-    hipFree(c->scan_tmp);
+    (void)hipFree(c->scan_tmp);
     c->scan_tmp_bytes = need;
     if (hipMalloc(&c->scan_tmp, need) != hipSuccess) {
       set_error("Insufficient memory on accelerator (scan)");
@@ -369,7 +486,9 @@ int neigh_build(mdx_ctx *c) {
   e = hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
   if (e != hipSuccess) return hip_fail(e, "bin scan");
   hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx_s, c->x, c->v, c->tag, c->orig,
-                     c->image, c->xb, c->vb, c->tagb, c->origb, c->imageb, c->inv, c->xhold);
+                     c->image, c->xb, c->vb, c->tagb, c->origb, c->imageb, c->inv, c->xhold,
+                     c->qpos ? c->xq : nullptr, c->qs);
+  if (c->qpos) c->xq_stale = false;
   std::swap(c->x, c->xb);
   std::swap(c->v, c->vb);
   std::swap(c->tag, c->tagb);

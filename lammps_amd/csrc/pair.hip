@@ -111,6 +111,7 @@ __device__ __forceinline__ double integrate_atom(int i, const double4 &xi, doubl
     p.y += ig.dtv * q.y;
     p.z += ig.dtv * q.z;
     ig.xn[i] = p;
+    if (ig.xqn) ig.xqn[i] = quantize(ig.qs, p);
     if (ig.check) {
       const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
       if (dx * dx + dy * dy + dz * dz > ig.triggersq) atomicOr(&ig.dflags[1], 1);
@@ -151,16 +152,28 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
 }
 
 // ------------------------------------------------------------------ LJ
-template <bool MIXED, bool EV, int TPA, bool INTEG>
+// Q (mixed only): neighbours are read from the 32-bit fixed-point mirror xq
+// (16 B instead of 32), their separation rebuilt in FP64 from the wrapping
+// integer difference; a pair whose rsq lies within kBand of the cutoff is
+// re-tested on the FP64 positions with its image shift, so the cutoff
+// decision is the FP64 one (the quantised rsq is within ~1e-7 relative).
+constexpr double kBand = 2e-6;
+
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
 __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
-                                           const double4 *__restrict__ x, const int *__restrict__ nbr,
-                                           const int *__restrict__ numneigh, const LJTab *__restrict__ tab,
-                                           double4 *__restrict__ f, double *__restrict__ vatom,
-                                           double *__restrict__ partials, IntegArgs ig) {
+                                           const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
+                                           const int *__restrict__ nbr, const int *__restrict__ numneigh,
+                                           const LJTab *__restrict__ tab, double4 *__restrict__ f,
+                                           double *__restrict__ vatom, double *__restrict__ partials, IntegArgs ig) {
   __shared__ double s_cutsq[kNT2], s_lj1[kNT2], s_lj2[kNT2], s_lj3[kNT2], s_lj4[kNT2], s_off[kNT2];
+  __shared__ double s_clo[Q ? kNT2 : 1], s_chi[Q ? kNT2 : 1];
   __shared__ float f_lj1[kNT2], f_lj2[kNT2], f_lj3[kNT2], f_lj4[kNT2], f_off[kNT2];
   const int nt2 = nt1 * nt1;
   for (int q = threadIdx.x; q < nt2; q += blockDim.x) {
+    if (Q) {
+      s_clo[q] = tab->cutsq[q] * (1.0 - kBand);
+      s_chi[q] = tab->cutsq[q] * (1.0 + kBand);
+    }
     s_cutsq[q] = tab->cutsq[q];
     s_lj1[q] = tab->lj1[q];
     s_lj2[q] = tab->lj2[q];
@@ -186,6 +199,7 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
   }
   if (i < n) {
     xi = x[i];
+    const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
     const int row = static_cast<int>(xi.w) * nt1;
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
@@ -193,19 +207,40 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
     // are issued before the first use, so a lane has UNROLL gathers in flight
     for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
       int jc[kUnroll];
-      double4 xj[kUnroll];
+      double4 xj[Q ? 1 : kUnroll];
+      int4 qj[Q ? kUnroll : 1];
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) xj[u] = x[jc[u] & MDX_IDX_MASK];
+      for (int u = 0; u < kUnroll; u++) {
+        if (Q) qj[u] = xq[jc[u] & MDX_IDX_MASK];
+        else xj[Q ? 0 : u] = x[jc[u] & MDX_IDX_MASK];
+      }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
-        double delx, dely, delz;
-        pair_delta(xi, xj[u], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
-        const double rsq = delx * delx + dely * dely + delz * delz;
-        const int t = row + static_cast<int>(xj[u].w);
-        if (rsq < s_cutsq[t]) {
+        double delx, dely, delz, rsq;
+        int t;
+        bool inside;
+        if (Q) {
+          delx = qdelta(qi.x, qj[u].x, qs.step[0]);
+          dely = qdelta(qi.y, qj[u].y, qs.step[1]);
+          delz = qdelta(qi.z, qj[u].z, qs.step[2]);
+          rsq = delx * delx + dely * dely + delz * delz;
+          t = row + qj[u].w;
+          inside = rsq < s_clo[t];
+          if (!inside && rsq < s_chi[t]) {
+            double ex, ey, ez;
+            pair_delta(xi, x[jc[u] & MDX_IDX_MASK], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, ex, ey, ez);
+            inside = ex * ex + ey * ey + ez * ez < s_cutsq[t];
+          }
+        } else {
+          pair_delta(xi, xj[Q ? 0 : u], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
+          rsq = delx * delx + dely * dely + delz * delz;
+          t = row + static_cast<int>(xj[Q ? 0 : u].w);
+          inside = rsq < s_cutsq[t];
+        }
+        if (inside) {
           double fpair, evdwl = 0.0;
           if (MIXED) {
             // hardware reciprocal (1 ulp): well inside the FP32 pair-math budget
@@ -253,36 +288,66 @@ __device__ __forceinline__ void spline_index(T r, T rdr, int nr, int &m, T &p) {
   p = p < T(1) ? p : T(1);
 }
 
-template <bool MIXED, bool EV, int TPA>
+// a pair inside the cutoff, for the FP64/mixed gathers (Q false) or the
+// fixed-point gathers with the FP64 re-test near the cutoff (Q true)
+template <bool Q>
+__device__ __forceinline__ bool pair_inside(const double4 &xi, const int4 &qi, const double4 *__restrict__ x,
+                                            int entry, const double4 &xj, const int4 &qj, const QScale &qs,
+                                            double px, double py, double pz, double cutsq, double clo, double chi,
+                                            double &delx, double &dely, double &delz, double &rsq) {
+  if (Q) {
+    delx = qdelta(qi.x, qj.x, qs.step[0]);
+    dely = qdelta(qi.y, qj.y, qs.step[1]);
+    delz = qdelta(qi.z, qj.z, qs.step[2]);
+    rsq = delx * delx + dely * dely + delz * delz;
+    if (rsq < clo) return true;
+    if (!(rsq < chi)) return false;
+    double ex, ey, ez;
+    pair_delta(xi, x[entry & MDX_IDX_MASK], (unsigned)entry >> MDX_IDX_BITS, px, py, pz, ex, ey, ez);
+    return ex * ex + ey * ey + ez * ez < cutsq;
+  }
+  pair_delta(xi, xj, (unsigned)entry >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
+  rsq = delx * delx + dely * dely + delz * delz;
+  return rsq < cutsq;
+}
+
+template <bool MIXED, bool EV, int TPA, bool Q>
 __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, double px, double py, double pz, EAMArgs a,
-                                                const double4 *__restrict__ x, const int *__restrict__ nbr,
-                                                const int *__restrict__ numneigh, double *__restrict__ rho_out,
-                                                double *__restrict__ fp_out, double *__restrict__ eemb) {
+                                                const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
+                                                const int *__restrict__ nbr, const int *__restrict__ numneigh,
+                                                double *__restrict__ rho_out, double *__restrict__ fp_out,
+                                                double *__restrict__ eemb) {
   const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   const int lane = threadIdx.x % TPA;
   double rho = 0.0;
   int ti = 0;
+  const double clo = a.cutforcesq * (1.0 - kBand), chi = a.cutforcesq * (1.0 + kBand);
   if (i < n) {
     const double4 xi = x[i];
+    const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
     ti = static_cast<int>(xi.w);
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
     for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
       int jc[kUnroll];
-      double4 xj[kUnroll];
+      double4 xj[Q ? 1 : kUnroll];
+      int4 qj[Q ? kUnroll : 1];
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) xj[u] = x[jc[u] & MDX_IDX_MASK];
+      for (int u = 0; u < kUnroll; u++) {
+        if (Q) qj[u] = xq[jc[u] & MDX_IDX_MASK];
+        else xj[Q ? 0 : u] = x[jc[u] & MDX_IDX_MASK];
+      }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
-        double delx, dely, delz;
-        pair_delta(xi, xj[u], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
-        const double rsq = delx * delx + dely * dely + delz * delz;
-        if (rsq < a.cutforcesq) {
-          const int tab = a.type2rhor[static_cast<int>(xj[u].w) * a.ntypes1 + ti];
+        double delx, dely, delz, rsq;
+        if (pair_inside<Q>(xi, qi, x, jc[u], xj[Q ? 0 : u], qj[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
+                           delx, dely, delz, rsq)) {
+          const int tj = Q ? qj[Q ? u : 0].w : static_cast<int>(xj[Q ? 0 : u].w);
+          const int tab = a.type2rhor[tj * a.ntypes1 + ti];
           int m;
           if (MIXED) {
             float p;
@@ -319,9 +384,10 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
   }
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG>
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
 __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
                                                   EAMArgs a, const double4 *__restrict__ x,
+                                                  const int4 *__restrict__ xq, QScale qs,
                                                   const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                                   const double *__restrict__ fp, const double *__restrict__ eemb,
                                                   double4 *__restrict__ f, double *__restrict__ vatom,
@@ -336,8 +402,10 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
     vpre = ig.v[i];
     if (ig.check) hpre = ig.xhold[i];
   }
+  const double clo = a.cutforcesq * (1.0 - kBand), chi = a.cutforcesq * (1.0 + kBand);
   if (i < n) {
     xi = x[i];
+    const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
     const int ti = static_cast<int>(xi.w);
     const double fpi = fp[i];
     const int jn = numneigh[i];
@@ -345,25 +413,25 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
     const int stride = (a.nr + 1) * 7;
     for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
       int jc[kUnroll];
-      double4 xjv[kUnroll];
+      double4 xjv[Q ? 1 : kUnroll];
+      int4 qjv[Q ? kUnroll : 1];
       double fpjv[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         const int jj = jc[u] & MDX_IDX_MASK;
-        xjv[u] = x[jj];
+        if (Q) qjv[u] = xq[jj];
+        else xjv[Q ? 0 : u] = x[jj];
         fpjv[u] = fp[jj];
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
       if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
-      const double4 xj = xjv[u];
-      double delx, dely, delz;
-      pair_delta(xi, xj, (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < a.cutforcesq) {
-        const int tj = static_cast<int>(xj.w);
+      double delx, dely, delz, rsq;
+      if (pair_inside<Q>(xi, qi, x, jc[u], xjv[Q ? 0 : u], qjv[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
+                         delx, dely, delz, rsq)) {
+        const int tj = Q ? qjv[Q ? u : 0].w : static_cast<int>(xjv[Q ? 0 : u].w);
         const int tij = ti * a.ntypes1 + tj, tji = tj * a.ntypes1 + ti;
         double fpair, phi;
         if (MIXED) {
@@ -424,42 +492,43 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
                               partials);
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG>
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
 void launch_lj(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
-                     c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->nbr, c->numneigh, c->lj_dev, c->f,
-                     c->vatom, c->partials, ig);
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+                     c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->xq, c->qs, c->nbr, c->numneigh,
+                     c->lj_dev, c->f, c->vatom, c->partials, ig);
   prof_end(c);
   c->last_grid = G;
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG>
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
 void launch_eam(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, MDX_K_EAM_RHO);
-  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
-                     c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
+  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+                     c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->rho, c->fp,
+                     c->eemb);
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
-  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
-                     c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->nbr, c->numneigh, c->fp, c->eemb,
-                     c->f, c->vatom, c->partials, ig);
+  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
+                     c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh,
+                     c->fp, c->eemb, c->f, c->vatom, c->partials, ig);
   prof_end(c);
   c->last_grid = G;
 }
 
-template <bool MIXED, bool EV, bool INTEG>
+template <bool MIXED, bool EV, bool INTEG, bool Q>
 int dispatch_tpa(mdx_ctx *c, int B, const IntegArgs &ig) {
   const bool lj = c->pair_style == MDX_PAIR_LJ;
   switch (c->launch.tpa) {
-    case 1: lj ? launch_lj<MIXED, EV, 1, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 1, INTEG>(c, B, ig); break;
-    case 2: lj ? launch_lj<MIXED, EV, 2, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 2, INTEG>(c, B, ig); break;
-    case 4: lj ? launch_lj<MIXED, EV, 4, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 4, INTEG>(c, B, ig); break;
-    case 8: lj ? launch_lj<MIXED, EV, 8, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 8, INTEG>(c, B, ig); break;
-    case 16: lj ? launch_lj<MIXED, EV, 16, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 16, INTEG>(c, B, ig); break;
-    case 32: lj ? launch_lj<MIXED, EV, 32, INTEG>(c, B, ig) : launch_eam<MIXED, EV, 32, INTEG>(c, B, ig); break;
+    case 1: lj ? launch_lj<MIXED, EV, 1, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 1, INTEG, Q>(c, B, ig); break;
+    case 2: lj ? launch_lj<MIXED, EV, 2, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 2, INTEG, Q>(c, B, ig); break;
+    case 4: lj ? launch_lj<MIXED, EV, 4, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 4, INTEG, Q>(c, B, ig); break;
+    case 8: lj ? launch_lj<MIXED, EV, 8, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 8, INTEG, Q>(c, B, ig); break;
+    case 16: lj ? launch_lj<MIXED, EV, 16, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 16, INTEG, Q>(c, B, ig); break;
+    case 32: lj ? launch_lj<MIXED, EV, 32, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 32, INTEG, Q>(c, B, ig); break;
     default:
       set_error("threads per atom must be one of 1, 2, 4, 8, 16, 32");
       return MDX_ERR_ARG;
@@ -470,9 +539,15 @@ int dispatch_tpa(mdx_ctx *c, int B, const IntegArgs &ig) {
 template <bool INTEG>
 int dispatch_prec(mdx_ctx *c, int ev, const IntegArgs &ig) {
   const int B = c->launch.block;
-  if (c->precision == MDX_MIXED)
-    return ev ? dispatch_tpa<true, true, INTEG>(c, B, ig) : dispatch_tpa<true, false, INTEG>(c, B, ig);
-  return ev ? dispatch_tpa<false, true, INTEG>(c, B, ig) : dispatch_tpa<false, false, INTEG>(c, B, ig);
+  if (c->precision == MDX_MIXED) {
+    if (c->qpos) {
+      int rc = refresh_xq(c);
+      if (rc) return rc;
+      return ev ? dispatch_tpa<true, true, INTEG, true>(c, B, ig) : dispatch_tpa<true, false, INTEG, true>(c, B, ig);
+    }
+    return ev ? dispatch_tpa<true, true, INTEG, false>(c, B, ig) : dispatch_tpa<true, false, INTEG, false>(c, B, ig);
+  }
+  return ev ? dispatch_tpa<false, true, INTEG, false>(c, B, ig) : dispatch_tpa<false, false, INTEG, false>(c, B, ig);
 }
 
 }  // namespace

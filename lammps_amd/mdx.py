@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdx.so")
 
 FP64, MIXED = 0, 1
+POS_FP64, POS_Q32 = 0, 1
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_OVERFLOW = 0, -1, -2, -3, -4
 
 _d = ctypes.c_double
@@ -70,6 +71,7 @@ def lib():
             "mdx_set_list_mode": [_p, _i, _i, _i, _i],
             "mdx_set_fused": [_p, _i],
             "mdx_set_profiling": [_p, _i],
+            "mdx_set_position_format": [_p, _i],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -198,6 +200,10 @@ class Context:
 
     def precision(self, p):
         self._ck(self._L.mdx_set_precision(self.h, p))
+
+    def position_format(self, fmt):
+        """POS_Q32 (default) or POS_FP64 neighbour positions in the mixed gathers."""
+        self._ck(self._L.mdx_set_position_format(self.h, fmt))
 
     def neigh_modify(self, skin, every=1, delay=10, check=True, one=2000):
         self._ck(self._L.mdx_neigh_modify(self.h, skin, every, delay, 1 if check else 0, one))

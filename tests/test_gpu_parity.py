@@ -61,13 +61,21 @@ def test_lj_neighbor_pairs_identical(lj_state):
     assert st["total_neighbors"] == 2 * len(ref_set)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_lj_forces_energy_virial(lj_state, prec):
+def precision_of(prec, ctx=None):
+    """fp64 | mixed (fixed-point neighbour positions, the default) | mixed_fp64pos"""
     from lammps_amd import mdx
+    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
+    if ctx is not None:
+        ctx.position_format(mdx.POS_FP64 if prec == "mixed_fp64pos" else mdx.POS_Q32)
+    return p
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_fp64pos"])
+def test_lj_forces_energy_virial(lj_state, prec):
     r = lj_state
     lo, hi = box_of(r, 20, A_LJ)
-    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
-    ctx = device_from_oracle(r, lo, hi, "lj", p, 0.3, 20, 0, False)
+    ctx = device_from_oracle(r, lo, hi, "lj", precision_of(prec), 0.3, 20, 0, False)
+    precision_of(prec, ctx)
     out = ctx.compute(3, 5, peratom=True)
     _, _, fref, _, _ = r.state()
     eref, wref = r.energy_virial()
@@ -154,13 +162,12 @@ def test_eam_neighbor_pairs_identical(eam_state):
     assert got == ref
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_fp64pos"])
 def test_eam_forces_energy_virial(eam_state, prec):
-    from lammps_amd import mdx
     r = eam_state
     lo, hi = box_of(r, 20, 3.615)
-    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
-    ctx = device_from_oracle(r, lo, hi, "eam", p, 1.0, 1, 5, True)
+    ctx = device_from_oracle(r, lo, hi, "eam", precision_of(prec), 1.0, 1, 5, True)
+    precision_of(prec, ctx)
     out = ctx.compute(3, 5, peratom=True)
     _, _, fref, _, _ = r.state()
     eref, wref = r.energy_virial()
@@ -250,3 +257,58 @@ def test_eam_bench_run_matches_reference_log():
     assert st["total_neighbors"] == 2 * g["total_neighbors"]
     assert st["ncalls"] == g["builds"]
     assert st["ndanger"] == g["dangerous"]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_fp64pos"])
+def test_lj_nonperiodic_box_forces(lj_state, prec):
+    """boundary f f f: the periodic crystal's atoms in a larger closed box (no
+    images); the fixed-point mirror then maps the box onto 2^30."""
+    from lammps_amd import mdx
+    r0 = lj_state
+    x, _, _, t, _ = r0.state()
+    L = 20 * A_LJ
+    lo, hi = (-1.0, -1.5, -2.0), (L + 1.0, L + 1.5, L + 2.0)
+    x = np.mod(x, L)
+    r = M.Ref(M.LJ)
+    r.box(lo, hi, periodic=(0, 0, 0))
+    r.add_atoms(x, t)
+    r.mass(1, 1.0)
+    r.pair_lj(2.5)
+    r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    r.neighbor(0.3)
+    r.setup()
+    _, _, fref, _, _ = r.state()
+    ctx = mdx.Context(0)
+    ctx.set_box(lo, hi, (0, 0, 0))
+    ctx.set_atoms(x, t, np.array([0.0, 1.0]), tags=np.arange(1, len(t) + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    ctx.precision(precision_of(prec))
+    precision_of(prec, ctx)
+    ctx.neigh_modify(0.3)
+    out = ctx.compute(1, 1)
+    mx, rms = force_errors(out["f"], fref)
+    print("LJ non-periodic %s force max_rel=%.3e rms_rel=%.3e" % (prec, mx, rms))
+    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
+    eref, _ = r.energy_virial()
+    assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
+
+
+@pytest.mark.parametrize("build_tpa", [4, 8])
+def test_list_build_variants_give_identical_rows(lj_state, build_tpa):
+    """The one-lane build (four gathers in flight, 16-B row stores) and the
+    TPA-lane ballot build produce the same rows entry for entry: FP64 forces
+    from either list are bitwise equal."""
+    from lammps_amd import mdx
+    r = lj_state
+    lo, hi = box_of(r, 20, A_LJ)
+    out = []
+    for tpa in (1, build_tpa):
+        ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+        ctx.set_build_launch(tpa)
+        ctx.neigh_build()
+        pairs = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+        out.append((pairs, ctx.compute(1, 1)["f"], ctx.stats()["total_neighbors"]))
+        ctx.close()
+    assert out[0][0] == out[1][0]
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][1], out[1][1])
