@@ -82,6 +82,7 @@ def run_case(kind, inp, precision, steps, warmup):
     ftm2v = mdx.UNIT_CONSTANTS[units]["ftm2v"]
     ctx = make_ctx(kind, inp, precision)
     ctx.setup()
+    launch = ctx.autotune()  # threads per atom and block of the pair kernels
     if warmup:
         ctx.run(warmup, dt, ftm2v, 0)
     st0 = ctx.stats()
@@ -96,7 +97,7 @@ def run_case(kind, inp, precision, steps, warmup):
     res = dict(n=n, seconds=secs, ms_per_step=1e3 * secs / steps, value=n * steps / secs / 1e6,
                builds=st["ncalls"] - st0["ncalls"], total_neighbors=st["total_neighbors"],
                max_neighbors=st["max_neighbors"], kernel_ms=st["kernel_ms"], kernel_launches=st["kernel_launches"],
-               end_row=rows[-1][1] if rows else None, ctx=ctx)
+               end_row=rows[-1][1] if rows else None, ctx=ctx, launch=launch)
     names = ("build", "lj", "eam_rho", "eam_force", "nve_initial", "nve_final", "fused")
     log("  kernels (ms/step, launches): " + ", ".join(
         "%s %.4f (%d)" % (nm, st["kernel_ms"][k] / steps, st["kernel_launches"][k])
@@ -499,7 +500,8 @@ def main():
         "data": "synthetic (seeded fcc lattice + velocity create, as the reference's own commands)",
         "config": {"workload": "lj/cut fcc 80^3 = 2,048,000 atoms, rho*=0.8442, rc=2.5, skin=0.3, "
                                "every 20 delay 0 check yes, T*=1.44 gaussian seed 87287, NVE dt=0.005",
-                   "precision": "mixed", "atoms": n, "builds_in_timed_steps": res["mixed"]["builds"]},
+                   "precision": "mixed", "atoms": n, "builds_in_timed_steps": res["mixed"]["builds"],
+                   "autotuned_launch": res["mixed"]["launch"]},
         "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"]},
         "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
         "force_error_vs_fp64_oracle": errors,

@@ -185,6 +185,12 @@ int mdx_copy_bandwidth(mdx_ctx *ctx, long nbytes, int iters, double *gbs);
 int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
 /* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default). */
 int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
+int mdx_get_launch(mdx_ctx *ctx, int *block, int *tpa, int *build_tpa);
+/* Autotuner (the role of `package gpu ... tpa` chosen by hand in the
+   reference, src/GPU/fix_gpu.cpp:112-150): times the pair kernels on the
+   current list for threads per atom {4, 8, 16} x block {128, 256}, `reps`
+   launches each, keeps the fastest and recomputes the forces with it. */
+int mdx_autotune(mdx_ctx *ctx, int reps);
 /* List layout (MDX_LIST_GATHER or MDX_LIST_BRICK) and, for bricks, the
    brick size in bins (0 = default 8x4x4; shrunk automatically until the
    staged halo fits the LDS).  Takes effect at the next list build. */

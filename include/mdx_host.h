@@ -45,6 +45,19 @@ typedef struct {
 int mdx_host_eam_funcfl(const char *path, mdx_eam_info *info, double *frho_spline, double *rhor_spline,
                         double *z2r_spline);
 
+/* eam/alloy (fs = 0) or eam/fs (fs = 1) DYNAMO setfl file, with atom type t
+   on the element named type_elements[t-1] (pair_coeff * * file E1 .. En),
+   as PairEAMAlloy/PairEAMFS read_file + coeff + file2array leave the
+   tables (src/MANYBODY/pair_eam_alloy.cpp:52-330, pair_eam_fs.cpp:52-330):
+   nfrho = nelements + 1 (a zero table last), nrhor = nelements (alloy) or
+   nelements^2 (fs, [i*n + j] = file block i, function j), nz2r =
+   n(n+1)/2 (lower triangle, file order).  Call with NULL spline arrays for
+   the sizes.  type2frho[ntypes+1], type2rhor/type2z2r[(ntypes+1)^2] and
+   mass[ntypes+1] (element masses per type) may be NULL. */
+int mdx_host_eam_setfl(const char *path, int fs, int ntypes, const char *const *type_elements, mdx_eam_info *info,
+                       double *frho_spline, double *rhor_spline, double *z2r_spline, int *type2frho, int *type2rhor,
+                       int *type2z2r, double *mass);
+
 /* Writes a funcfl file tabulating F(rho) = -E0 sqrt(rho/rho_e),
    Z(r) = Z0 exp(-beta r) and rho(r) = fe exp(-chi (r/re - 1)) on nrho/nr
    points, cutoff `cut`; rho_e is the perfect-fcc density at lattice

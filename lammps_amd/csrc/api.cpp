@@ -781,6 +781,68 @@ int mdx_set_launch(mdx_ctx *c, int block, int tpa) {
   return MDX_OK;
 }
 
+int mdx_get_launch(mdx_ctx *c, int *block, int *tpa, int *build_tpa) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (block) *block = c->launch.block;
+  if (tpa) *tpa = c->launch.tpa;
+  if (build_tpa) *build_tpa = c->launch.build_tpa;
+  return MDX_OK;
+}
+
+// Times the pair kernels (no energy/virial) on the current list for each
+// (threads per atom, block) candidate, keeps the fastest, then recomputes
+// the forces with it so f is that of the chosen shape.
+int mdx_autotune(mdx_ctx *c, int reps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (reps < 1) reps = 3;
+  if ((rc = ensure_built(c))) return rc;
+  if (c->list_mode != MDX_LIST_GATHER) return MDX_OK;  // the brick kernels have one shape
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return hip_fail(e, "autotune events");
+  if ((e = hipEventCreate(&e1)) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_fail(e, "autotune events");
+  }
+  const mdx::Launch keep = c->launch;
+  const bool prof = c->profile;
+  c->profile = false;
+  float best = 1e30f;
+  mdx::Launch pick = keep;
+  for (int tpa : {4, 8, 16})
+    for (int block : {128, 256}) {
+      c->launch.tpa = tpa;
+      c->launch.block = block;
+      if ((rc = pair_compute(c, 0))) break;  // warm-up
+      e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = pair_compute(c, 0);
+      if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (rc || e != hipSuccess) break;
+      if (ms < best) {
+        best = ms;
+        pick = c->launch;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->profile = prof;
+  if (rc) {
+    c->launch = keep;
+    return rc;
+  }
+  if (e != hipSuccess) {
+    c->launch = keep;
+    return hip_fail(e, "autotune");
+  }
+  c->launch = pick;
+  return pair_compute(c, 0);
+}
+
 int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
   int rc = check_ctx(c);
   if (rc) return rc;

@@ -306,6 +306,122 @@ extern "C" int mdx_host_eam_funcfl(const char *path, mdx_eam_info *info, double 
   return MDX_OK;
 }
 
+namespace {
+// PairEAM::grab semantics: n values taken from whole lines
+bool grab(std::istream &in, int n, double *out) {
+  std::string line;
+  int i = 0;
+  while (i < n) {
+    if (!std::getline(in, line)) return false;
+    std::istringstream s(line);
+    double v;
+    while (i < n && (s >> v)) out[i++] = v;
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" int mdx_host_eam_setfl(const char *path, int fs, int ntypes, const char *const *type_elements,
+                                  mdx_eam_info *info, double *frho_spline, double *rhor_spline, double *z2r_spline,
+                                  int *type2frho, int *type2rhor, int *type2z2r, double *mass) {
+  if (!path || !info || ntypes < 1 || ntypes > 8 || !type_elements) return fail("invalid setfl arguments");
+  std::ifstream in(path);
+  if (!in) return fail("Cannot open EAM potential file");
+  std::string line;
+  for (int k = 0; k < 3; k++)
+    if (!std::getline(in, line)) return fail("setfl header");
+  if (!std::getline(in, line)) return fail("setfl header");
+  std::vector<std::string> names;
+  int nel = 0;
+  {
+    std::istringstream s(line);
+    if (!(s >> nel)) return fail("Incorrect element names in EAM potential file");
+    std::string w;
+    while (s >> w) names.push_back(w);
+    if (nel < 1 || nel > 8 || (int)names.size() != nel) return fail("Incorrect element names in EAM potential file");
+  }
+  int nrho, nr;
+  double drho, dr, cut;
+  if (!std::getline(in, line)) return fail("setfl line 5");
+  {
+    std::istringstream s(line);
+    if (!(s >> nrho >> drho >> nr >> dr >> cut)) return fail("setfl line 5");
+  }
+  if (nrho < 5 || nr < 5 || !(drho > 0) || !(dr > 0) || !(cut > 0)) return fail("setfl table sizes");
+  const int per_rho = fs ? nel : 1;
+  std::vector<double> el_mass(nel), F((size_t)nel * nrho), R((size_t)nel * per_rho * nr),
+      Z((size_t)nel * (nel + 1) / 2 * nr);
+  for (int i = 0; i < nel; i++) {
+    if (!std::getline(in, line)) return fail("setfl element line");
+    std::istringstream s(line);
+    int z;
+    if (!(s >> z >> el_mass[i])) return fail("setfl element line");
+    if (!grab(in, nrho, &F[(size_t)i * nrho])) return fail("setfl table truncated");
+    for (int j = 0; j < per_rho; j++)
+      if (!grab(in, nr, &R[((size_t)i * per_rho + j) * nr])) return fail("setfl table truncated");
+  }
+  for (size_t k = 0; k < (size_t)nel * (nel + 1) / 2; k++)
+    if (!grab(in, nr, &Z[k * nr])) return fail("setfl table truncated");
+  // pair_coeff * * file E1 .. En: every type on an element (pair_eam_alloy.cpp:52-106)
+  std::vector<int> map(ntypes + 1, 0);
+  for (int t = 1; t <= ntypes; t++) {
+    const char *want = type_elements[t - 1];
+    int e = -1;
+    for (int k = 0; k < nel && want; k++)
+      if (names[k] == want) e = k;
+    if (e < 0) return fail("No matching element in EAM potential file (NULL types need pair hybrid, not supported)");
+    map[t] = e;
+  }
+  // file2array (pair_eam_alloy.cpp:220-330, pair_eam_fs.cpp): tables as read
+  info->nrho = nrho;
+  info->nr = nr;
+  info->nfrho = nel + 1;
+  info->nrhor = nel * per_rho;
+  info->nz2r = nel * (nel + 1) / 2;
+  info->drho = drho;
+  info->dr = dr;
+  info->rhomax = (nrho - 1) * drho;
+  info->cutmax = cut;
+  info->mass = el_mass[map[1]];
+  if (!frho_spline || !rhor_spline || !z2r_spline) return MDX_OK;
+  std::memset(frho_spline, 0, sizeof(double) * info->nfrho * (nrho + 1) * 7);
+  std::memset(rhor_spline, 0, sizeof(double) * info->nrhor * (nr + 1) * 7);
+  std::memset(z2r_spline, 0, sizeof(double) * info->nz2r * (nr + 1) * 7);
+  std::vector<double> tab;
+  for (int i = 0; i <= nel; i++) {  // + the zero table of non-EAM types
+    tab.assign(nrho + 1, 0.0);
+    if (i < nel)
+      for (int m = 1; m <= nrho; m++) tab[m] = F[(size_t)i * nrho + m - 1];
+    spline7(nrho, drho, tab, frho_spline + (size_t)i * (nrho + 1) * 7);
+  }
+  for (int k = 0; k < info->nrhor; k++) {
+    tab.assign(nr + 1, 0.0);
+    for (int m = 1; m <= nr; m++) tab[m] = R[(size_t)k * nr + m - 1];
+    spline7(nr, dr, tab, rhor_spline + (size_t)k * (nr + 1) * 7);
+  }
+  for (int k = 0; k < info->nz2r; k++) {
+    tab.assign(nr + 1, 0.0);
+    for (int m = 1; m <= nr; m++) tab[m] = Z[(size_t)k * nr + m - 1];
+    spline7(nr, dr, tab, z2r_spline + (size_t)k * (nr + 1) * 7);
+  }
+  const int nt1 = ntypes + 1;
+  for (int i = 0; i < nt1; i++) {
+    if (type2frho) type2frho[i] = i ? map[i] : 0;
+    if (mass) mass[i] = i ? el_mass[map[i]] : 0.0;
+    for (int j = 0; j < nt1; j++) {
+      int rh = 0, zz = 0;
+      if (i && j) {
+        rh = fs ? map[i] * nel + map[j] : map[i];
+        int irow = std::max(map[i], map[j]), icol = std::min(map[i], map[j]);
+        zz = irow * (irow + 1) / 2 + icol;
+      }
+      if (type2rhor) type2rhor[i * nt1 + j] = rh;
+      if (type2z2r) type2z2r[i * nt1 + j] = zz;
+    }
+  }
+  return MDX_OK;
+}
+
 extern "C" int mdx_host_write_synthetic_funcfl(const char *path, int seed, double a, int nrho, int nr, double dr,
                                                double cut, double mass, double *params_out) {
   if (!path || seed <= 0 || !(a > 0) || nrho < 5 || nr < 5 || !(dr > 0) || !(cut > 0)) return fail("invalid arguments");

@@ -6,7 +6,7 @@
 // short-range path (src/input.cpp:532-600 dispatch): variable (index,
 // equal), units, atom_style atomic, boundary, lattice, region block,
 // create_box, create_atoms, mass, velocity create, pair_style
-// lj/cut|eam, pair_coeff, pair_modify shift, neighbor, neigh_modify,
+// lj/cut|eam|eam/alloy|eam/fs, pair_coeff, pair_modify shift, neighbor, neigh_modify,
 // fix nve, timestep, thermo, run, plus `mdx` (precision, threads per atom,
 // list layout, fusion).  Output follows the reference's log: thermo rows
 // "Step Temp E_pair E_mol TotEng Press" (thermo.cpp: %8d, %12.8g), the

@@ -176,9 +176,49 @@ void PairEAM::upload(mdx_ctx *ctx) {
                           t2r.data(), t2z.data()));
 }
 
+// ---------------------------------------------------------------- eam/alloy, eam/fs
+void PairEAMSetfl::settings(const std::vector<std::string> &args) {
+  if (!args.empty()) throw Error("Illegal pair_style command");
+}
+
+void PairEAMSetfl::coeff(const std::vector<std::string> &args) {
+  if (!ntypes) throw Error("Pair coeff for a style without atom types");
+  if ((int)args.size() != 3 + ntypes) throw Error("Incorrect args for pair coefficients");
+  if (args[0] != "*" || args[1] != "*") throw Error("Incorrect args for pair coefficients");
+  file = args[2];
+  std::vector<const char *> names;
+  for (int t = 0; t < ntypes; t++) names.push_back(args[3 + t].c_str());
+  check(mdx_host_eam_setfl(file.c_str(), fs_ ? 1 : 0, ntypes, names.data(), &info, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr));
+  const int nt1 = ntypes + 1;
+  frho.assign((size_t)info.nfrho * (info.nrho + 1) * 7, 0.0);
+  rhor.assign((size_t)info.nrhor * (info.nr + 1) * 7, 0.0);
+  z2r.assign((size_t)info.nz2r * (info.nr + 1) * 7, 0.0);
+  t2f.assign(nt1, 0);
+  t2r.assign(nt1 * nt1, 0);
+  t2z.assign(nt1 * nt1, 0);
+  mass.assign(nt1, 0.0);
+  check(mdx_host_eam_setfl(file.c_str(), fs_ ? 1 : 0, ntypes, names.data(), &info, frho.data(), rhor.data(),
+                           z2r.data(), t2f.data(), t2r.data(), t2z.data(), mass.data()));
+}
+
+void PairEAMSetfl::init_style() {
+  if (file.empty()) throw Error("All pair coeffs are not set");
+}
+
+double PairEAMSetfl::init_one(int, int) { return info.cutmax; }
+
+void PairEAMSetfl::upload(mdx_ctx *ctx) {
+  check(mdx_pair_eam_init(ctx, ntypes, info.nrho, info.drho, info.nr, info.dr, info.rhomax, info.cutmax * info.cutmax,
+                          info.nfrho, frho.data(), info.nrhor, rhor.data(), info.nz2r, z2r.data(), t2f.data(),
+                          t2r.data(), t2z.data()));
+}
+
 Pair *create_pair(const std::string &style) {
   if (style == "lj/cut" || style == "lj/cut/hip") return new PairLJCut();
   if (style == "eam" || style == "eam/hip") return new PairEAM();
+  if (style == "eam/alloy") return new PairEAMSetfl(false);
+  if (style == "eam/fs") return new PairEAMSetfl(true);
   throw Error("Invalid pair style: " + style);
 }
 

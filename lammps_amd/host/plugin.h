@@ -91,7 +91,29 @@ class PairEAM : public Pair {
   std::vector<double> frho, rhor, z2r;
 };
 
-Pair *create_pair(const std::string &style);  // "lj/cut" | "eam" (force.cpp:128-147 style map)
+// pair_style eam/alloy | eam/fs: one DYNAMO setfl file, pair_coeff * * file
+// E1 .. En with one element name per atom type (pair_eam_alloy.cpp:52-106,
+// pair_eam_fs.cpp:52-106); each type takes its element's mass
+class PairEAMSetfl : public Pair {
+ public:
+  explicit PairEAMSetfl(bool fs) : fs_(fs) {}
+  void settings(const std::vector<std::string> &args) override;
+  void coeff(const std::vector<std::string> &args) override;
+  void init_style() override;
+  double init_one(int i, int j) override;
+  void upload(mdx_ctx *ctx) override;
+  bool sets_mass() const override { return true; }
+  double mass_of(int type) const override { return type < (int)mass.size() ? mass[type] : 0.0; }
+ protected:
+  void allocate(int n) override { ntypes = n; }
+  bool fs_;
+  std::string file;
+  mdx_eam_info info{};
+  std::vector<double> frho, rhor, z2r, mass;
+  std::vector<int> t2f, t2r, t2z;
+};
+
+Pair *create_pair(const std::string &style);  // lj/cut | eam | eam/alloy | eam/fs (force.cpp:128-147 style map)
 
 // `neighbor <skin> bin` + `neigh_modify` (src/neighbor.cpp:1731-1862)
 struct Neighbor {

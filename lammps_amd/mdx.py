@@ -72,6 +72,8 @@ def lib():
             "mdx_set_fused": [_p, _i],
             "mdx_set_profiling": [_p, _i],
             "mdx_set_position_format": [_p, _i],
+            "mdx_get_launch": [_p, _ip, _ip, _ip],
+            "mdx_autotune": [_p, _i],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -214,6 +216,16 @@ class Context:
     def set_build_launch(self, tpa=0):
         self._ck(self._L.mdx_set_build_launch(self.h, tpa))
 
+    def get_launch(self):
+        v = np.zeros(3, dtype=np.int32)
+        self._ck(self._L.mdx_get_launch(self.h, _ptr(v[0:], _ip), _ptr(v[1:], _ip), _ptr(v[2:], _ip)))
+        return dict(block=int(v[0]), tpa=int(v[1]), build_tpa=int(v[2]))
+
+    def autotune(self, reps=3):
+        """Pick threads per atom and block of the pair kernels by timing (mdx_autotune)."""
+        self._ck(self._L.mdx_autotune(self.h, reps))
+        return self.get_launch()
+
     def list_mode(self, mode, brick=(0, 0, 0)):
         """mode 'gather' or 'brick' (LDS-staged bricks of brick[0]x[1]x[2] bins)."""
         m = {"gather": 0, "brick": 1}[mode]
@@ -299,6 +311,9 @@ def _host_lib():
         L.mdx_host_eam_funcfl.restype = _i
         L.mdx_host_write_synthetic_funcfl.argtypes = [ctypes.c_char_p, _i, _d, _i, _i, _d, _d, _d, _dp]
         L.mdx_host_write_synthetic_funcfl.restype = _i
+        L.mdx_host_eam_setfl.argtypes = [ctypes.c_char_p, _i, _i, ctypes.c_void_p, ctypes.POINTER(EamInfo), _dp, _dp,
+                                         _dp, _ip, _ip, _ip, _dp]
+        L.mdx_host_eam_setfl.restype = _i
         L.mdx_host_ranpark_uniform.argtypes = [_ip]
         L.mdx_host_ranpark_uniform.restype = _d
         L._host_ready = True
@@ -347,6 +362,38 @@ def eam_funcfl(path):
     _host_ck(L.mdx_host_eam_funcfl(path.encode(), ctypes.byref(info), _ptr(fr), _ptr(rr), _ptr(zr)))
     return dict(nrho=info.nrho, nr=info.nr, drho=info.drho, dr=info.dr, rhomax=info.rhomax, cutmax=info.cutmax,
                 mass=info.mass, frho_spline=fr, rhor_spline=rr, z2r_spline=zr)
+
+
+def eam_setfl(path, elements, fs=False):
+    """pair_style eam/alloy (fs=False) or eam/fs + pair_coeff * * path elements...:
+    sizes, spline arrays, type maps and per-type masses (mdx_host_eam_setfl)."""
+    L = _host_lib()
+    nt = len(elements)
+    names = (ctypes.c_char_p * nt)(*[e.encode() for e in elements])
+    info = EamInfo()
+    args = (path.encode(), 1 if fs else 0, nt, ctypes.cast(names, ctypes.c_void_p), ctypes.byref(info))
+    _host_ck(L.mdx_host_eam_setfl(*args, None, None, None, None, None, None, None))
+    fr = np.zeros((info.nfrho, info.nrho + 1, 7))
+    rr = np.zeros((info.nrhor, info.nr + 1, 7))
+    zr = np.zeros((info.nz2r, info.nr + 1, 7))
+    nt1 = nt + 1
+    t2f = np.zeros(nt1, dtype=np.int32)
+    t2r = np.zeros(nt1 * nt1, dtype=np.int32)
+    t2z = np.zeros(nt1 * nt1, dtype=np.int32)
+    mass = np.zeros(nt1)
+    _host_ck(L.mdx_host_eam_setfl(*args, _ptr(fr), _ptr(rr), _ptr(zr), _ptr(t2f, _ip), _ptr(t2r, _ip),
+                                  _ptr(t2z, _ip), _ptr(mass)))
+    return dict(nrho=info.nrho, nr=info.nr, drho=info.drho, dr=info.dr, rhomax=info.rhomax, cutmax=info.cutmax,
+                frho_spline=fr, rhor_spline=rr, z2r_spline=zr, type2frho=t2f, type2rhor=t2r, type2z2r=t2z,
+                mass=mass)
+
+
+def eam_context_setfl(ctx, path, elements, fs=False):
+    """pair_style eam/alloy|eam/fs + pair_coeff * * path elements on a context; returns the tables."""
+    t = eam_setfl(path, elements, fs)
+    ctx.pair_eam(len(elements), t["nrho"], t["drho"], t["nr"], t["dr"], t["rhomax"], t["cutmax"] ** 2,
+                 t["frho_spline"], t["rhor_spline"], t["z2r_spline"], t["type2frho"], t["type2rhor"], t["type2z2r"])
+    return t
 
 
 def write_synthetic_funcfl(path, seed=4951, a=3.615, nrho=500, nr=500, dr=0.01, cut=4.95, mass=63.55):

@@ -109,6 +109,12 @@ struct mdref {
   double ff_drho, ff_dr, ff_cut, ff_mass;
   double *ff_frho, *ff_zr, *ff_rhor;
   int eam_map[MAXT + 1];
+  /* eam/alloy (sf_kind 1) or eam/fs (sf_kind 2) setfl file */
+  int sf_kind, sf_nel, sf_nrho, sf_nr;
+  double sf_drho, sf_dr, sf_cut, sf_mass[MAXT];
+  double *sf_frho; /* [nel][nrho] */
+  double *sf_rhor; /* alloy [nel][nr], fs [nel][nel][nr] */
+  double *sf_z2r;  /* [nel*(nel+1)/2][nr], i >= j in file order */
   int nrho, nr, nfrho, nrhor, nz2r;
   double drho, dr, rdr, rdrho, rhomax, cutmax, cutforcesq;
   double *frho, *rhor, *z2r;                         /* [n][nr+1] */
@@ -241,6 +247,7 @@ void mdref_free(mdref *s) {
   free(s->xhold); free(s->binhead); free(s->bins); free(s->stencil);
   free(s->ilist); free(s->numneigh); free(s->firstneigh); free(s->neighbors);
   free(s->ff_frho); free(s->ff_zr); free(s->ff_rhor);
+  free(s->sf_frho); free(s->sf_rhor); free(s->sf_z2r);
   free(s->frho); free(s->rhor); free(s->z2r);
   free(s->frho_spline); free(s->rhor_spline); free(s->z2r_spline);
   free(s->rho); free(s->fp); free(s->eatom); free(s->vatom);
@@ -566,6 +573,7 @@ int mdref_pair_eam_file(mdref *s, const char *path) {
   }
   fclose(fp);
   s->pair_style = MDREF_PAIR_EAM;
+  s->sf_kind = 0;
   /* pair_coeff * * file: every type maps to file 0 and takes its mass
      (pair_eam.cpp:346-391) */
   for (int t = 1; t <= s->ntypes; t++) {
@@ -573,6 +581,86 @@ int mdref_pair_eam_file(mdref *s, const char *path) {
     s->mass[t] = s->ff_mass;
     s->mass_set[t] = 1;
   }
+  return 0;
+}
+
+/* PairEAM::grab (pair_eam.cpp:760-772): n values from whole lines */
+static int grab_lines(FILE *fp, int n, double *out) {
+  char line[4096];
+  int i = 0;
+  while (i < n) {
+    if (!fgets(line, sizeof(line), fp)) return -1;
+    char *tok = strtok(line, " \t\n\r\f");
+    while (tok && i < n) {
+      out[i++] = atof(tok);
+      tok = strtok(NULL, " \t\n\r\f");
+    }
+  }
+  return 0;
+}
+
+/* PairEAMAlloy::read_file / PairEAMFS::read_file (pair_eam_alloy.cpp:113-214,
+   pair_eam_fs.cpp:113-214) and coeff's element mapping (:52-106): every
+   type is mapped to an element by name (NULL types, for pair hybrid, are
+   not supported) and takes its mass. */
+int mdref_pair_eam_setfl(mdref *s, const char *path, int fs, const char *const *type_elements) {
+  FILE *fp = fopen(path, "r");
+  if (!fp) return fail(s, "Cannot open EAM potential file");
+  char line[4096];
+  for (int k = 0; k < 3; k++)
+    if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(s, "setfl header"); }
+  if (!fgets(line, sizeof(line), fp)) { fclose(fp); return fail(s, "setfl header"); }
+  char names[MAXT][64];
+  int nel = 0;
+  {
+    char *tok = strtok(line, " \t\n\r\f");
+    int declared = tok ? atoi(tok) : 0;
+    while ((tok = strtok(NULL, " \t\n\r\f")) && nel < MAXT) {
+      snprintf(names[nel], sizeof(names[nel]), "%s", tok);
+      nel++;
+    }
+    if (declared != nel || nel < 1) { fclose(fp); return fail(s, "Incorrect element names in EAM potential file"); }
+  }
+  if (!fgets(line, sizeof(line), fp) ||
+      sscanf(line, "%d %lg %d %lg %lg", &s->sf_nrho, &s->sf_drho, &s->sf_nr, &s->sf_dr, &s->sf_cut) != 5) {
+    fclose(fp);
+    return fail(s, "setfl line 5");
+  }
+  const int nrho = s->sf_nrho, nr = s->sf_nr;
+  free(s->sf_frho); free(s->sf_rhor); free(s->sf_z2r);
+  s->sf_frho = (double *)calloc((size_t)nel * nrho, sizeof(double));
+  s->sf_rhor = (double *)calloc((size_t)nel * (fs ? nel : 1) * nr, sizeof(double));
+  s->sf_z2r = (double *)calloc((size_t)nel * (nel + 1) / 2 * nr, sizeof(double));
+  for (int i = 0; i < nel; i++) {
+    int z;
+    if (!fgets(line, sizeof(line), fp) || sscanf(line, "%d %lg", &z, &s->sf_mass[i]) != 2) {
+      fclose(fp);
+      return fail(s, "setfl element line");
+    }
+    if (grab_lines(fp, nrho, &s->sf_frho[(size_t)i * nrho])) { fclose(fp); return fail(s, "setfl truncated"); }
+    for (int j = 0; j < (fs ? nel : 1); j++)
+      if (grab_lines(fp, nr, &s->sf_rhor[((size_t)i * (fs ? nel : 1) + j) * nr])) {
+        fclose(fp);
+        return fail(s, "setfl truncated");
+      }
+  }
+  int n = 0;
+  for (int i = 0; i < nel; i++)
+    for (int j = 0; j <= i; j++, n++)
+      if (grab_lines(fp, nr, &s->sf_z2r[(size_t)n * nr])) { fclose(fp); return fail(s, "setfl truncated"); }
+  fclose(fp);
+  for (int t = 1; t <= s->ntypes; t++) {
+    int e = -1;
+    for (int k = 0; k < nel; k++)
+      if (strcmp(type_elements[t - 1], names[k]) == 0) e = k;
+    if (e < 0) return fail(s, "No matching element in EAM potential file");
+    s->eam_map[t] = e;
+    s->mass[t] = s->sf_mass[e];
+    s->mass_set[t] = 1;
+  }
+  s->sf_nel = nel;
+  s->sf_kind = fs ? 2 : 1;
+  s->pair_style = MDREF_PAIR_EAM;
   return 0;
 }
 
@@ -618,7 +706,50 @@ static double eam_regrid(double r, double fdelta, int fn, const double *tab) {
 
 /* pair_eam.cpp:397-405 (init_style) -> file2array 488-694 -> array2spline 698-719;
    restricted to one funcfl file */
+static void eam_spline_all(mdref *s);
+
+/* PairEAMAlloy::file2array / PairEAMFS::file2array (pair_eam_alloy.cpp:220-330):
+   tables copied as read; one zero frho for non-EAM types; type maps */
+static void eam_setfl_arrays(mdref *s) {
+  const int nel = s->sf_nel, nrho = s->sf_nrho, nr = s->sf_nr, fs = s->sf_kind == 2;
+  s->nrho = nrho;
+  s->nr = nr;
+  s->drho = s->sf_drho;
+  s->dr = s->sf_dr;
+  s->rhomax = (nrho - 1) * s->drho;
+  s->nfrho = nel + 1;
+  s->nrhor = fs ? nel * nel : nel;
+  s->nz2r = nel * (nel + 1) / 2;
+  free(s->frho); free(s->rhor); free(s->z2r);
+  s->frho = (double *)calloc((size_t)s->nfrho * (nrho + 1), sizeof(double));
+  s->rhor = (double *)calloc((size_t)s->nrhor * (nr + 1), sizeof(double));
+  s->z2r = (double *)calloc((size_t)s->nz2r * (nr + 1), sizeof(double));
+  for (int i = 0; i < nel; i++)
+    for (int m = 1; m <= nrho; m++) s->frho[(size_t)i * (nrho + 1) + m] = s->sf_frho[(size_t)i * nrho + m - 1];
+  for (int k = 0; k < s->nrhor; k++)
+    for (int m = 1; m <= nr; m++) s->rhor[(size_t)k * (nr + 1) + m] = s->sf_rhor[(size_t)k * nr + m - 1];
+  for (int k = 0; k < s->nz2r; k++)
+    for (int m = 1; m <= nr; m++) s->z2r[(size_t)k * (nr + 1) + m] = s->sf_z2r[(size_t)k * nr + m - 1];
+  for (int i = 1; i <= s->ntypes; i++) {
+    s->type2frho[i] = s->eam_map[i];
+    for (int j = 1; j <= s->ntypes; j++) {
+      s->type2rhor[i][j] = fs ? s->eam_map[i] * nel + s->eam_map[j] : s->eam_map[i];
+      int irow = s->eam_map[i], icol = s->eam_map[j];
+      if (irow < icol) {
+        irow = s->eam_map[j];
+        icol = s->eam_map[i];
+      }
+      s->type2z2r[i][j] = irow * (irow + 1) / 2 + icol;
+    }
+  }
+  eam_spline_all(s);
+}
+
 static void eam_init_style(mdref *s) {
+  if (s->sf_kind) {
+    eam_setfl_arrays(s);
+    return;
+  }
   s->dr = s->ff_dr;
   s->drho = s->ff_drho;
   double rmax = (s->ff_nr - 1) * s->ff_dr;
@@ -652,7 +783,11 @@ static void eam_init_style(mdref *s) {
   }
   for (int i = 1; i <= s->ntypes; i++)
     for (int j = 1; j <= s->ntypes; j++) s->type2z2r[i][j] = 0;
-  /* array2spline */
+  eam_spline_all(s);
+}
+
+/* array2spline (pair_eam.cpp:698-719) */
+static void eam_spline_all(mdref *s) {
   s->rdr = 1.0 / s->dr;
   s->rdrho = 1.0 / s->drho;
   free(s->frho_spline); free(s->rhor_spline); free(s->z2r_spline);
@@ -672,7 +807,7 @@ static void eam_init_style(mdref *s) {
 
 /* pair_eam.cpp:411-427 */
 static double eam_init_one(mdref *s) {
-  s->cutmax = s->ff_cut;
+  s->cutmax = s->sf_kind ? s->sf_cut : s->ff_cut;
   s->cutforcesq = s->cutmax * s->cutmax;
   return s->cutmax;
 }

@@ -45,6 +45,9 @@ int mdref_set_velocities(mdref *s, int n, const double *v);   /* by local index 
 int mdref_pair_lj(mdref *s, double cut_global, int shift);
 int mdref_pair_lj_coeff(mdref *s, int i, int j, double eps, double sigma, double cut /* <0: global */);
 int mdref_pair_eam_file(mdref *s, const char *path);   /* pair_coeff * * file (funcfl, 1 type) */
+/* pair_style eam/alloy (fs = 0) or eam/fs (fs = 1): pair_coeff * * file E1 .. En,
+   type_elements[t-1] = element name of type t */
+int mdref_pair_eam_setfl(mdref *s, const char *path, int fs, const char *const *type_elements);
 /* EAM table accessors after init (pair_eam.h public members) */
 int mdref_eam_info(mdref *s, int *nrho, double *drho, int *nr, double *dr, double *rhomax, double *cutmax,
                    int *nfrho, int *nrhor, int *nz2r);

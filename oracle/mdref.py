@@ -55,6 +55,7 @@ def lib():
             "mdref_pair_lj": [_p, _d, _i],
             "mdref_pair_lj_coeff": [_p, _i, _i, _d, _d, _d],
             "mdref_pair_eam_file": [_p, ctypes.c_char_p],
+            "mdref_pair_eam_setfl": [_p, ctypes.c_char_p, _i, ctypes.c_void_p],
             "mdref_neighbor": [_p, _d, _i, _i, _i, _i],
             "mdref_newton": [_p, _i],
             "mdref_list_style": [_p, _i],
@@ -160,6 +161,11 @@ class Ref:
 
     def pair_eam(self, path):
         self._ck(self._L.mdref_pair_eam_file(self.h, path.encode()))
+
+    def pair_eam_setfl(self, path, elements, fs=False):
+        """pair_style eam/alloy (or eam/fs) + pair_coeff * * path elements..."""
+        arr = (ctypes.c_char_p * len(elements))(*[e.encode() for e in elements])
+        self._ck(self._L.mdref_pair_eam_setfl(self.h, path.encode(), 1 if fs else 0, ctypes.cast(arr, ctypes.c_void_p)))
 
     def neighbor(self, skin, every=1, delay=10, check=True, one=2000):
         self._ck(self._L.mdref_neighbor(self.h, skin, every, delay, 1 if check else 0, one))
