@@ -39,8 +39,10 @@ enum {
 
 enum { MDX_FP64 = 0, MDX_MIXED = 1 };        /* pair arithmetic precision */
 /* neighbour-list layout: global 32-bit indices read by per-neighbour gathers
-   (the default), or 16-bit slots into LDS-staged bricks of bins */
-enum { MDX_LIST_GATHER = 0, MDX_LIST_BRICK = 1 };
+   (the default), 16-bit slots into LDS-staged bricks of bins, or one bit per
+   stencil candidate (the candidates re-enumerated from the bins by the pair
+   kernels; ~40 B per atom instead of 4 B per neighbour) */
+enum { MDX_LIST_GATHER = 0, MDX_LIST_BRICK = 1, MDX_LIST_MASK = 2 };
 enum { MDX_PAIR_NONE = 0, MDX_PAIR_LJ = 1, MDX_PAIR_EAM = 2 };
 /* neighbour positions read by the mixed-precision gather kernels: 32-bit
    fixed point (16 B, the default) or the FP64 positions (32 B) */
@@ -183,7 +185,10 @@ int mdx_copy_bandwidth(mdx_ctx *ctx, long nbytes, int iters, double *gbs);
    threads per atom (1, 2, 4, 8, 16 or 32; the `threads_per_atom` keyword of
    the reference's `package gpu`, src/GPU/fix_gpu.cpp:112-150). */
 int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
-/* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default). */
+/* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default), or
+   64: one wavefront per bin, the bin's candidates loaded once for all its
+   atoms (index and mask lists; stencils of up to 64 bin rows, otherwise
+   the one-lane build runs).  Every variant yields the same rows. */
 int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
 int mdx_get_launch(mdx_ctx *ctx, int *block, int *tpa, int *build_tpa);
 /* Autotuner (the role of `package gpu ... tpa` chosen by hand in the

@@ -5,6 +5,12 @@
 Every HIP translation unit is compiled with packed-FP32 instruction
 selection disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
 Objects go to lammps_amd/_build/, the library to lammps_amd/libmdx.so.
+
+libmdx.so needs the HIP runtime by its soname (libamdhip64.so.7).  PyTorch's
+ROCm wheel carries its own copy under that soname; when torch is imported
+first, the dynamic loader binds libmdx.so to torch's copy, so the process
+holds one HIP runtime (two copies in one process do not both see the GPU).
+The Python layer imports torch before loading the library for that reason.
 """
 import os
 import shutil

@@ -95,7 +95,7 @@ int check_ctx(mdx_ctx *c) {
 void free_atoms(mdx_ctx *c) {
   void *ptrs[] = {c->x, c->xb, c->v, c->vb, c->f, c->xhold, c->tag, c->tagb, c->orig, c->origb, c->image,
                   c->imageb, c->inv, c->vatom, c->dscratch, c->bkey, c->bkey_s, c->bidx, c->bidx_s,
-                  c->numneigh, c->nbr, c->nbr16, c->rho, c->fp, c->eemb, c->partials, c->xq, c->xqb};
+                  c->numneigh, c->nbr, c->nbr16, c->rho, c->fp, c->eemb, c->partials, c->xq, c->xqb, c->nmask};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->x = c->xb = c->v = c->vb = c->f = c->xhold = nullptr;
@@ -106,6 +106,8 @@ void free_atoms(mdx_ctx *c) {
   c->nbr16 = nullptr;
   c->rho = c->fp = c->eemb = c->partials = nullptr;
   c->xq = c->xqb = nullptr;
+  c->nmask = nullptr;
+  c->mwords = 0;
   c->xq_stale = true;
   if (c->host_stage) (void)hipHostFree(c->host_stage);
   if (c->host_istage) (void)hipHostFree(c->host_istage);
@@ -146,7 +148,9 @@ int prepare(mdx_ctx *c) {
   int rc = neigh_setup_grid(c);
   if (rc) return rc;
   // first-guess list capacity from the mean density; neigh_build regrows
-  const bool have_list = c->list_mode == MDX_LIST_BRICK ? c->nbr16 != nullptr : c->nbr != nullptr;
+  // (mask lists are sized by neigh_build)
+  const bool have_list = c->list_mode == MDX_LIST_MASK ||
+                         (c->list_mode == MDX_LIST_BRICK ? c->nbr16 != nullptr : c->nbr != nullptr);
   if (!have_list && c->n > 0) {
     const double vol = c->prd[0] * c->prd[1] * c->prd[2];
     const double rc3 = c->cutneighmax * c->cutneighmax * c->cutneighmax;
@@ -241,7 +245,7 @@ int mdx_destroy(mdx_ctx *c) {
 
 // the fixed-point mirror is kept while mixed precision gathers it
 static void update_qpos(mdx_ctx *c) {
-  const bool on = c->precision == MDX_MIXED && c->list_mode == MDX_LIST_GATHER && c->pos_format == MDX_POS_Q32;
+  const bool on = c->precision == MDX_MIXED && c->list_mode != MDX_LIST_BRICK && c->pos_format == MDX_POS_Q32;
   if (on && !c->qpos) c->xq_stale = true;
   c->qpos = on;
 }
@@ -566,6 +570,53 @@ int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
     brick_of.resize(n);
     for (int i = 0; i < n; i++) brick_of[i] = brick_of_bin(c->br, c->na, (int)bk[i]);
     for (int b = 0; b < c->br.nbricks; b++) brick_slot_map(c->na, c->br, b, bs.data(), slotmap[b]);
+  } else if (c->list_mode == MDX_LIST_MASK) {
+    // expand the candidate bits into index rows (host copy of the device walk)
+    std::vector<unsigned> bk(n), mk((size_t)c->mwords * n);
+    std::vector<int> bs(c->nbins + 1);
+    if (n > 0 &&
+        ((e = hipMemcpy(mk.data(), c->nmask, sizeof(unsigned) * mk.size(), hipMemcpyDeviceToHost)) != hipSuccess ||
+         (e = hipMemcpy(bk.data(), c->bkey_s, sizeof(unsigned) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
+         (e = hipMemcpy(bs.data(), c->bin_start, sizeof(int) * bs.size(), hipMemcpyDeviceToHost)) != hipSuccess))
+      return hip_fail(e, "list copy");
+    const int maxn = *std::max_element(num.begin(), num.end());
+    nb.assign((size_t)std::max(maxn, 1) * n, 0);
+    const NeighArgs &a = c->na;
+    const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
+    for (int i = 0; i < n; i++) {
+      const unsigned key = bk[i];
+      const int ix = key % a.nbin[0], iy = (key / a.nbin[0]) % a.nbin[1], iz = key / (a.nbin[0] * a.nbin[1]);
+      long bit = 0;
+      int k = 0;
+      for (int r = 0; r < nrow; r++) {
+        RowSegs rs;
+        if (!row_segments(a, bs.data(), ix, iy, iz, r, rs)) continue;
+        for (int sgi = 0; sgi < 3; sgi++)
+          for (int j = rs.j0[sgi]; j < rs.j1[sgi]; j++, bit++)
+            if ((mk[(size_t)i * c->mwords + bit / 32] >> (bit % 32)) & 1u) {
+              if (k < maxn) nb[(size_t)i * maxn + k] = j;
+              k++;
+            }
+      }
+      if (k != num[i]) {
+        set_error("mask list decode does not match the neighbour count (internal error)");
+        return MDX_ERR_ARG;
+      }
+    }
+    long cnt = 0;
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < num[i]; k++) {
+        const int j = nb[(size_t)i * maxn + k];
+        if (tag[i] < tag[j]) {
+          if (pairs && cnt < cap) {
+            pairs[2 * cnt] = tag[i];
+            pairs[2 * cnt + 1] = tag[j];
+          }
+          cnt++;
+        }
+      }
+    if (npairs) *npairs = cnt;
+    return MDX_OK;
   } else {
     nb.resize((size_t)c->maxneigh * c->npad);
     if ((e = hipMemcpy(nb.data(), c->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost)) != hipSuccess)
@@ -671,7 +722,7 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
   // integrate of n and the initial integrate of n+1 (writing x(n+1) to the
   // spare position buffer), so only the first step of a call runs a
   // standalone initial integrate and no step runs a standalone final one.
-  const bool fused = c->fuse && c->list_mode == MDX_LIST_GATHER;
+  const bool fused = c->fuse && c->list_mode != MDX_LIST_BRICK;
   bool pending_initial = true;
   auto is_decide = [c](int ago) { return ago >= c->delay && ago % c->every == 0; };
   for (int s = 0; s < nsteps; s++) {
@@ -798,7 +849,7 @@ int mdx_autotune(mdx_ctx *c, int reps) {
   if (rc) return rc;
   if (reps < 1) reps = 3;
   if ((rc = ensure_built(c))) return rc;
-  if (c->list_mode != MDX_LIST_GATHER) return MDX_OK;  // the brick kernels have one shape
+  if (c->list_mode == MDX_LIST_BRICK) return MDX_OK;  // the brick kernels have one shape
   hipEvent_t e0, e1;
   hipError_t e = hipEventCreate(&e0);
   if (e != hipSuccess) return hip_fail(e, "autotune events");
@@ -846,8 +897,8 @@ int mdx_autotune(mdx_ctx *c, int reps) {
 int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (mode != MDX_LIST_GATHER && mode != MDX_LIST_BRICK) {
-    set_error("list mode must be MDX_LIST_GATHER or MDX_LIST_BRICK");
+  if (mode != MDX_LIST_GATHER && mode != MDX_LIST_BRICK && mode != MDX_LIST_MASK) {
+    set_error("list mode must be MDX_LIST_GATHER, MDX_LIST_BRICK or MDX_LIST_MASK");
     return MDX_ERR_ARG;
   }
   if (rx < 0 || ry < 0 || rz < 0 || rx > 64 || ry > 64 || rz > 64) {
@@ -857,8 +908,11 @@ int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
   if (mode != c->list_mode) {
     (void)hipFree(c->nbr);
     (void)hipFree(c->nbr16);
+    (void)hipFree(c->nmask);
     c->nbr = nullptr;
     c->nbr16 = nullptr;
+    c->nmask = nullptr;
+    c->mwords = 0;
     c->maxneigh = 0;
   }
   c->list_mode = mode;
@@ -874,8 +928,8 @@ int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
 int mdx_set_build_launch(mdx_ctx *c, int tpa) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (tpa != 0 && tpa != 1 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32) {
-    set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
+  if (tpa != 0 && tpa != 1 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32 && tpa != 64) {
+    set_error("build threads per atom must be one of 1, 4, 8, 16, 32, or 64 (one wavefront per bin)");
     return MDX_ERR_ARG;
   }
   c->launch.build_tpa = tpa ? tpa : 1;

@@ -79,6 +79,76 @@ struct EAMArgs {
   const float *frho_f, *rhor_f, *z2r_f;
 };
 
+// Candidate segments of one stencil row of the atom in bin (ix, iy, iz).
+// Rows are numbered r = (dz + Sz)(2Sy + 1) + (dy + Sy), dz outer, the list
+// build's order; a row is at most three contiguous atom ranges (x wrapped
+// low, main, x wrapped high), each with its periodic image code
+// (sx+1) + 3(sy+1) + 9(sz+1).  A row outside a non-periodic box is empty.
+struct RowSegs {
+  int j0[3], j1[3], code[3];
+};
+__host__ __device__ inline int imax2(int a, int b) { return a > b ? a : b; }
+__host__ __device__ inline int imin2(int a, int b) { return a < b ? a : b; }
+
+__host__ __device__ inline int row_segments(const NeighArgs &a, const int *__restrict__ bin_start, int ix, int iy,
+                                            int iz, int r, RowSegs &rs) {
+  const int wy = 2 * a.S[1] + 1;
+  const int dz = r / wy - a.S[2], dy = r % wy - a.S[1];
+  const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
+#pragma unroll
+  for (int s = 0; s < 3; s++) rs.j0[s] = rs.j1[s] = rs.code[s] = 0;
+  int jz = iz + dz, sz = 0, jy = iy + dy, sy = 0;
+  if (jz < 0) {
+    if (!a.periodic[2]) return 0;
+    jz += nbz;
+    sz = -1;
+  } else if (jz >= nbz) {
+    if (!a.periodic[2]) return 0;
+    jz -= nbz;
+    sz = 1;
+  }
+  if (jy < 0) {
+    if (!a.periodic[1]) return 0;
+    jy += nby;
+    sy = -1;
+  } else if (jy >= nby) {
+    if (!a.periodic[1]) return 0;
+    jy -= nby;
+    sy = 1;
+  }
+  const int rowbase = (jz * nby + jy) * nbx;
+  const int x0 = ix - a.S[0], x1 = ix + a.S[0];
+  int total = 0;
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    int lo, hi, sx;
+    bool on;
+    if (s == 0) {
+      on = x0 < 0 && a.periodic[0];
+      lo = imax2(x0 + nbx, 0);
+      hi = nbx - 1;
+      sx = -1;
+    } else if (s == 1) {
+      on = true;
+      lo = imax2(x0, 0);
+      hi = imin2(x1, nbx - 1);
+      sx = 0;
+    } else {
+      on = x1 >= nbx && a.periodic[0];
+      lo = 0;
+      hi = imin2(x1 - nbx, nbx - 1);
+      sx = 1;
+    }
+    if (on && lo <= hi) {
+      rs.j0[s] = bin_start[rowbase + lo];
+      rs.j1[s] = bin_start[rowbase + hi + 1];
+      rs.code[s] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+      total += rs.j1[s] - rs.j0[s];
+    }
+  }
+  return total;
+}
+
 // 32-bit fixed-point positions read by the mixed-precision gathers (16 B
 // per neighbour instead of 32).  Per dimension q = round((x - lo) * scale)
 // modulo 2^32, with scale = 2^32 / prd in a periodic dimension, so that the
@@ -166,6 +236,10 @@ struct mdx_ctx {
   // overflow, [6] brick count scratch, [8..9] 64-bit neighbour total
   int *nbr = nullptr, *numneigh = nullptr, *dflags = nullptr;
   unsigned short *nbr16 = nullptr;  // brick mode: LDS slot numbers, [npad][cap]
+  // mask lists: one bit per stencil candidate of each atom, in the build's
+  // enumeration order, [npad][mwords] (mwords a multiple of 4)
+  unsigned *nmask = nullptr;
+  int mwords = 0;
   int list_mode = MDX_LIST_GATHER;
   int brick_req[3] = {8, 4, 4};
   mdx::BrickArgs br{};
@@ -239,6 +313,7 @@ int prof_resolve(mdx_ctx *c);
 int neigh_setup_grid(mdx_ctx *c);
 int neigh_build(mdx_ctx *c);
 int alloc_list(mdx_ctx *c, int cap);
+int alloc_mask(mdx_ctx *c, int words);
 int neigh_check_displacement(mdx_ctx *c, int *flag);
 
 // LDS-staged bricks (brick.hip)

@@ -345,6 +345,206 @@ __global__ __launch_bounds__(256) void k_build_lane(NeighArgs a, const double4 *
   atomicAdd(nsum, (unsigned long long)cnt);
 }
 
+// Mask list: the same candidate enumeration and keep test as k_build_lane,
+// one bit per candidate (self included, as a zero bit) in enumeration
+// order; words are stored four at a time.  dflags[2] gets the largest
+// number of words an atom needs (capacity regrowth).
+__global__ __launch_bounds__(256) void k_build_mask(NeighArgs a, const double4 *__restrict__ x,
+                                                   const unsigned *__restrict__ bkey,
+                                                   const int *__restrict__ bin_start, unsigned *__restrict__ mask,
+                                                   int mwords, int *__restrict__ numneigh, int *__restrict__ dflags,
+                                                   unsigned long long *__restrict__ nsum) {
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const double4 xi = x[i];
+  const int ti = static_cast<int>(xi.w);
+  const unsigned b = bkey[i];
+  const int ix = b % a.nbin[0], iy = (b / a.nbin[0]) % a.nbin[1], iz = b / (a.nbin[0] * a.nbin[1]);
+  unsigned *row = mask + (size_t)i * mwords;  // mwords is a multiple of 4: 16-B aligned rows
+  int ncand = 0, cnt = 0, nw = 0;
+  unsigned w = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+  const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
+  for (int r = 0; r < nrow; r++) {
+    RowSegs rs;
+    if (!row_segments(a, bin_start, ix, iy, iz, r, rs)) continue;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      const int j0 = rs.j0[s], j1 = rs.j1[s], code = rs.code[s];
+      if (j0 >= j1) continue;
+      const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+      const double shx = sx * a.prd[0], shy = sy * a.prd[1], shz = sz * a.prd[2];
+      for (int base = j0; base < j1; base += 4) {
+        double4 xj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) xj[u] = x[min(base + u, j1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = base + u;
+          if (j >= j1) continue;
+          bool keep = false;
+          if (!(code == MDX_NOSHIFT && j == i)) {
+            const int tj = static_cast<int>(xj[u].w);
+            const double g0 = xj[u].x + shx, g1 = xj[u].y + shy, g2 = xj[u].z + shz;
+            const double c = a.cutneighsq[ti * a.ntypes1 + tj];
+            keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
+            if (code != MDX_NOSHIFT) {
+              const bool keep_b = rsq_exact(xj[u].x, xj[u].y, xj[u].z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
+              if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+            }
+          }
+          w |= (keep ? 1u : 0u) << (ncand & 31);
+          cnt += keep ? 1 : 0;
+          ncand++;
+          if ((ncand & 31) == 0) {
+            const int slot = nw & 3;
+            q0 = slot == 0 ? w : q0;
+            q1 = slot == 1 ? w : q1;
+            q2 = slot == 2 ? w : q2;
+            q3 = slot == 3 ? w : q3;
+            nw++;
+            if (slot == 3 && nw <= mwords) *reinterpret_cast<uint4 *>(row + nw - 4) = make_uint4(q0, q1, q2, q3);
+            w = 0;
+          }
+        }
+      }
+    }
+  }
+  if (ncand & 31) {  // the last, partial word
+    const int slot = nw & 3;
+    q0 = slot == 0 ? w : q0;
+    q1 = slot == 1 ? w : q1;
+    q2 = slot == 2 ? w : q2;
+    q3 = slot == 3 ? w : q3;
+    nw++;
+    if (slot == 3 && nw <= mwords) *reinterpret_cast<uint4 *>(row + nw - 4) = make_uint4(q0, q1, q2, q3);
+  }
+  const int tail = nw & 3, t0 = nw - tail;
+  if (tail > 0 && t0 < mwords) row[t0] = q0;
+  if (tail > 1 && t0 + 1 < mwords) row[t0 + 1] = q1;
+  if (tail > 2 && t0 + 2 < mwords) row[t0 + 2] = q2;
+  numneigh[i] = cnt;
+  atomicMax(&dflags[0], cnt);
+  atomicMax(&dflags[2], nw);
+  atomicAdd(nsum, (unsigned long long)cnt);
+}
+
+// Bin-cooperative build: one wavefront per bin.  The atoms of a bin share one
+// stencil, so its candidates are enumerated once and each candidate position
+// is loaded once for all of them: lane r < nrow finds the segments of
+// stencil row r, a wavefront prefix sum flattens the rows, and pass k hands
+// candidate 64k + lane to each lane (consecutive lanes, consecutive atoms:
+// coalesced loads), the lane's row found by a binary search over the row
+// prefix held in the lanes.  Each atom of the bin then tests the 64
+// candidates; the ballot is 64 bits of its mask row (MASK) or, through a
+// popcount prefix, the slots of the kept entries in its index row.  The
+// candidate order, keep test and tie rule are those of k_build_lane, so rows
+// and masks are identical entry for entry.  nrow <= 64 (checked on the host).
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_build_bin(NeighArgs a, const double4 *__restrict__ x,
+                                                  const int *__restrict__ bin_start, int nbins,
+                                                  int *__restrict__ nbr, unsigned *__restrict__ mask, int cap,
+                                                  int *__restrict__ numneigh, int *__restrict__ dflags,
+                                                  unsigned long long *__restrict__ nsum) {
+  const int lane = threadIdx.x & 63;
+  const long wid = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6;
+  if (wid >= nbins) return;  // whole wavefronts leave together
+  const int b = (int)wid;
+  const int i0 = bin_start[b], i1 = bin_start[b + 1];
+  if (i0 == i1) return;
+  const int ix = b % a.nbin[0], iy = (b / a.nbin[0]) % a.nbin[1], iz = b / (a.nbin[0] * a.nbin[1]);
+  const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
+  RowSegs rs;
+  int cnt = 0;
+#pragma unroll
+  for (int s = 0; s < 3; s++) rs.j0[s] = rs.j1[s] = rs.code[s] = 0;
+  if (lane < nrow) cnt = row_segments(a, bin_start, ix, iy, iz, lane, rs);
+  int incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  const int total = __shfl(incl, 63, 64);
+  // row r's candidate p is j = p + dseg of its segment (segment bounds b1, b2)
+  const int excl = incl - cnt;
+  const int rb1 = excl + (rs.j1[0] - rs.j0[0]);
+  const int rb2 = rb1 + (rs.j1[1] - rs.j0[1]);
+  const int rd0 = rs.j0[0] - excl, rd1 = rs.j0[1] - rb1, rd2 = rs.j0[2] - rb2;
+  const int rcodes = rs.code[0] | (rs.code[1] << 5) | (rs.code[2] << 10);
+  const int npass = (total + 63) >> 6;
+  const int words = (total + 31) >> 5;  // mask words an atom of this bin needs
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int wmax = 0;
+  unsigned long long wsum = 0;
+  for (int ib = i0; ib < i1; ib += 64) {  // atoms of the bin, 64 at a time (normally one chunk)
+    const int mi = min(64, i1 - ib);
+    int mycnt = 0;  // lane k: kept entries of atom ib + k
+    for (int pass = 0; pass < npass; pass++) {
+      const int p = pass * 64 + lane;
+      const bool valid = p < total;
+      // row of p = number of rows whose inclusive end is <= p (incl is monotone)
+      int r = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(incl, r + step - 1, 64) <= p) r += step;
+      const int b1 = __shfl(rb1, r, 64), b2 = __shfl(rb2, r, 64);
+      const int d0 = __shfl(rd0, r, 64), d1 = __shfl(rd1, r, 64), d2 = __shfl(rd2, r, 64);
+      const int codes = __shfl(rcodes, r, 64);
+      const int sg = p < b1 ? 0 : (p < b2 ? 1 : 2);
+      const int j = valid ? p + (sg == 0 ? d0 : (sg == 1 ? d1 : d2)) : i0;
+      const int code = (codes >> (5 * sg)) & 31;
+      const double4 xj = x[j];
+      const int tj = static_cast<int>(xj.w);
+      const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+      const double shx = sx * a.prd[0], shy = sy * a.prd[1], shz = sz * a.prd[2];
+      const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
+      for (int k = 0; k < mi; k++) {
+        const int i = ib + k;
+        const double4 xi = x[i];
+        const int ti = static_cast<int>(xi.w);
+        bool keep = false;
+        if (valid && !(code == MDX_NOSHIFT && j == i)) {
+          const double c = a.cutneighsq[ti * a.ntypes1 + tj];
+          keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
+          if (code != MDX_NOSHIFT) {
+            // the reference evaluates an owned-ghost pair once, from one side
+            const bool keep_b = rsq_exact(xj.x, xj.y, xj.z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
+            if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+          }
+        }
+        const unsigned long long m = __ballot(keep);
+        const int pc = __popcll(m);
+        if (MASK) {
+          unsigned *row = mask + (size_t)i * cap;
+          if (lane == 0 && 2 * pass < cap) row[2 * pass] = (unsigned)m;
+          if (lane == 1 && 2 * pass + 1 < cap && 2 * pass + 1 < words) row[2 * pass + 1] = (unsigned)(m >> 32);
+        } else {
+          const int base = __builtin_amdgcn_readlane(mycnt, k);
+          const int pos = base + __popcll(m & below);
+          if (keep && pos < cap) nbr[(size_t)i * cap + pos] = j | (code << MDX_IDX_BITS);
+        }
+        if (lane == k) mycnt += pc;
+      }
+    }
+    if (lane < mi) {
+      numneigh[ib + lane] = mycnt;
+      wmax = max(wmax, mycnt);
+      wsum += (unsigned long long)mycnt;
+    }
+  }
+  // one set of atomics per wavefront
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    wmax = max(wmax, __shfl_down(wmax, off, 64));
+    wsum += __shfl_down(wsum, off, 64);
+  }
+  if (lane == 0) {
+    atomicMax(&dflags[0], wmax);
+    if (MASK) atomicMax(&dflags[2], words);
+    atomicAdd(nsum, wsum);
+  }
+}
+
 template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
@@ -433,6 +633,19 @@ int alloc_list(mdx_ctx *c, int cap) {
   return MDX_OK;
 }
 
+int alloc_mask(mdx_ctx *c, int words) {
+  words = (std::max(words, 4) + 3) / 4 * 4;  // 16-B aligned rows
+  (void)hipFree(c->nmask);
+  c->nmask = nullptr;
+  c->mwords = 0;
+  if (hipMalloc(&c->nmask, sizeof(unsigned) * (size_t)words * c->npad) != hipSuccess) {
+    set_error("Insufficient memory on accelerator (neighbour masks)");
+    return MDX_ERR_NOMEM;
+  }
+  c->mwords = words;
+  return MDX_OK;
+}
+
 static int end_bit_for(int nbins) {
   int bits = 1;
   while ((1L << bits) < nbins) bits++;
@@ -495,25 +708,54 @@ int neigh_build(mdx_ctx *c) {
   std::swap(c->orig, c->origb);
   std::swap(c->image, c->imageb);
   const bool brick = c->list_mode == MDX_LIST_BRICK;
+  const bool mask = c->list_mode == MDX_LIST_MASK;
+  const bool bin_build =
+      !brick && c->launch.build_tpa == 64 && (2 * a.S[1] + 1) * (2 * a.S[2] + 1) <= 64;
   if (brick) {
     int rc = brick_prepare(c);
     if (rc) return rc;
   }
-  if (!(brick ? (void *)c->nbr16 : (void *)c->nbr)) {
+  if (mask) {
+    if (!c->nmask) {
+      // first guess: the stencil's mean candidate count with 50% margin
+      const int nst = (2 * a.S[0] + 1) * (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
+      const double per_bin = (double)c->n / std::max(1, c->nbins);
+      int rc = alloc_mask(c, (int)(1.5 * nst * per_bin / 32.0) + 4);
+      if (rc) return rc;
+    }
+  } else if (!(brick ? (void *)c->nbr16 : (void *)c->nbr)) {
     int rc = alloc_list(c, c->maxneigh);
     if (rc) return rc;
   }
   // list build; regrow the per-atom capacity and rebuild on overflow
   for (int attempt = 0; attempt < 4; attempt++) {
     a.maxneigh = c->maxneigh;
-    if ((e = hipMemsetAsync(c->dflags, 0, sizeof(int), c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
+    if ((e = hipMemsetAsync(c->dflags, 0, 3 * sizeof(int), c->stream)) != hipSuccess)
+      return hip_fail(e, "memset flags");
     if ((e = hipMemsetAsync(c->dflags + 8, 0, 8, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
     if (brick) {
       int rc = brick_build_list(c);
       if (rc) return rc;
+    } else if (bin_build) {
+      // four wavefronts (bins) per block
+      const int B = 256, G = xcd_grid(((long)c->nbins + 3) / 4);
+      auto *nsum = reinterpret_cast<unsigned long long *>(c->dflags + 8);
+      if (mask)
+        hipLaunchKernelGGL(k_build_bin<true>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bin_start, c->nbins,
+                           nullptr, c->nmask, c->mwords, c->numneigh, c->dflags, nsum);
+      else
+        hipLaunchKernelGGL(k_build_bin<false>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bin_start, c->nbins,
+                           c->nbr, nullptr, c->maxneigh, c->numneigh, c->dflags, nsum);
+    } else if (mask) {
+      const int B = 256, G = xcd_grid(((long)c->n + B - 1) / B);
+      hipLaunchKernelGGL(k_build_mask, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start, c->nmask,
+                         c->mwords, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
     } else {
       switch (c->launch.build_tpa) {
-        case 1: launch_build<1>(c); break;
+        case 1:
+        case 64:  // stencil too tall for the bin-cooperative build
+          launch_build<1>(c);
+          break;
         case 4: launch_build<4>(c); break;
         case 8: launch_build<8>(c); break;
         case 16: launch_build<16>(c); break;
@@ -544,6 +786,12 @@ int neigh_build(mdx_ctx *c) {
     memcpy(&tot, &flags[8], sizeof(tot));
     c->total_neighbors = (long)tot;
     c->max_neighbors = mx;
+    if (mask) {
+      if (flags[2] <= c->mwords) break;
+      int rc = alloc_mask(c, flags[2] + flags[2] / 4 + 4);
+      if (rc) return rc;
+      continue;
+    }
     if (mx <= c->maxneigh) break;
     // regrow: the accelerator library's max_nbors resize (lal_neighbor.cpp:582-595)
     int cap = ((mx + mx / 4 + 15) / 16) * 16;
@@ -551,7 +799,7 @@ int neigh_build(mdx_ctx *c) {
     int rc = alloc_list(c, cap);
     if (rc) return rc;
   }
-  if (c->max_neighbors > c->maxneigh) {
+  if (!mask && c->max_neighbors > c->maxneigh) {
     set_error("neighbour list capacity could not be grown to fit");
     return MDX_ERR_OVERFLOW;
   }

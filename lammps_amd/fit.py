@@ -28,6 +28,12 @@ def ext():
     if _ext is None:
         import torch  # noqa: F401  (libtorch must be loaded first)
 
+        from lammps_amd import mdx
+
+        mdx.lib()
+        if mdx.LOADED_FLAVOUR != "torch":
+            raise RuntimeError("libmdx was loaded without torch in the process (MDX_HIP_RUNTIME=rocm), so it "
+                               "runs on ROCm's HIP runtime, not torch's; the torch extension needs one runtime")
         if not os.path.exists(EXT_PATH):
             raise RuntimeError("_mdx_torch.so is not built (run __graft_entry__.build())")
         spec = importlib.util.spec_from_file_location("_mdx_torch", EXT_PATH)

@@ -2,14 +2,33 @@
 
 Loads the in-tree lammps_amd/libmdx.so and fails loudly when it is missing:
 there is no CPU fallback for the device path.
+
+A process holds one HIP runtime.  PyTorch's ROCm wheel carries its own copy
+of the runtime under the soname libmdx.so needs (libamdhip64.so.7); two
+copies in one process do not both see the GPU.  So torch, when installed, is
+imported before the library is loaded: the loader then binds libmdx.so to
+torch's copy and device tensors and the library share one runtime.
+MDX_HIP_RUNTIME=rocm skips the import (ROCm's runtime; no torch in the
+process).
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdx.so")
+
+
+def hip_runtime_paths():
+    """Files of the HIP runtime(s) mapped into this process (Linux)."""
+    try:
+        with open("/proc/self/maps") as fh:
+            return sorted({ln.split()[-1] for ln in fh if "libamdhip64" in ln and "/" in ln})
+    except OSError:
+        return []
+
 
 FP64, MIXED = 0, 1
 POS_FP64, POS_Q32 = 0, 1
@@ -21,6 +40,7 @@ _p = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lib = None
+LOADED_FLAVOUR = None  # at the first load: 'torch' (torch's HIP runtime) or 'rocm'
 
 THERMO_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(ctypes.c_double))
 
@@ -43,9 +63,16 @@ class MdxError(RuntimeError):
 def lib():
     global _lib
     if _lib is None:
+        if os.environ.get("MDX_HIP_RUNTIME", "") != "rocm":
+            try:
+                import torch  # noqa: F401  (its HIP runtime becomes the process's)
+            except ImportError:
+                pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libmdx.so is not built (run __graft_entry__.build() or python -m lammps_amd.build)")
         L = ctypes.CDLL(LIB_PATH)
+        global LOADED_FLAVOUR
+        LOADED_FLAVOUR = "torch" if "torch" in sys.modules else "rocm"
         sig = {
             "mdx_version": [],
             "mdx_create": [_i, ctypes.POINTER(_p)],
@@ -227,8 +254,9 @@ class Context:
         return self.get_launch()
 
     def list_mode(self, mode, brick=(0, 0, 0)):
-        """mode 'gather' or 'brick' (LDS-staged bricks of brick[0]x[1]x[2] bins)."""
-        m = {"gather": 0, "brick": 1}[mode]
+        """mode 'gather' (32-bit index rows), 'brick' (LDS-staged bricks of
+        brick[0]x[1]x[2] bins) or 'mask' (one bit per stencil candidate)."""
+        m = {"gather": 0, "brick": 1, "mask": 2}[mode]
         self._ck(self._L.mdx_set_list_mode(self.h, m, int(brick[0]), int(brick[1]), int(brick[2])))
 
     def fused(self, on=True):

@@ -28,6 +28,30 @@ def test_library_exports_every_declared_symbol():
     assert set(mdx.exported_symbols()) <= names
 
 
+def test_one_hip_runtime_in_a_process_with_torch():
+    """torch's wheel carries a HIP runtime under libmdx.so's soname: loaded
+    after torch, the library binds to that copy (one runtime mapped)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from lammps_amd import mdx\n"
+            "mdx.lib()\n"
+            "import torch\n"
+            "print(mdx.LOADED_FLAVOUR, len(mdx.hip_runtime_paths()), mdx.hip_runtime_paths()[0])\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    flavour, n, path = out.stdout.split()
+    assert flavour == "torch" and n == "1" and "torch" in path, out.stdout
+    env = dict(os.environ, MDX_HIP_RUNTIME="rocm")
+    code2 = ("import sys; sys.path.insert(0, %r)\n"
+             "from lammps_amd import mdx\n"
+             "mdx.lib()\n"
+             "print(mdx.LOADED_FLAVOUR, 'torch' in sys.modules, len(mdx.hip_runtime_paths()))\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code2], capture_output=True, text=True, timeout=300, env=env)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["rocm", "False", "1"], out.stdout
+
+
 def test_version_and_error_string_without_gpu():
     from lammps_amd import mdx
     L = mdx.lib()

@@ -122,6 +122,7 @@ def test_lj_brick_layout_matches_gather_bitwise(lj_state, brick):
     out = {}
     for mode in ("gather", "brick"):
         ctx = device_from_oracle(r, lo, hi, "lj", mdx.MIXED, 0.3, 20, 0, False)
+        ctx.position_format(mdx.POS_FP64)  # bricks stage FP64 positions
         ctx.list_mode(mode, brick)
         out[mode] = ctx.compute(3, 5, peratom=True)
         if mode == "brick":
@@ -143,6 +144,7 @@ def test_eam_brick_layout_matches_gather_bitwise(eam_state, prec):
     out = {}
     for mode in ("gather", "brick"):
         ctx = device_from_oracle(r, lo, hi, "eam", p, 1.0, 1, 5, True)
+        ctx.position_format(mdx.POS_FP64)  # bricks stage FP64 positions
         ctx.list_mode(mode, (4, 2, 2))
         out[mode] = ctx.compute(3, 5, peratom=True)
         out[mode]["rho"] = ctx.eam_rho_fp()[0]
@@ -293,11 +295,12 @@ def test_lj_nonperiodic_box_forces(lj_state, prec):
     assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
 
 
-@pytest.mark.parametrize("build_tpa", [4, 8])
+@pytest.mark.parametrize("build_tpa", [4, 8, 64])
 def test_list_build_variants_give_identical_rows(lj_state, build_tpa):
-    """The one-lane build (four gathers in flight, 16-B row stores) and the
-    TPA-lane ballot build produce the same rows entry for entry: FP64 forces
-    from either list are bitwise equal."""
+    """The one-lane build (four gathers in flight, 16-B row stores), the
+    TPA-lane ballot build and the bin-cooperative build (64: one wavefront
+    per bin) produce the same rows entry for entry: FP64 forces from either
+    list are bitwise equal."""
     from lammps_amd import mdx
     r = lj_state
     lo, hi = box_of(r, 20, A_LJ)
@@ -312,3 +315,135 @@ def test_list_build_variants_give_identical_rows(lj_state, build_tpa):
     assert out[0][0] == out[1][0]
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 1), (0, 1, 1)])
+@pytest.mark.parametrize("layout", ["gather", "mask"])
+def test_bin_build_matches_lane_build_with_walls(lj_state, periodic, layout):
+    """Stencil rows cut by non-periodic walls and wrapped segments on the
+    periodic faces: the bin-cooperative build (64) gives the one-lane build's
+    pair set, and FP64 forces from either list are bitwise equal."""
+    from lammps_amd import mdx
+    x, _, _, t, _ = lj_state.state()
+    L = 20 * A_LJ
+    x = np.mod(x, L)
+    lo = tuple(0.0 if p else -1.0 for p in periodic)
+    hi = tuple(L if p else L + 1.0 for p in periodic)
+    out = []
+    for tpa in (1, 64):
+        ctx = mdx.Context(0)
+        ctx.set_box(lo, hi, periodic)
+        ctx.set_atoms(x, t, np.array([0.0, 1.0]), tags=np.arange(1, len(t) + 1))
+        ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+        ctx.precision(mdx.FP64)
+        ctx.neigh_modify(0.3)
+        ctx.list_mode(layout)
+        ctx.set_build_launch(tpa)
+        ctx.neigh_build()
+        pairs = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+        st = ctx.stats()
+        out.append((pairs, ctx.compute(1, 1)["f"], st["total_neighbors"], st["max_neighbors"]))
+        ctx.close()
+    assert len(out[0][0]) > 0
+    assert out[0][0] == out[1][0]
+    assert out[0][2:] == out[1][2:]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+# ---------------------------------------------------------------- mask lists
+@pytest.mark.parametrize("build_tpa", [1, 64])
+@pytest.mark.parametrize("style", ["lj", "eam"])
+def test_mask_list_pair_set_identical(lj_state, eam_state, style, build_tpa):
+    from lammps_amd import mdx
+    r = lj_state if style == "lj" else eam_state
+    if style == "lj":
+        lo, hi = box_of(r, 20, A_LJ)
+        ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+    else:
+        lo, hi = box_of(r, 20, 3.615)
+        ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
+    ctx.list_mode("mask")
+    ctx.set_build_launch(build_tpa)
+    ctx.neigh_build()
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
+    assert got == ref
+    assert ctx.stats()["total_neighbors"] == 2 * len(ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_fp64pos"])
+@pytest.mark.parametrize("style", ["lj", "eam"])
+def test_mask_list_forces(lj_state, eam_state, style, prec):
+    r = lj_state if style == "lj" else eam_state
+    if style == "lj":
+        lo, hi = box_of(r, 20, A_LJ)
+        ctx = device_from_oracle(r, lo, hi, "lj", precision_of(prec), 0.3, 20, 0, False)
+    else:
+        lo, hi = box_of(r, 20, 3.615)
+        ctx = device_from_oracle(r, lo, hi, "eam", precision_of(prec), 1.0, 1, 5, True)
+    precision_of(prec, ctx)
+    ctx.list_mode("mask")
+    out = ctx.compute(3, 5, peratom=True)
+    fref = r.state()[2]
+    eref, wref = r.energy_virial()
+    mx, rms = force_errors(out["f"], fref)
+    print("mask %s %s max_rel=%.3e rms_rel=%.3e" % (style, prec, mx, rms))
+    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
+    etol = 1e-11 if prec == "fp64" else 1e-6
+    assert abs(out["eng"] - eref) <= etol * abs(eref)
+    np.testing.assert_allclose(out["virial"], wref, rtol=etol, atol=etol * np.abs(wref).max())
+
+
+@pytest.mark.parametrize("tpa", [1, 2, 4, 16, 32])
+def test_mask_list_threads_per_atom_agree(lj_state, tpa):
+    from lammps_amd import mdx
+    r = lj_state
+    lo, hi = box_of(r, 20, A_LJ)
+    out = []
+    for t in (8, tpa):
+        ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+        ctx.list_mode("mask")
+        ctx.set_launch(0, t)
+        out.append(ctx.compute(1, 1)["f"])
+        ctx.close()
+    assert force_errors(out[1], out[0])[0] <= 1e-13
+
+
+@pytest.mark.parametrize("style", ["lj", "eam"])
+def test_mask_list_runs_match_reference_logs(style):
+    """bench/in.lj and bench/in.eam (FP64) with the mask layout: the log's
+    thermo digits, neighbours and builds; fused and unfused integration."""
+    from lammps_amd import mdx
+    for fused in (False, True):
+        if style == "lj":
+            g = GOLD["lj_bench_32k"]
+            r = M.Ref(M.LJ)
+            r.lattice_box(M.FCC, 0.8442, (0, 0, 0), (20, 20, 20))
+            r.mass(1, 1.0)
+            r.velocity_create(1.44, 87287, dist="uniform", loop="geom")
+            lo, hi = box_of(r, 20, A_LJ)
+            r.pair_lj(2.5)
+            r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+            ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
+            units, ftm2v, every = "lj", 1.0, 0
+        else:
+            g = GOLD["eam_bench_32k"]
+            r = M.Ref(M.METAL)
+            r.lattice_box(M.FCC, 3.615, (0, 0, 0), (20, 20, 20))
+            r.pair_eam(os.path.join(GOLDEN, "Cu_u3.eam"))
+            r.velocity_create(1600.0, 376847, dist="uniform", loop="geom")
+            r.neighbor(1.0, every=1, delay=5, check=True)
+            r.compute(1, 0)
+            lo, hi = box_of(r, 20, 3.615)
+            ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
+            units, ftm2v, every = "metal", 1.0 / 1.0364269e-4, 50
+        ctx.list_mode("mask")
+        ctx.fused(fused)
+        rows = ctx.setup() + ctx.run(100, 0.005, ftm2v, every)
+        prd = np.array(hi) - np.array(lo)
+        got = [[s] + mdx.thermo_columns(raw, 32000, prd, units) for s, raw in rows]
+        assert [fmt(x) for x in got] == g["thermo_text"], (style, fused)
+        st = ctx.stats()
+        assert st["total_neighbors"] == 2 * g["total_neighbors"]
+        assert st["ncalls"] == g["builds"]
+        ctx.close()
