@@ -43,6 +43,29 @@ constexpr int kUnroll = 4;  // neighbour entries in flight per lane (FP64 gather
 // the results
 constexpr int kUnrollQ = MDX_GATHER_UNROLL_Q;
 
+#ifndef MDX_PAIR_MIN_WAVES
+#define MDX_PAIR_MIN_WAVES 1
+#endif
+#define MDX_PAIR_BOUNDS __launch_bounds__(256, MDX_PAIR_MIN_WAVES)
+
+#ifndef MDX_LIST_CHUNK
+#define MDX_LIST_CHUNK 0
+#endif
+// lane's share of a row: entries lane, lane+TPA, ... (strided) or chunks of
+// four consecutive entries, 4*lane .. 4*lane+3, then 4*(lane+TPA) .. (one
+// 16-B index load per chunk: rows start 16-B aligned)
+constexpr bool kChunk = MDX_LIST_CHUNK != 0;
+
+#ifndef MDX_STREAM_LIST
+#define MDX_STREAM_LIST 0
+#endif
+// list entries are read once per launch: a streaming (non-temporal) load
+// keeps them from evicting the neighbour positions the gathers reuse from L2
+__device__ __forceinline__ int list_entry(const int *p) {
+  if (MDX_STREAM_LIST) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *__restrict__ out) {
   __shared__ double s[NV][16];
@@ -167,7 +190,7 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
 constexpr double kBand = 2e-6;
 
 template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-__global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
+__global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
                                            const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                            const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                            const LJTab *__restrict__ tab, double4 *__restrict__ f,
@@ -212,13 +235,22 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
     const int *nb = nbr + (size_t)i * cap;
     // UNROLL entries per lane per trip: all index loads, then all gathers,
     // are issued before the first use, so a lane has UNROLL gathers in flight
-    constexpr int U = Q ? kUnrollQ : kUnroll;
-    for (int k0 = lane; k0 < jn; k0 += U * TPA) {
+    constexpr int U = kChunk ? 4 : (Q ? kUnrollQ : kUnroll);
+    constexpr int KS = kChunk ? 1 : TPA;  // entry stride within a trip
+    const int self = i | (MDX_NOSHIFT << MDX_IDX_BITS);
+    for (int k0 = kChunk ? 4 * lane : lane; k0 < jn; k0 += U * TPA) {
       int jc[U];
       double4 xj[Q ? 1 : U];
       int4 qj[Q ? U : 1];
+      if (kChunk) {
+        const int4 e4 = *reinterpret_cast<const int4 *>(nb + k0);
+        const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
-      for (int u = 0; u < U; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
+        for (int u = 0; u < U; u++) jc[u] = k0 + u < jn ? ev[u] : self;
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) jc[u] = (k0 + u * KS < jn) ? list_entry(nb + k0 + u * KS) : self;
+      }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (Q) qj[u] = xq[jc[u] & MDX_IDX_MASK];
@@ -226,7 +258,7 @@ __global__ __launch_bounds__(256) void k_lj(int n, int npad, int cap, int nt1, d
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
+        if (k0 + u * KS >= jn) continue;  // entries carry image bits: test the index, not the sign
         double delx, dely, delz, rsq;
         int t;
         bool inside;
@@ -320,7 +352,7 @@ __device__ __forceinline__ bool pair_inside(const double4 &xi, const int4 &qi, c
 }
 
 template <bool MIXED, bool EV, int TPA, bool Q>
-__global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, double px, double py, double pz, EAMArgs a,
+__global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, double py, double pz, EAMArgs a,
                                                 const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                                 const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                                 double *__restrict__ rho_out, double *__restrict__ fp_out,
@@ -337,13 +369,22 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
-    constexpr int U = Q ? kUnrollQ : kUnroll;
-    for (int k0 = lane; k0 < jn; k0 += U * TPA) {
+    constexpr int U = kChunk ? 4 : (Q ? kUnrollQ : kUnroll);
+    constexpr int KS = kChunk ? 1 : TPA;  // entry stride within a trip
+    const int self = i | (MDX_NOSHIFT << MDX_IDX_BITS);
+    for (int k0 = kChunk ? 4 * lane : lane; k0 < jn; k0 += U * TPA) {
       int jc[U];
       double4 xj[Q ? 1 : U];
       int4 qj[Q ? U : 1];
+      if (kChunk) {
+        const int4 e4 = *reinterpret_cast<const int4 *>(nb + k0);
+        const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
-      for (int u = 0; u < U; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
+        for (int u = 0; u < U; u++) jc[u] = k0 + u < jn ? ev[u] : self;
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) jc[u] = (k0 + u * KS < jn) ? list_entry(nb + k0 + u * KS) : self;
+      }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (Q) qj[u] = xq[jc[u] & MDX_IDX_MASK];
@@ -351,7 +392,7 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
+        if (k0 + u * KS >= jn) continue;  // entries carry image bits: test the index, not the sign
         double delx, dely, delz, rsq;
         if (pair_inside<Q>(xi, qi, x, jc[u], xj[Q ? 0 : u], qj[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
                            delx, dely, delz, rsq)) {
@@ -394,7 +435,7 @@ __global__ __launch_bounds__(256) void k_eam_rho(int n, int npad, int cap, doubl
 }
 
 template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-__global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
+__global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
                                                   EAMArgs a, const double4 *__restrict__ x,
                                                   const int4 *__restrict__ xq, QScale qs,
                                                   const int *__restrict__ nbr, const int *__restrict__ numneigh,
@@ -420,14 +461,23 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
     const int jn = numneigh[i];
     const int *nb = nbr + (size_t)i * cap;
     const int stride = (a.nr + 1) * 7;
-    constexpr int U = Q ? kUnrollQ : kUnroll;
-    for (int k0 = lane; k0 < jn; k0 += U * TPA) {
+    constexpr int U = kChunk ? 4 : (Q ? kUnrollQ : kUnroll);
+    constexpr int KS = kChunk ? 1 : TPA;  // entry stride within a trip
+    const int self = i | (MDX_NOSHIFT << MDX_IDX_BITS);
+    for (int k0 = kChunk ? 4 * lane : lane; k0 < jn; k0 += U * TPA) {
       int jc[U];
       double4 xjv[Q ? 1 : U];
       int4 qjv[Q ? U : 1];
       double fpjv[U];
+      if (kChunk) {
+        const int4 e4 = *reinterpret_cast<const int4 *>(nb + k0);
+        const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
-      for (int u = 0; u < U; u++) jc[u] = (k0 + u * TPA < jn) ? nb[k0 + u * TPA] : (i | (MDX_NOSHIFT << MDX_IDX_BITS));
+        for (int u = 0; u < U; u++) jc[u] = k0 + u < jn ? ev[u] : self;
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) jc[u] = (k0 + u * KS < jn) ? list_entry(nb + k0 + u * KS) : self;
+      }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int jj = jc[u] & MDX_IDX_MASK;
@@ -437,7 +487,7 @@ __global__ __launch_bounds__(256) void k_eam_force(int n, int npad, int cap, dou
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-      if (k0 + u * TPA >= jn) continue;  // entries carry image bits: test the index, not the sign
+      if (k0 + u * KS >= jn) continue;  // entries carry image bits: test the index, not the sign
       double delx, dely, delz, rsq;
       if (pair_inside<Q>(xi, qi, x, jc[u], xjv[Q ? 0 : u], qjv[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
                          delx, dely, delz, rsq)) {
