@@ -468,7 +468,7 @@ def main():
     if not args.no_cpu_baseline:
         log("oracle force check + CPU baseline sample")
         errors, cpu, t_force = cpu_baseline_and_errors(
-            "lj", lj, {"mixed": res["mixed"]["ctx"], "fp64": res["fp64"]["ctx"]}, sample_steps=5)
+            "lj", lj, {"mixed": res["mixed"]["ctx"], "fp64": res["fp64"]["ctx"]}, sample_steps=40)
 
     drift = None
     if args.drift_steps:

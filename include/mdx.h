@@ -185,16 +185,17 @@ int mdx_copy_bandwidth(mdx_ctx *ctx, long nbytes, int iters, double *gbs);
    threads per atom (1, 2, 4, 8, 16 or 32; the `threads_per_atom` keyword of
    the reference's `package gpu`, src/GPU/fix_gpu.cpp:112-150). */
 int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
-/* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default), or
-   64: one wavefront per bin, the bin's candidates loaded once for all its
-   atoms (index and mask lists; stencils of up to 64 bin rows, otherwise
-   the one-lane build runs).  Every variant yields the same rows. */
+/* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default).
+   Every variant yields the same rows. */
 int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
 int mdx_get_launch(mdx_ctx *ctx, int *block, int *tpa, int *build_tpa);
 /* Autotuner (the role of `package gpu ... tpa` chosen by hand in the
    reference, src/GPU/fix_gpu.cpp:112-150): times the pair kernels on the
-   current list for threads per atom {4, 8, 16} x block {128, 256}, `reps`
-   launches each, keeps the fastest and recomputes the forces with it. */
+   current list for threads per atom {4, 8, 16} x block {128, 256}, and (index
+   lists) the list build for threads per atom {1, 4, 8}, `reps` launches
+   each; keeps the fastest and recomputes the list and forces with them.  The
+   extra builds are at unchanged positions (same rows, same order) but restart
+   the rebuild-trigger reference; call it right after setup. */
 int mdx_autotune(mdx_ctx *ctx, int reps);
 /* List layout (MDX_LIST_GATHER or MDX_LIST_BRICK) and, for bricks, the
    brick size in bins (0 = default 8x4x4; shrunk automatically until the

@@ -229,7 +229,7 @@ int mdx_destroy(mdx_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_atoms(c);
-  void *ptrs[] = {c->dflags, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->bin_count, c->bin_start,
+  void *ptrs[] = {c->dflags, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->eam_pk, c->eam_pk_f, c->bin_count, c->bin_start,
                   c->sort_tmp, c->scan_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -456,6 +456,25 @@ int mdx_pair_eam_init(mdx_ctx *c, int ntypes, int nrho, double drho, int nr, dou
   if ((e = hipMemcpy(c->eam_tab, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(c->eam_tab_f, allf.data(), sizeof(float) * allf.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "eam table upload");
+  // rhor and z2r repacked for the gather kernels: per knot {c0,c1,c2,0} then
+  // {c3,c4,c5,c6}, so a pair's derivative or value coefficients are one
+  // 16-B (float) or 32-B (double) load instead of three or four scalars
+  const size_t nk = (size_t)(nrhor + nz2r) * (nr + 1);
+  std::vector<double> pk(nk * 8, 0.0);
+  for (size_t q = 0; q < nk; q++) {
+    const double *src = all.data() + nf + q * 7;  // rhor then z2r, contiguous in `all`
+    double *dst = pk.data() + q * 8;
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+    for (int t = 3; t < 7; t++) dst[1 + t] = src[t];
+  }
+  std::vector<float> pkf(pk.size());
+  for (size_t k = 0; k < pk.size(); k++) pkf[k] = (float)pk[k];
+  if ((rc = dev_alloc(&c->eam_pk, pk.size())) || (rc = dev_alloc(&c->eam_pk_f, pkf.size()))) return rc;
+  if ((e = hipMemcpy(c->eam_pk, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(c->eam_pk_f, pkf.data(), sizeof(float) * pkf.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "eam table upload");
   EAMArgs &a = c->eam;
   memset(&a, 0, sizeof(a));
   a.nrho = nrho;
@@ -481,6 +500,10 @@ int mdx_pair_eam_init(mdx_ctx *c, int ntypes, int nrho, double drho, int nr, dou
   a.frho_f = c->eam_tab_f;
   a.rhor_f = c->eam_tab_f + nf;
   a.z2r_f = c->eam_tab_f + nf + nrr;
+  a.rhor_p = reinterpret_cast<const double4 *>(c->eam_pk);
+  a.z2r_p = reinterpret_cast<const double4 *>(c->eam_pk) + 2 * (size_t)nrhor * (nr + 1);
+  a.rhor_pf = reinterpret_cast<const float4 *>(c->eam_pk_f);
+  a.z2r_pf = reinterpret_cast<const float4 *>(c->eam_pk_f) + 2 * (size_t)nrhor * (nr + 1);
   // PairEAM::init_one: one global cutoff for every type pair (pair_eam.cpp:411-427)
   for (int q = 0; q < MDX_NT1 * MDX_NT1; q++) c->cutsq[q] = 0.0;
   for (int i = 1; i <= ntypes; i++)
@@ -881,16 +904,52 @@ int mdx_autotune(mdx_ctx *c, int reps) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  c->profile = prof;
-  if (rc) {
+  if (rc || e != hipSuccess) {
+    c->profile = prof;
     c->launch = keep;
-    return rc;
-  }
-  if (e != hipSuccess) {
-    c->launch = keep;
-    return hip_fail(e, "autotune");
+    return rc ? rc : hip_fail(e, "autotune");
   }
   c->launch = pick;
+  // threads per atom of the list build: rebuilding at unchanged positions
+  // gives the same rows in the same order (the bin sort is stable), so after
+  // setup the extra builds leave the state as it was
+  if (c->list_mode == MDX_LIST_GATHER) {
+    float bbest = 1e30f;
+    int bpick = c->launch.build_tpa;
+    for (int bt : {1, 4, 8}) {
+      c->launch.build_tpa = bt;
+      if ((rc = neigh_build(c))) break;  // warm-up (and capacity)
+      e = hipEventCreate(&e0);
+      if (e == hipSuccess && (e = hipEventCreate(&e1)) != hipSuccess) (void)hipEventDestroy(e0);
+      if (e != hipSuccess) {
+        c->profile = prof;
+        return hip_fail(e, "autotune events");
+      }
+      e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = neigh_build(c);
+      if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (rc || e != hipSuccess) break;
+      if (ms < bbest) {
+        bbest = ms;
+        bpick = bt;
+      }
+    }
+    if (rc || e != hipSuccess) {
+      c->profile = prof;
+      return rc ? rc : hip_fail(e, "autotune");
+    }
+    c->launch.build_tpa = bpick;
+    if ((rc = neigh_build(c))) {
+      c->profile = prof;
+      return rc;
+    }
+  }
+  c->profile = prof;
   return pair_compute(c, 0);
 }
 
@@ -928,8 +987,8 @@ int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
 int mdx_set_build_launch(mdx_ctx *c, int tpa) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (tpa != 0 && tpa != 1 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32 && tpa != 64) {
-    set_error("build threads per atom must be one of 1, 4, 8, 16, 32, or 64 (one wavefront per bin)");
+  if (tpa != 0 && tpa != 1 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32) {
+    set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
     return MDX_ERR_ARG;
   }
   c->launch.build_tpa = tpa ? tpa : 1;

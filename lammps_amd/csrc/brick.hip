@@ -32,6 +32,7 @@ namespace {
 constexpr int kNT2 = MDX_NT1 * MDX_NT1;
 constexpr int kMaxEntries = 4096;  // halo rows x window bins per brick
 constexpr int kUnroll = 4;         // list entries loaded ahead per lane
+static_assert(kUnroll == 4, "k_lj_brick walks chunks of four entries, as k_lj does");
 
 __device__ __forceinline__ double rsq_exact(double a0, double a1, double a2, double b0, double b1, double b2) {
 #pragma clang fp contract(off)
@@ -399,10 +400,11 @@ __global__ __launch_bounds__(256) void k_lj_brick(NeighArgs a, BrickArgs br, con
         const int row = static_cast<int>(xi.w) * nt1;
         const int jn = numneigh[gi];
         const unsigned short *nb = nbr + (size_t)gi * br.cap;
-        for (int k0 = lane; k0 < jn; k0 += kUnroll * TPA) {
+        // chunks of four consecutive entries per lane, k_lj's order
+        for (int k0 = 4 * lane; k0 < jn; k0 += kUnroll * TPA) {
           int sl[kUnroll];
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) sl[u] = (k0 + u * TPA < jn) ? (int)nb[k0 + u * TPA] : -1;
+          for (int u = 0; u < kUnroll; u++) sl[u] = (k0 + u < jn) ? (int)nb[k0 + u] : -1;
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
           if (sl[u] < 0) continue;
@@ -464,16 +466,6 @@ __global__ __launch_bounds__(256) void k_lj_brick(NeighArgs a, BrickArgs br, con
 }
 
 // ------------------------------------------------------------------ EAM
-template <typename T>
-__device__ __forceinline__ void spline_index(T r, T rdr, int nr, int &m, T &p) {
-  // pair_eam.cpp:188-192: p = r*rdr + 1; m = min(int(p), nr-1); p = min(p - m, 1)
-  p = r * rdr + T(1);
-  m = static_cast<int>(p);
-  m = min(max(m, 1), nr - 1);  // m >= 1 for any r >= 0; the max only guards NaN
-  p -= m;
-  p = p < T(1) ? p : T(1);
-}
-
 template <bool MIXED, bool EV, int TPA>
 __global__ __launch_bounds__(256) void k_eam_rho_brick(NeighArgs a, BrickArgs br, EAMArgs ea,
                                                       const double4 *__restrict__ x,
@@ -492,7 +484,6 @@ __global__ __launch_bounds__(256) void k_eam_rho_brick(NeighArgs a, BrickArgs br
   const int lane = threadIdx.x % TPA;
   const int group = threadIdx.x / TPA, ngroups = blockDim.x / TPA;
   const int niter = (st.ni + ngroups - 1) / ngroups;
-  const int stride = (ea.nr + 1) * 7;
   for (int it = 0; it < niter; it++) {
     const int v = group + it * ngroups;
     double rho = 0.0;
@@ -510,19 +501,7 @@ __global__ __launch_bounds__(256) void k_eam_rho_brick(NeighArgs a, BrickArgs br
         const double delx = xi.x - xj.x, dely = xi.y - xj.y, delz = xi.z - xj.z;
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq < ea.cutforcesq) {
-          const int tab = ea.type2rhor[static_cast<int>(xj.w) * ea.ntypes1 + ti];
-          int m;
-          if (MIXED) {
-            float p;
-            spline_index<float>(sqrtf((float)rsq), (float)ea.rdr, ea.nr, m, p);
-            const float *c = ea.rhor_f + (size_t)tab * stride + m * 7;
-            rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
-          } else {
-            double p;
-            spline_index<double>(sqrt(rsq), ea.rdr, ea.nr, m, p);
-            const double *c = ea.rhor + (size_t)tab * stride + m * 7;
-            rho += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
-          }
+          rho += eam_rho_term<MIXED>(ea, ea.type2rhor[static_cast<int>(xj.w) * ea.ntypes1 + ti], rsq);
         }
       }
     }
@@ -567,7 +546,6 @@ __global__ __launch_bounds__(256) void k_eam_force_brick(NeighArgs a, BrickArgs 
     const int lane = threadIdx.x % TPA;
     const int group = threadIdx.x / TPA, ngroups = blockDim.x / TPA;
     const int niter = (st.ni + ngroups - 1) / ngroups;
-    const int stride = (ea.nr + 1) * 7;
     for (int it = 0; it < niter; it++) {
       const int v = group + it * ngroups;
       double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
@@ -591,42 +569,8 @@ __global__ __launch_bounds__(256) void k_eam_force_brick(NeighArgs a, BrickArgs 
             const int tj = static_cast<int>(xj.w);
             const int tij = ti * ea.ntypes1 + tj, tji = tj * ea.ntypes1 + ti;
             double fpair, phi;
-            if (MIXED) {
-              const float r = sqrtf((float)rsq);
-              int m;
-              float p;
-              spline_index<float>(r, (float)ea.rdr, ea.nr, m, p);
-              const float *ci = ea.rhor_f + (size_t)ea.type2rhor[tij] * stride + m * 7;
-              const float *cj = ea.rhor_f + (size_t)ea.type2rhor[tji] * stride + m * 7;
-              const float *cz = ea.z2r_f + (size_t)ea.type2z2r[tij] * stride + m * 7;
-              const float rhoip = (ci[0] * p + ci[1]) * p + ci[2];
-              const float rhojp = (cj[0] * p + cj[1]) * p + cj[2];
-              const float z2p = (cz[0] * p + cz[1]) * p + cz[2];
-              const float z2 = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
-              const float recip = __builtin_amdgcn_rcpf(r);  // hardware reciprocal, 1 ulp
-              const float phif = z2 * recip;
-              const float phip = z2p * recip - phif * recip;
-              const float psip = (float)fpi * rhojp + (float)s.fp[sj] * rhoip + phip;
-              fpair = (double)(-psip * recip);
-              phi = (double)phif;
-            } else {
-              const double r = sqrt(rsq);
-              int m;
-              double p;
-              spline_index<double>(r, ea.rdr, ea.nr, m, p);
-              const double *ci = ea.rhor + (size_t)ea.type2rhor[tij] * stride + m * 7;
-              const double *cj = ea.rhor + (size_t)ea.type2rhor[tji] * stride + m * 7;
-              const double *cz = ea.z2r + (size_t)ea.type2z2r[tij] * stride + m * 7;
-              const double rhoip = (ci[0] * p + ci[1]) * p + ci[2];
-              const double rhojp = (cj[0] * p + cj[1]) * p + cj[2];
-              const double z2p = (cz[0] * p + cz[1]) * p + cz[2];
-              const double z2 = ((cz[3] * p + cz[4]) * p + cz[5]) * p + cz[6];
-              const double recip = 1.0 / r;
-              phi = z2 * recip;
-              const double phip = z2p * recip - phi * recip;
-              const double psip = fpi * rhojp + s.fp[sj] * rhoip + phip;
-              fpair = -psip * recip;
-            }
+            eam_pair_term<MIXED>(ea, ea.type2rhor[tij], ea.type2rhor[tji], ea.type2z2r[tij], rsq, fpi, s.fp[sj], fpair,
+                                 phi);
             fx += delx * fpair;
             fy += dely * fpair;
             fz += delz * fpair;

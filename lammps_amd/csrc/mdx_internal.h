@@ -77,7 +77,87 @@ struct EAMArgs {
   int type2z2r[MDX_NT1 * MDX_NT1];
   const double *frho, *rhor, *z2r;  // [k][n+1][7]
   const float *frho_f, *rhor_f, *z2r_f;
+  // rhor and z2r repacked per knot as two 4-vectors {c0,c1,c2,0},{c3,c4,c5,c6}
+  // (derivative, value): element 2*(k*(nr+1)+m) and the next one
+  const double4 *rhor_p, *z2r_p;
+  const float4 *rhor_pf, *z2r_pf;
 };
+
+// ---- eam spline evaluation, shared by every list layout (gather, mask,
+// brick) so that the same pair gives the same bits in each.  The spline
+// index arithmetic is PairEAM::compute's (pair_eam.cpp:188-192); tables are
+// the per-knot packed rhor/z2r of EAMArgs (MIXED: FP32 coefficients).
+template <typename T>
+__device__ __forceinline__ void eam_spline_index(T r, T rdr, int nr, int &m, T &p) {
+  p = r * rdr + T(1);
+  m = static_cast<int>(p);
+  m = m < 1 ? 1 : (m > nr - 1 ? nr - 1 : m);  // m >= 1 for any r >= 0; the lower clamp only guards NaN
+  p -= m;
+  p = p < T(1) ? p : T(1);
+}
+
+// rho contribution of a neighbour at rsq through table `tab` (pair_eam.cpp:193-196)
+template <bool MIXED>
+__device__ __forceinline__ double eam_rho_term(const EAMArgs &a, int tab, double rsq) {
+  int m;
+  if (MIXED) {
+    float p;
+    eam_spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
+    const float4 c = a.rhor_pf[2 * ((size_t)tab * (a.nr + 1) + m) + 1];  // c3..c6
+    return (double)(((c.x * p + c.y) * p + c.z) * p + c.w);
+  }
+  double p;
+  eam_spline_index<double>(sqrt(rsq), a.rdr, a.nr, m, p);
+  const double4 c = a.rhor_p[2 * ((size_t)tab * (a.nr + 1) + m) + 1];
+  return ((c.x * p + c.y) * p + c.z) * p + c.w;
+}
+
+// pair force factor fpair (f_i += del * fpair) and phi of the force pass
+// (pair_eam.cpp:245-282); ri = type2rhor[ti][tj], rj = type2rhor[tj][ti],
+// zi = type2z2r[ti][tj]
+template <bool MIXED>
+__device__ __forceinline__ void eam_pair_term(const EAMArgs &a, int ri, int rj, int zi, double rsq, double fpi,
+                                              double fpj, double &fpair, double &phi) {
+  const size_t kn = (size_t)(a.nr + 1);
+  if (MIXED) {
+    const float r = sqrtf((float)rsq);
+    int m;
+    float p;
+    eam_spline_index<float>(r, (float)a.rdr, a.nr, m, p);
+    const float4 ci = a.rhor_pf[2 * (ri * kn + m)];  // c0..c2
+    float4 cj = ci;  // (a conditional lvalue here would put ci in scratch memory)
+    if (ri != rj) cj = a.rhor_pf[2 * (rj * kn + m)];
+    const float4 zd = a.z2r_pf[2 * (zi * kn + m)], zv = a.z2r_pf[2 * (zi * kn + m) + 1];
+    const float rhoip = (ci.x * p + ci.y) * p + ci.z;
+    const float rhojp = (cj.x * p + cj.y) * p + cj.z;
+    const float z2p = (zd.x * p + zd.y) * p + zd.z;
+    const float z2 = ((zv.x * p + zv.y) * p + zv.z) * p + zv.w;
+    const float recip = __builtin_amdgcn_rcpf(r);  // hardware reciprocal, 1 ulp
+    const float phif = z2 * recip;
+    const float phip = z2p * recip - phif * recip;
+    const float psip = (float)fpi * rhojp + (float)fpj * rhoip + phip;
+    fpair = (double)(-psip * recip);
+    phi = (double)phif;
+    return;
+  }
+  const double r = sqrt(rsq);
+  int m;
+  double p;
+  eam_spline_index<double>(r, a.rdr, a.nr, m, p);
+  const double4 ci = a.rhor_p[2 * (ri * kn + m)];
+  double4 cj = ci;
+  if (ri != rj) cj = a.rhor_p[2 * (rj * kn + m)];
+  const double4 zd = a.z2r_p[2 * (zi * kn + m)], zv = a.z2r_p[2 * (zi * kn + m) + 1];
+  const double rhoip = (ci.x * p + ci.y) * p + ci.z;
+  const double rhojp = (cj.x * p + cj.y) * p + cj.z;
+  const double z2p = (zd.x * p + zd.y) * p + zd.z;
+  const double z2 = ((zv.x * p + zv.y) * p + zv.z) * p + zv.w;
+  const double recip = 1.0 / r;
+  phi = z2 * recip;
+  const double phip = z2p * recip - phi * recip;
+  const double psip = fpi * rhojp + fpj * rhoip + phip;
+  fpair = -psip * recip;
+}
 
 // Candidate segments of one stencil row of the atom in bin (ix, iy, iz).
 // Rows are numbered r = (dz + Sz)(2Sy + 1) + (dy + Sy), dz outer, the list
@@ -262,6 +342,8 @@ struct mdx_ctx {
   mdx::EAMArgs eam{};
   double *eam_tab = nullptr;  // device doubles: frho | rhor | z2r
   float *eam_tab_f = nullptr;
+  double *eam_pk = nullptr;  // packed rhor | z2r (EAMArgs::rhor_p, z2r_p)
+  float *eam_pk_f = nullptr;
   double *rho = nullptr, *fp = nullptr, *eemb = nullptr;
   // reductions
   double *partials = nullptr;

@@ -448,152 +448,6 @@ __global__ MDX_BUILD_BOUNDS void k_build_mask(NeighArgs a, const double4 *__rest
   atomicAdd(nsum, (unsigned long long)cnt);
 }
 
-// Bin-cooperative build.  The atoms of a bin share one stencil, so its
-// candidates are enumerated once and each candidate position is loaded once
-// for all of them.  A wavefront takes `bpw` consecutive bins in turn.  Per
-// bin: lane r < nrow finds the segments of stencil row r, a wavefront prefix
-// sum flattens the rows, and pass k hands candidate 64k + lane to each lane
-// (consecutive lanes, consecutive atoms: coalesced loads; the loads of four
-// passes are in flight together), the lane's row found by a binary search
-// over the row prefix held in the lanes.  Each atom of the bin, its position
-// broadcast from the lane that loaded it, then tests the 64 candidates of a
-// pass; the ballot is 64 bits of its mask row (MASK) or, through a popcount
-// prefix, the slots of its kept entries in its index row.  The candidate
-// order, keep test and tie rule are those of k_build_lane, so rows and masks
-// are identical entry for entry.  The list maximum and total leave with one
-// atomic per wavefront.  nrow <= 64 (checked on the host).
-__device__ __forceinline__ double bcast_lane(double v, int k) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
-  return __hiloint2double(hi, lo);
-}
-
-template <bool MASK>
-__global__ MDX_BUILD_BOUNDS void k_build_bin(NeighArgs a, const double4 *__restrict__ x,
-                                                  const int *__restrict__ bin_start, int nbins, int bpw,
-                                                  int *__restrict__ nbr, unsigned *__restrict__ mask, int cap,
-                                                  int *__restrict__ numneigh, int *__restrict__ dflags,
-                                                  unsigned long long *__restrict__ nsum) {
-  constexpr int PU = 4;  // passes whose candidate loads are in flight together
-  const int lane = threadIdx.x & 63;
-  const long wid = ((long)xcd_block() * blockDim.x + threadIdx.x) >> 6;
-  __shared__ double s_cut[MDX_NT1 * MDX_NT1];
-  stage_cutneighsq(a, s_cut);
-  const long bfirst = wid * bpw;
-  if (bfirst >= nbins) return;  // whole wavefronts leave together
-  const int blast = (int)min((long)nbins, bfirst + bpw);
-  const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
-  const unsigned long long below = (1ull << lane) - 1ull;
-  int wmax = 0, wwords = 0;
-  unsigned long long wsum = 0;
-  for (int b = (int)bfirst; b < blast; b++) {
-    const int i0 = bin_start[b], i1 = bin_start[b + 1];
-    if (i0 == i1) continue;
-    const int ix = b % a.nbin[0], iy = (b / a.nbin[0]) % a.nbin[1], iz = b / (a.nbin[0] * a.nbin[1]);
-    RowSegs rs;
-    int cnt = 0;
-#pragma unroll
-    for (int s = 0; s < 3; s++) rs.j0[s] = rs.j1[s] = rs.code[s] = 0;
-    if (lane < nrow) cnt = row_segments(a, bin_start, ix, iy, iz, lane, rs);
-    int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
-    const int total = __shfl(incl, 63, 64);
-    // row r's candidate p is j = p + d of its segment (segment bounds b1, b2)
-    const int excl = incl - cnt;
-    const int rb1 = excl + (rs.j1[0] - rs.j0[0]);
-    const int rb2 = rb1 + (rs.j1[1] - rs.j0[1]);
-    const int rd0 = rs.j0[0] - excl, rd1 = rs.j0[1] - rb1, rd2 = rs.j0[2] - rb2;
-    const int rcodes = rs.code[0] | (rs.code[1] << 5) | (rs.code[2] << 10);
-    const int npass = (total + 63) >> 6;
-    const int words = (total + 31) >> 5;  // mask words an atom of this bin needs
-    wwords = max(wwords, words);
-    for (int ib = i0; ib < i1; ib += 64) {  // atoms of the bin, 64 at a time (normally one chunk)
-      const int mi = min(64, i1 - ib);
-      const double4 xmine = x[lane < mi ? ib + lane : ib];  // lane k holds atom ib + k
-      int mycnt = 0;                                        // lane k: kept entries of atom ib + k
-      for (int p0 = 0; p0 < npass; p0 += PU) {
-        int jv[PU], cv[PU];
-        double4 xv[PU];
-#pragma unroll
-        for (int u = 0; u < PU; u++) {
-          const int p = (p0 + u) * 64 + lane;
-          // row of p = number of rows whose inclusive end is <= p (incl is monotone)
-          int r = 0;
-#pragma unroll
-          for (int step = 32; step > 0; step >>= 1)
-            if (__shfl(incl, r + step - 1, 64) <= p) r += step;
-          const int b1 = __shfl(rb1, r, 64), b2 = __shfl(rb2, r, 64);
-          const int d0 = __shfl(rd0, r, 64), d1 = __shfl(rd1, r, 64), d2 = __shfl(rd2, r, 64);
-          const int codes = __shfl(rcodes, r, 64);
-          const int sg = p < b1 ? 0 : (p < b2 ? 1 : 2);
-          const bool valid = p < total;
-          jv[u] = valid ? p + (sg == 0 ? d0 : (sg == 1 ? d1 : d2)) : -1;
-          cv[u] = (codes >> (5 * sg)) & 31;
-          xv[u] = x[valid ? jv[u] : i0];
-        }
-#pragma unroll
-        for (int u = 0; u < PU; u++) {
-          const int pass = p0 + u;
-          if (pass >= npass) break;
-          const int j = jv[u], code = cv[u];
-          const bool valid = j >= 0;
-          const double4 xj = xv[u];
-          const int tj = static_cast<int>(xj.w);
-          const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
-          const double shx = sx * a.prd[0], shy = sy * a.prd[1], shz = sz * a.prd[2];
-          const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
-          for (int k = 0; k < mi; k++) {
-            const int i = ib + k;
-            const double xi0 = bcast_lane(xmine.x, k), xi1 = bcast_lane(xmine.y, k), xi2 = bcast_lane(xmine.z, k);
-            const int ti = __builtin_amdgcn_readlane(static_cast<int>(xmine.w), k);
-            bool keep = false;
-            if (valid && !(code == MDX_NOSHIFT && j == i)) {
-              const double c = s_cut[ti * a.ntypes1 + tj];
-              keep = rsq_exact(xi0, xi1, xi2, g0, g1, g2) <= c;
-              if (code != MDX_NOSHIFT) {
-                // the reference evaluates an owned-ghost pair once, from one side
-                const bool keep_b = rsq_exact(xj.x, xj.y, xj.z, xi0 - shx, xi1 - shy, xi2 - shz) <= c;
-                if (keep_b != keep && !ref_owner_side(a.ref, xi0, xi1, xi2, g0, g1, g2)) keep = keep_b;
-              }
-            }
-            const unsigned long long m = __ballot(keep);
-            if (MASK) {
-              unsigned *row = mask + (size_t)i * cap;
-              if (lane == 0 && 2 * pass < cap) row[2 * pass] = (unsigned)m;
-              if (lane == 1 && 2 * pass + 1 < cap && 2 * pass + 1 < words) row[2 * pass + 1] = (unsigned)(m >> 32);
-            } else {
-              const int base = __builtin_amdgcn_readlane(mycnt, k);
-              const int pos = base + __popcll(m & below);
-              if (keep && pos < cap) nbr[(size_t)i * cap + pos] = j | (code << MDX_IDX_BITS);
-            }
-            if (lane == k) mycnt += __popcll(m);
-          }
-        }
-      }
-      if (lane < mi) {
-        numneigh[ib + lane] = mycnt;
-        wmax = max(wmax, mycnt);
-        wsum += (unsigned long long)mycnt;
-      }
-    }
-  }
-  // one set of atomics per wavefront
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    wmax = max(wmax, __shfl_down(wmax, off, 64));
-    wsum += __shfl_down(wsum, off, 64);
-  }
-  if (lane == 0) {
-    atomicMax(&dflags[0], wmax);
-    if (MASK) atomicMax(&dflags[2], wwords);
-    atomicAdd(nsum, wsum);
-  }
-}
-
 template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
@@ -657,6 +511,7 @@ int neigh_setup_grid(mdx_ctx *c) {
     c->scan_tmp_bytes = 0;
   }
   a.ntypes1 = c->ntypes + 1;
+
   return MDX_OK;
 }
 
@@ -758,8 +613,6 @@ int neigh_build(mdx_ctx *c) {
   std::swap(c->image, c->imageb);
   const bool brick = c->list_mode == MDX_LIST_BRICK;
   const bool mask = c->list_mode == MDX_LIST_MASK;
-  const bool bin_build =
-      !brick && c->launch.build_tpa == 64 && (2 * a.S[1] + 1) * (2 * a.S[2] + 1) <= 64;
   if (brick) {
     int rc = brick_prepare(c);
     if (rc) return rc;
@@ -785,29 +638,13 @@ int neigh_build(mdx_ctx *c) {
     if (brick) {
       int rc = brick_build_list(c);
       if (rc) return rc;
-    } else if (bin_build) {
-      // bins per wavefront: about 24k wavefronts (four rounds of a full chip
-      // at six wavefronts per SIMD), four wavefronts per block
-      const int bpw = std::max(1, (c->nbins + 24575) / 24576);
-      const long nw = ((long)c->nbins + bpw - 1) / bpw;
-      const int B = 256, G = xcd_grid((nw + 3) / 4);
-      auto *nsum = reinterpret_cast<unsigned long long *>(c->dflags + 8);
-      if (mask)
-        hipLaunchKernelGGL(k_build_bin<true>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bin_start, c->nbins,
-                           bpw, nullptr, c->nmask, c->mwords, c->numneigh, c->dflags, nsum);
-      else
-        hipLaunchKernelGGL(k_build_bin<false>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bin_start, c->nbins,
-                           bpw, c->nbr, nullptr, c->maxneigh, c->numneigh, c->dflags, nsum);
     } else if (mask) {
       const int B = 256, G = xcd_grid(((long)c->n + B - 1) / B);
       hipLaunchKernelGGL(k_build_mask, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start, c->nmask,
                          c->mwords, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
     } else {
       switch (c->launch.build_tpa) {
-        case 1:
-        case 64:  // stencil too tall for the bin-cooperative build
-          launch_build<1>(c);
-          break;
+        case 1: launch_build<1>(c); break;
         case 4: launch_build<4>(c); break;
         case 8: launch_build<8>(c); break;
         case 16: launch_build<16>(c); break;

@@ -1,0 +1,192 @@
+"""GPU edge cases of the C ABI: tiny, empty and ragged systems, closed forms,
+list capacity regrowth, and the errors the reference raises for them.
+
+Bars (stated here, in the test):
+  * closed forms (lj/cut dimer, pair_lj_cut.cpp:68-141): FP64 rel <= 1e-13,
+    mixed rel <= 1e-6;
+  * ragged random systems against the FP64 oracle: FP64 max_rel <= 1e-11,
+    mixed <= 1e-5, pair sets identical;
+  * error paths: the status code and the reference's message, no crash.
+"""
+import numpy as np
+import pytest
+
+from gpu_helpers import force_errors
+from oracle import mdref as M
+
+pytestmark = pytest.mark.gpu
+
+MASS = np.array([0.0, 1.0])
+
+
+def lj_context(lo, hi, periodic, x, prec, skin=0.3, one=2000, shift=False):
+    from lammps_amd import mdx
+    ctx = mdx.Context(0)
+    ctx.set_box(lo, hi, periodic)
+    ctx.set_atoms(x, np.ones(len(x), dtype=np.int32), MASS, tags=np.arange(1, len(x) + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)}, shift=shift)
+    ctx.precision(prec)
+    ctx.neigh_modify(skin, one=one)
+    return ctx
+
+
+def lj_dimer(r):
+    """force on atom 0 along +x (atom 1 at +r) and the pair energy, 12-6 LJ"""
+    f = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7)
+    return -f, 4.0 * (r ** -12 - r ** -6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("where", ["inside", "across_boundary", "closed_box"])
+def test_lj_dimer_matches_closed_form(prec, where):
+    from lammps_amd import mdx
+    r0 = 1.05
+    L = 10.0
+    if where == "inside":
+        x = np.array([[4.0, 5.0, 5.0], [4.0 + r0, 5.0, 5.0]])
+        per = (1, 1, 1)
+    elif where == "across_boundary":  # the pair only exists through the periodic image
+        x = np.array([[L - 0.4, 5.0, 5.0], [r0 - 0.4, 5.0, 5.0]])
+        per = (1, 1, 1)
+    else:
+        x = np.array([[4.0, 5.0, 5.0], [4.0 + r0, 5.0, 5.0]])
+        per = (0, 0, 0)
+    # the separation of these doubles as the reference forms it: the ghost
+    # image x_j + L, then x_i - ghost (atom_vec_atomic.cpp:290-297)
+    ghost = x[1, 0] + L if where == "across_boundary" else x[1, 0]
+    r = ghost - x[0, 0]
+    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
+    ctx = lj_context((0, 0, 0), (L, L, L), per, x, p)
+    out = ctx.compute(1, 1)
+    fx, e = lj_dimer(r)
+    tol = 1e-13 if prec == "fp64" else 1e-6
+    assert abs(out["f"][0, 0] - fx) <= tol * abs(fx)
+    assert abs(out["f"][1, 0] + fx) <= tol * abs(fx)
+    assert np.all(np.abs(out["f"][:, 1:]) <= tol * abs(fx))
+    assert abs(out["eng"] - e) <= tol * abs(e)
+    # virial xx of one pair: del_x * f_x (pair.cpp:769-855)
+    assert abs(out["virial"][0] - r * (-fx)) <= tol * abs(r * fx)
+    assert ctx.stats()["total_neighbors"] == 2
+    ctx.close()
+
+
+def test_single_atom_and_empty_system():
+    from lammps_amd import mdx
+    ctx = lj_context((0, 0, 0), (10, 10, 10), (1, 1, 1), np.array([[1.0, 2.0, 3.0]]), mdx.FP64)
+    out = ctx.compute(1, 1)
+    assert np.all(out["f"] == 0.0) and out["eng"] == 0.0 and np.all(out["virial"] == 0.0)
+    assert ctx.stats()["total_neighbors"] == 0
+    ctx.close()
+    ctx = lj_context((0, 0, 0), (10, 10, 10), (1, 1, 1), np.zeros((0, 3)), mdx.MIXED)
+    out = ctx.compute(1, 1)
+    assert out["f"].shape == (0, 3) and out["eng"] == 0.0 and np.all(out["virial"] == 0.0)
+    ctx.setup()
+    ctx.run(5, 0.005, 1.0, 5)
+    ctx.close()
+
+
+def random_gas(n, L, dmin, seed):
+    """n points in [0, L)^3 with minimum-image separation >= dmin"""
+    rng = np.random.default_rng(seed)
+    pts = np.zeros((0, 3))
+    while len(pts) < n:
+        p = rng.uniform(0.0, L, 3)
+        d = (pts - p + L / 2) % L - L / 2
+        if len(pts) == 0 or np.min(np.einsum("ij,ij->i", d, d)) >= dmin * dmin:
+            pts = np.vstack([pts, p])
+    return pts
+
+
+@pytest.mark.parametrize("n", [37, 300, 1025])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_ragged_random_systems_match_the_oracle(n, prec):
+    """atom counts that fill no block or wavefront evenly"""
+    from lammps_amd import mdx
+    L = 8.0 if n < 300 else 11.0
+    x = random_gas(n, L, 0.85, seed=n)
+    t = np.ones(n, dtype=np.int32)
+    ref = M.Ref(M.LJ)
+    ref.box((0, 0, 0), (L, L, L))
+    ref.add_atoms(x, t)
+    ref.mass(1, 1.0)
+    ref.pair_lj(2.5)
+    ref.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ref.neighbor(0.3)
+    ref.setup()
+    _, _, fref, _, _ = ref.state()
+    eref, _ = ref.energy_virial()
+    ctx = lj_context((0, 0, 0), (L, L, L), (1, 1, 1), x, mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    out = ctx.compute(1, 1)
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    assert got == {(min(a, b), max(a, b)) for a, b in ref.list_pairs()}
+    mx, _ = force_errors(out["f"], fref)
+    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
+    assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
+    ctx.close()
+
+
+def test_list_capacity_regrows_for_a_dense_cluster():
+    """A compact fcc droplet in a large periodic box: the first-guess row
+    capacity comes from the mean density (far below the droplet's ~78
+    neighbours), so the build must regrow it (the accelerator library's
+    max_nbors resize) and still give the oracle's forces."""
+    from lammps_amd import mdx
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*(np.arange(5),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = ((g[:, None] + basis[None]) * a).reshape(-1, 3) + 10.0
+    L = 40.0
+    ctx = lj_context((0, 0, 0), (L, L, L), (1, 1, 1), x, mdx.FP64)
+    out = ctx.compute(1, 1)
+    st = ctx.stats()
+    assert st["max_neighbors"] > 60 and st["capacity"] >= st["max_neighbors"]
+    ref = M.Ref(M.LJ)
+    ref.box((0, 0, 0), (L, L, L))
+    ref.add_atoms(x, np.ones(len(x), dtype=np.int32))
+    ref.mass(1, 1.0)
+    ref.pair_lj(2.5)
+    ref.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ref.neighbor(0.3)
+    ref.setup()
+    mx, _ = force_errors(out["f"], ref.state()[2])
+    assert mx <= 1e-11
+    ctx.close()
+
+
+def test_neighbor_overflow_is_reported():
+    """neigh_modify one below the per-atom count: the reference's error
+    (neigh_half_bin.cpp:414-415), as a status code, not a crash"""
+    from lammps_amd import mdx
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    _, _, x = mdx.lattice("fcc", 0.8442, "lj", (0, 0, 0), (6, 6, 6))
+    ctx = lj_context((0, 0, 0), (6 * a,) * 3, (1, 1, 1), x, mdx.MIXED, one=20)
+    with pytest.raises(mdx.MdxError) as e:
+        ctx.compute(1, 1)
+    assert e.value.code == mdx.ERR_OVERFLOW and "boost neigh_modify one" in str(e.value)
+    ctx.close()
+
+
+def test_lost_atom_in_a_closed_box_is_reported():
+    """boundary f: an atom outside the box is lost (the reference's
+    'Lost atoms' check, thermo.cpp lost/error), an error and no crash"""
+    from lammps_amd import mdx
+    x = random_gas(20, 6.0, 0.9, seed=3) + 2.0
+    ctx = lj_context((0, 0, 0), (10, 10, 10), (0, 0, 0), x, mdx.FP64)
+    ctx.compute(1, 1)
+    x2 = x.copy()
+    x2[7, 1] = 10.5
+    ctx.set_positions(x2)
+    with pytest.raises(mdx.MdxError) as e:
+        ctx.neigh_build()
+    assert e.value.code == mdx.ERR_ARG and "Lost atoms" in str(e.value)
+    ctx.close()
+
+
+def test_periodic_box_below_twice_the_cutoff_is_rejected():
+    from lammps_amd import mdx
+    x = random_gas(5, 5.0, 1.0, seed=4)
+    ctx = lj_context((0, 0, 0), (5.0, 5.0, 5.0), (1, 1, 1), x, mdx.FP64)
+    with pytest.raises(mdx.MdxError) as e:
+        ctx.compute(1, 1)
+    assert e.value.code == mdx.ERR_ARG and "2x the neighbour cutoff" in str(e.value)
+    ctx.close()

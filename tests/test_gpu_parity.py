@@ -295,12 +295,11 @@ def test_lj_nonperiodic_box_forces(lj_state, prec):
     assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
 
 
-@pytest.mark.parametrize("build_tpa", [4, 8, 64])
+@pytest.mark.parametrize("build_tpa", [4, 8])
 def test_list_build_variants_give_identical_rows(lj_state, build_tpa):
-    """The one-lane build (four gathers in flight, 16-B row stores), the
-    TPA-lane ballot build and the bin-cooperative build (64: one wavefront
-    per bin) produce the same rows entry for entry: FP64 forces from either
-    list are bitwise equal."""
+    """The one-lane build (four gathers in flight, 16-B row stores) and the
+    TPA-lane ballot build produce the same rows entry for entry: FP64 forces
+    from either list are bitwise equal."""
     from lammps_amd import mdx
     r = lj_state
     lo, hi = box_of(r, 20, A_LJ)
@@ -319,10 +318,10 @@ def test_list_build_variants_give_identical_rows(lj_state, build_tpa):
 
 @pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 1), (0, 1, 1)])
 @pytest.mark.parametrize("layout", ["gather", "mask"])
-def test_bin_build_matches_lane_build_with_walls(lj_state, periodic, layout):
+def test_build_variants_agree_with_walls(lj_state, periodic, layout):
     """Stencil rows cut by non-periodic walls and wrapped segments on the
-    periodic faces: the bin-cooperative build (64) gives the one-lane build's
-    pair set, and FP64 forces from either list are bitwise equal."""
+    periodic faces: the one-lane build and the 8-lane ballot build give the
+    same pair set, and FP64 forces from either list are bitwise equal."""
     from lammps_amd import mdx
     x, _, _, t, _ = lj_state.state()
     L = 20 * A_LJ
@@ -330,7 +329,7 @@ def test_bin_build_matches_lane_build_with_walls(lj_state, periodic, layout):
     lo = tuple(0.0 if p else -1.0 for p in periodic)
     hi = tuple(L if p else L + 1.0 for p in periodic)
     out = []
-    for tpa in (1, 64):
+    for tpa in (1, 8):
         ctx = mdx.Context(0)
         ctx.set_box(lo, hi, periodic)
         ctx.set_atoms(x, t, np.array([0.0, 1.0]), tags=np.arange(1, len(t) + 1))
@@ -351,9 +350,8 @@ def test_bin_build_matches_lane_build_with_walls(lj_state, periodic, layout):
 
 
 # ---------------------------------------------------------------- mask lists
-@pytest.mark.parametrize("build_tpa", [1, 64])
 @pytest.mark.parametrize("style", ["lj", "eam"])
-def test_mask_list_pair_set_identical(lj_state, eam_state, style, build_tpa):
+def test_mask_list_pair_set_identical(lj_state, eam_state, style):
     from lammps_amd import mdx
     r = lj_state if style == "lj" else eam_state
     if style == "lj":
@@ -363,7 +361,6 @@ def test_mask_list_pair_set_identical(lj_state, eam_state, style, build_tpa):
         lo, hi = box_of(r, 20, 3.615)
         ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
     ctx.list_mode("mask")
-    ctx.set_build_launch(build_tpa)
     ctx.neigh_build()
     got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
     ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
