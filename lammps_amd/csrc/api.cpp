@@ -608,7 +608,8 @@ int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
     const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
     for (int i = 0; i < n; i++) {
       const unsigned key = bk[i];
-      const int ix = key % a.nbin[0], iy = (key / a.nbin[0]) % a.nbin[1], iz = key / (a.nbin[0] * a.nbin[1]);
+      int ix, iy, iz;
+      bin_coords(a, (int)key, ix, iy, iz);
       long bit = 0;
       int k = 0;
       for (int r = 0; r < nrow; r++) {

@@ -126,7 +126,7 @@ __device__ __host__ inline bool brick_entry(const NeighArgs &a, const BrickArgs 
       s[d] = 1;
     }
   }
-  bin = (c[2] * a.nbin[1] + c[1]) * a.nbin[0] + c[0];
+  bin = bin_index(a, c[0], c[1], c[2]);
   sx = s[0];
   sy = s[1];
   sz = s[2];
@@ -756,7 +756,8 @@ void brick_slot_map(const NeighArgs &a, const BrickArgs &br, int brick, const in
 }
 
 int brick_of_bin(const BrickArgs &br, const NeighArgs &a, int bin) {
-  const int ix = bin % a.nbin[0], iy = (bin / a.nbin[0]) % a.nbin[1], iz = bin / (a.nbin[0] * a.nbin[1]);
+  int ix, iy, iz;
+  bin_coords(a, bin, ix, iy, iz);
   return ((iz / br.R[2]) * br.nbrick[1] + iy / br.R[1]) * br.nbrick[0] + ix / br.R[0];
 }
 

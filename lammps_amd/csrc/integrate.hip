@@ -82,22 +82,31 @@ __global__ __launch_bounds__(256) void k_ke(int n, MassTab mt, const double4 *__
   }
 }
 
-// fixed-order final reduction of per-block partials [nblocks][nslots]
+// fixed-order final reduction of per-block partials [nblocks][nslots],
+// nslots <= MDX_RED_SLOTS: one pass over the records (a thread reads whole
+// records b = t, t + 256, ...), then a fixed tree per slot
 __global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__ partials, int nblocks, int nslots,
                                                      double *__restrict__ out) {
-  __shared__ double s[256];
-  for (int slot = 0; slot < nslots; slot++) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) acc += partials[(size_t)b * nslots + slot];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[slot] = s[0];
+  __shared__ double s[MDX_RED_SLOTS][256];
+  double acc[MDX_RED_SLOTS];
+#pragma unroll
+  for (int k = 0; k < MDX_RED_SLOTS; k++) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) {
+    const double *rec = partials + (size_t)b * nslots;
+#pragma unroll
+    for (int k = 0; k < MDX_RED_SLOTS; k++)
+      if (k < nslots) acc[k] += rec[k];
+  }
+#pragma unroll
+  for (int k = 0; k < MDX_RED_SLOTS; k++) s[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < MDX_RED_SLOTS; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
     __syncthreads();
   }
+  if ((int)threadIdx.x < nslots) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 // caller-order host positions -> device (sorted) order, type kept in w

@@ -170,6 +170,36 @@ struct RowSegs {
 __host__ __device__ inline int imax2(int a, int b) { return a > b ? a : b; }
 __host__ __device__ inline int imin2(int a, int b) { return a < b ? a : b; }
 
+// Bin linearisation.  x-rows of bins stay contiguous (x fastest), so a
+// stencil row is at most three contiguous atom ranges; the rows themselves
+// are ordered in y-tiles of kBinTileY rows, z-major inside a tile:
+// row(y, z) = y0*nbz + z*th + (y - y0), y0 = the tile's first row, th its
+// height.  Kernels walking atoms in storage order then sweep a tile's
+// z-planes, so the neighbour positions a wavefront window needs (within two
+// rows and planes) stay resident in L2 instead of being re-fetched for every
+// plane of a z-major order.
+#ifndef MDX_BIN_TILE_Y
+#define MDX_BIN_TILE_Y 8
+#endif
+constexpr int kBinTileY = MDX_BIN_TILE_Y;
+__host__ __device__ inline int bin_row(const NeighArgs &a, int y, int z) {
+  const int y0 = (y / kBinTileY) * kBinTileY;
+  const int th = imin2(kBinTileY, a.nbin[1] - y0);
+  return y0 * a.nbin[2] + z * th + (y - y0);
+}
+__host__ __device__ inline int bin_index(const NeighArgs &a, int x, int y, int z) {
+  return bin_row(a, y, z) * a.nbin[0] + x;
+}
+__host__ __device__ inline void bin_coords(const NeighArgs &a, int b, int &x, int &y, int &z) {
+  x = b % a.nbin[0];
+  const int r = b / a.nbin[0];
+  const int y0 = (r / (kBinTileY * a.nbin[2])) * kBinTileY;
+  const int th = imin2(kBinTileY, a.nbin[1] - y0);
+  const int rem = r - y0 * a.nbin[2];
+  z = rem / th;
+  y = y0 + rem % th;
+}
+
 __host__ __device__ inline int row_segments(const NeighArgs &a, const int *__restrict__ bin_start, int ix, int iy,
                                             int iz, int r, RowSegs &rs) {
   const int wy = 2 * a.S[1] + 1;
@@ -196,7 +226,7 @@ __host__ __device__ inline int row_segments(const NeighArgs &a, const int *__res
     jy -= nby;
     sy = 1;
   }
-  const int rowbase = (jz * nby + jy) * nbx;
+  const int rowbase = bin_row(a, jy, jz) * nbx;
   const int x0 = ix - a.S[0], x1 = ix + a.S[0];
   int total = 0;
 #pragma unroll

@@ -121,7 +121,7 @@ __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict
   p.z = c[2];
   x[i] = p;
   image[i] = (im[2] << 20) | (im[1] << 10) | im[0];
-  unsigned b = (unsigned)((ib[2] * a.nbin[1] + ib[1]) * a.nbin[0] + ib[0]);
+  unsigned b = (unsigned)bin_index(a, ib[0], ib[1], ib[2]);
   key[i] = b;
   idx[i] = i;
   atomicAdd(&count[b], 1);
@@ -168,7 +168,8 @@ __global__ MDX_BUILD_BOUNDS void k_build_full(NeighArgs a, const double4 *__rest
   const int ti = static_cast<int>(xi.w);
   const unsigned b = bkey[i];
   const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
-  const int ix = b % nbx, iy = (b / nbx) % nby, iz = b / (nbx * nby);
+  int ix, iy, iz;
+  bin_coords(a, (int)b, ix, iy, iz);
   int *row = nbr + (size_t)i * a.maxneigh;
   int cnt = 0;
   for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
@@ -193,7 +194,7 @@ __global__ MDX_BUILD_BOUNDS void k_build_full(NeighArgs a, const double4 *__rest
         jy -= nby;
         sy = 1;
       }
-      const int rowbase = (jz * nby + jy) * nbx;
+      const int rowbase = bin_row(a, jy, jz) * nbx;
       const int x0 = ix - a.S[0], x1 = ix + a.S[0];
       const double shy = sy * a.prd[1], shz = sz * a.prd[2];
       // segments in stencil order: wrapped low part, main part, wrapped high part
@@ -271,7 +272,8 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
   const int ti = static_cast<int>(xi.w);
   const unsigned b = bkey[i];
   const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
-  const int ix = b % nbx, iy = (b / nbx) % nby, iz = b / (nbx * nby);
+  int ix, iy, iz;
+  bin_coords(a, (int)b, ix, iy, iz);
   int *row = nbr + (size_t)i * a.maxneigh;  // maxneigh is a multiple of 4: rows are 16-B aligned
   int cnt = 0;
   int q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // pending entries cnt&~3 .. cnt-1
@@ -297,7 +299,7 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
         jy -= nby;
         sy = 1;
       }
-      const int rowbase = (jz * nby + jy) * nbx;
+      const int rowbase = bin_row(a, jy, jz) * nbx;
       const int x0 = ix - a.S[0], x1 = ix + a.S[0];
       const double shy = sy * a.prd[1], shz = sz * a.prd[2];
       for (int s = 0; s < 3; s++) {
@@ -379,7 +381,8 @@ __global__ MDX_BUILD_BOUNDS void k_build_mask(NeighArgs a, const double4 *__rest
   const double4 xi = x[i];
   const int ti = static_cast<int>(xi.w);
   const unsigned b = bkey[i];
-  const int ix = b % a.nbin[0], iy = (b / a.nbin[0]) % a.nbin[1], iz = b / (a.nbin[0] * a.nbin[1]);
+  int ix, iy, iz;
+  bin_coords(a, (int)b, ix, iy, iz);
   unsigned *row = mask + (size_t)i * mwords;  // mwords is a multiple of 4: 16-B aligned rows
   int ncand = 0, cnt = 0, nw = 0;
   unsigned w = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;

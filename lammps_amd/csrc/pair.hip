@@ -189,7 +189,9 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
 // decision is the FP64 one (the quantised rsq is within ~1e-7 relative).
 constexpr double kBand = 2e-6;
 
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
+// ONE: a single atom type, so every pair uses the coefficients of type pair
+// (1,1): the table reads are loop-invariant and leave the neighbour loop.
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
                                            const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                            const int *__restrict__ nbr, const int *__restrict__ numneigh,
@@ -268,7 +270,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
           dely = qdelta(qi.y, qj[u].y, qs.step[1]);
           delz = qdelta(qi.z, qj[u].z, qs.step[2]);
           rsq = delx * delx + dely * dely + delz * delz;
-          t = row + qj[u].w;
+          t = ONE ? nt1 + 1 : row + qj[u].w;
           inside = rsq < s_clo[t];
           if (!inside && rsq < s_chi[t]) {
             double ex, ey, ez;
@@ -278,7 +280,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
         } else {
           pair_delta(xi, xj[Q ? 0 : u], (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz);
           rsq = delx * delx + dely * dely + delz * delz;
-          t = row + static_cast<int>(xj[Q ? 0 : u].w);
+          t = ONE ? nt1 + 1 : row + static_cast<int>(xj[Q ? 0 : u].w);
           inside = rsq < s_cutsq[t];
         }
         if (inside) {
@@ -342,7 +344,7 @@ __device__ __forceinline__ bool pair_inside(const double4 &xi, const int4 &qi, c
   return rsq < cutsq;
 }
 
-template <bool MIXED, bool EV, int TPA, bool Q>
+template <bool MIXED, bool EV, int TPA, bool Q, bool ONE>
 __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, double py, double pz, EAMArgs a,
                                                 const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                                 const int *__restrict__ nbr, const int *__restrict__ numneigh,
@@ -392,7 +394,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
         if (pair_inside<Q>(xi, qi, x, jc[u], xj[Q ? 0 : u], qj[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
                            delx, dely, delz, rsq)) {
           const int tj = Q ? qj[Q ? u : 0].w : static_cast<int>(xj[Q ? 0 : u].w);
-          rho += eam_rho_term<MIXED>(a, s_t2r[tj * a.ntypes1 + ti], rsq);
+          rho += eam_rho_term<MIXED>(a, s_t2r[ONE ? a.ntypes1 + 1 : tj * a.ntypes1 + ti], rsq);
         }
       }
     }
@@ -417,7 +419,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
   }
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
                                                   EAMArgs a, const double4 *__restrict__ x,
                                                   const int4 *__restrict__ xq, QScale qs,
@@ -482,7 +484,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
       if (pair_inside<Q>(xi, qi, x, jc[u], xjv[Q ? 0 : u], qjv[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
                          delx, dely, delz, rsq)) {
         const int tj = Q ? qjv[Q ? u : 0].w : static_cast<int>(xjv[Q ? 0 : u].w);
-        const int tij = ti * a.ntypes1 + tj, tji = tj * a.ntypes1 + ti;
+        const int tij = ONE ? a.ntypes1 + 1 : ti * a.ntypes1 + tj, tji = ONE ? tij : tj * a.ntypes1 + ti;
         double fpair, phi;
         eam_pair_term<MIXED>(a, s_t2r[tij], s_t2r[tji], s_t2z[tij], rsq, fpi, fpjv[u], fpair, phi);
         fx += delx * fpair;
@@ -533,7 +535,8 @@ __device__ __forceinline__ int group_excl_scan(int v, int lane, int &total) {
 template <int TPA, typename Body>
 __device__ __forceinline__ void mask_neighbors(const NeighArgs &na, const int *__restrict__ bin_start, unsigned key,
                                                const unsigned *__restrict__ mrow, int lane, int self, Body &&body) {
-  const int ix = key % na.nbin[0], iy = (key / na.nbin[0]) % na.nbin[1], iz = key / (na.nbin[0] * na.nbin[1]);
+  int ix, iy, iz;
+  bin_coords(na, (int)key, ix, iy, iz);
   const int nrow = (2 * na.S[1] + 1) * (2 * na.S[2] + 1);
   int base = 0;
   for (int k = 0; k < nrow; k += TPA) {  // the same trip count on every lane of the group
@@ -856,31 +859,47 @@ int dispatch_mask(mdx_ctx *c, int B, const IntegArgs &ig) {
   return MDX_OK;
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-void launch_lj(mdx_ctx *c, int B, const IntegArgs &ig) {
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
+void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
                      c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->xq, c->qs, c->nbr, c->numneigh,
                      c->lj_dev, c->f, c->vatom, c->partials, ig);
   prof_end(c);
   c->last_grid = G;
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-void launch_eam(mdx_ctx *c, int B, const IntegArgs &ig) {
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
+void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, MDX_K_EAM_RHO);
-  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
                      c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->rho, c->fp,
                      c->eemb);
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
-  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
+  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
                      c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh,
                      c->fp, c->eemb, c->f, c->vatom, c->partials, ig);
   prof_end(c);
   c->last_grid = G;
+}
+
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
+void launch_lj(mdx_ctx *c, int B, const IntegArgs &ig) {
+  if constexpr (TPA == 4 || TPA == 8 || TPA == 16) {
+    if (c->ntypes == 1) return launch_lj_t<MIXED, EV, TPA, INTEG, Q, true>(c, B, ig);
+  }
+  launch_lj_t<MIXED, EV, TPA, INTEG, Q, false>(c, B, ig);
+}
+
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
+void launch_eam(mdx_ctx *c, int B, const IntegArgs &ig) {
+  if constexpr (TPA == 4 || TPA == 8 || TPA == 16) {
+    if (c->ntypes == 1) return launch_eam_t<MIXED, EV, TPA, INTEG, Q, true>(c, B, ig);
+  }
+  launch_eam_t<MIXED, EV, TPA, INTEG, Q, false>(c, B, ig);
 }
 
 template <bool MIXED, bool EV, bool INTEG, bool Q>

@@ -190,3 +190,40 @@ def test_periodic_box_below_twice_the_cutoff_is_rejected():
         ctx.compute(1, 1)
     assert e.value.code == mdx.ERR_ARG and "2x the neighbour cutoff" in str(e.value)
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("tpa", [4, 32])
+def test_two_type_lj_matches_the_oracle(prec, tpa):
+    """Kob-Andersen coefficients (two types, per-pair cutoffs): the general
+    pair kernels (per-pair type lookups; single-type systems take a
+    specialised path) against the oracle"""
+    from lammps_amd import mdx
+    L, n = 9.0, 600
+    x = random_gas(n, L, 0.85, seed=11)
+    t = np.where(np.random.default_rng(12).random(n) < 0.2, 2, 1).astype(np.int32)
+    coeffs = {(1, 1): (1.0, 1.0, 2.5), (1, 2): (1.5, 0.8, 2.0), (2, 2): (0.5, 0.88, 2.2)}
+    ref = M.Ref(M.LJ)
+    ref.box((0, 0, 0), (L, L, L), ntypes=2)
+    ref.add_atoms(x, t)
+    ref.mass(1, 1.0)
+    ref.mass(2, 1.0)
+    ref.pair_lj(2.5)
+    for (i, j), (e, s, c) in coeffs.items():
+        ref.pair_coeff(i, j, e, s, c)
+    ref.neighbor(0.3)
+    ref.setup()
+    _, _, fref, _, _ = ref.state()
+    eref, _ = ref.energy_virial()
+    ctx = mdx.Context(0)
+    ctx.set_box((0, 0, 0), (L, L, L), (1, 1, 1))
+    ctx.set_atoms(x, t, np.array([0.0, 1.0, 1.0]), tags=np.arange(1, n + 1))
+    ctx.pair_lj_from_coeffs(2, coeffs)
+    ctx.precision(mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    ctx.neigh_modify(0.3)
+    ctx.set_launch(0, tpa)
+    out = ctx.compute(1, 1)
+    mx, _ = force_errors(out["f"], fref)
+    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
+    assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
+    ctx.close()
