@@ -330,13 +330,18 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
           for (int u = 0; u < 4; u++) xj[u] = x[min(base + u, j1 - 1)];  // in range; unused tail ignored
 #pragma unroll
           for (int u = 0; u < 4; u++) {
+            // no early exit: every loaded candidate is used, so the four
+            // loads stay in flight together
             const int j = base + u;
-            if (j >= j1 || (code == MDX_NOSHIFT && j == i)) continue;
+            const bool valid = j < j1 && !(code == MDX_NOSHIFT && j == i);
             const int tj = static_cast<int>(xj[u].w);
             const double g0 = xj[u].x + shx, g1 = xj[u].y + shy, g2 = xj[u].z + shz;
             const double c = s_cut[ti * a.ntypes1 + tj];
-            bool keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
-            if (code != MDX_NOSHIFT) {
+            const double r = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2);
+            bool keep = valid && r <= c;
+            // the mirrored rsq of an image pair differs from r by a few ulps,
+            // so the reference's side can only matter within a hair of c
+            if (valid && code != MDX_NOSHIFT && fabs(r - c) <= 1e-10 * c) {
               const bool keep_b = rsq_exact(xj[u].x, xj[u].y, xj[u].z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
               if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
             }
