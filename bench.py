@@ -450,9 +450,12 @@ def main():
     bytes_per_launch = n * (4.0 * (n_h + 1.0) + 32.0 + (72.0 if fused else 24.0))
     achieved = bytes_per_launch / (lj_ms * 1e-3) / 1e9 if lj_ms > 0 else None
     total_ms = res["mixed"]["seconds"] * 1e3
-    # the kernel as rocprofv3 names it: k_lj<MIXED, EV, TPA, INTEG, Q> with the
-    # autotuned threads per atom and the fixed-point positions of the mixed path
-    kname = "k_lj<true, false, %d, %s, true>" % (res["mixed"]["launch"]["tpa"], "true" if fused else "false")
+    # the kernel as rocprofv3 names it: k_lj<MIXED, EV, TPA, INTEG, Q, ONE> with
+    # the autotuned threads per atom, the fixed-point positions of the mixed
+    # path and the single-type specialisation (taken for TPA 4, 8 and 16)
+    tpa = res["mixed"]["launch"]["tpa"]
+    kname = "k_lj<true, false, %d, %s, true, %s>" % (tpa, "true" if fused else "false",
+                                                      "true" if tpa in (4, 8, 16) else "false")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=None if achieved is None else achieved / HBM_PEAK_GBS,
                     traffic=profiled_traffic(kname), kernel=kname, avg_launch_ms=lj_ms,

@@ -63,7 +63,7 @@ def build(verbose=False):
     # the LAMMPS-style script runner: C++ plugin classes over the C ABI
     exe = os.path.join(HERE, "lmp_mdx")
     cmd = ["g++", "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(HERE, "host", "plugin.cpp"),
-           os.path.join(HERE, "host", "lmp_mdx.cpp"), "-o", exe + ".tmp", "-L" + HERE, "-l:libmdx.so",
+           os.path.join(HERE, "host", "output.cpp"), os.path.join(HERE, "host", "lmp_mdx.cpp"), "-o", exe + ".tmp", "-L" + HERE, "-l:libmdx.so",
            "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -7,10 +7,13 @@
 // equal), units, atom_style atomic, boundary, lattice, region block,
 // create_box, create_atoms, mass, velocity create, pair_style
 // lj/cut|eam|eam/alloy|eam/fs, pair_coeff, pair_modify shift, neighbor, neigh_modify,
-// fix nve, timestep, thermo, run, plus `mdx` (precision, threads per atom,
-// list layout, fusion).  Output follows the reference's log: thermo rows
+// fix nve, timestep, thermo, compute pe/atom|stress/atom, dump custom,
+// dump_modify, undump, run, plus `mdx` (precision, threads per atom, list
+// layout, fusion).  Output follows the reference's log: thermo rows
 // "Step Temp E_pair E_mol TotEng Press" (thermo.cpp: %8d, %12.8g), the
-// loop time and the neighbour statistics (src/finish.cpp).
+// loop time and the neighbour statistics (src/finish.cpp); dumps follow
+// src/dump_custom.cpp.
+#include <algorithm>
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -23,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "output.h"
 #include "plugin.h"
 
 namespace lmpmdx {
@@ -124,6 +128,7 @@ class Lammps {
   int region_lo_[3] = {0, 0, 0}, region_hi_[3] = {0, 0, 0};
   bool region_ = false;
   int periodic_[3] = {1, 1, 1};
+  std::string boundstr_ = "pp pp pp";
   double boxlo_[3] = {0, 0, 0}, boxhi_[3] = {0, 0, 0};
   bool box_ = false;
   int ntypes_ = 0;
@@ -139,17 +144,25 @@ class Lammps {
   int tpa_ = 0, list_mode_ = MDX_LIST_GATHER, fused_ = 1;
   bool uploaded_ = false;
   long step_ = 0;
+  std::map<std::string, ComputePerAtom> computes_;
+  std::vector<std::unique_ptr<DumpCustom>> dumps_;
 
   std::string substitute(const std::string &line);
   void command(const std::vector<std::string> &w);
   void run(int nsteps);
   void upload();
+  void write_dumps(bool setup);
+  DumpCustom *find_dump(const std::string &id);
 };
 
+// $x and ${name} outside quotes (src/input.cpp:405-500)
 std::string Lammps::substitute(const std::string &line) {
   std::string out;
+  char quote = 0;
   for (size_t k = 0; k < line.size(); k++) {
-    if (line[k] != '$' || k + 1 >= line.size()) {
+    if (line[k] == quote) quote = 0;
+    else if (!quote && (line[k] == '"' || line[k] == '\'')) quote = line[k];
+    if (quote || line[k] != '$' || k + 1 >= line.size()) {
       out += line[k];
       continue;
     }
@@ -175,13 +188,41 @@ void Lammps::file(const std::string &path) {
   if (!in) throw Error("Cannot open input script " + path);
   std::string line;
   while (std::getline(in, line)) {
-    const size_t hash = line.find('#');
-    if (hash != std::string::npos) line = line.substr(0, hash);
+    // a '#' outside single or double quotes starts a comment (src/input.cpp:318-335)
+    char quote = 0;
+    for (size_t k = 0; k < line.size(); k++) {
+      if (line[k] == quote) quote = 0;
+      else if (!quote && (line[k] == '"' || line[k] == '\'')) quote = line[k];
+      else if (line[k] == '#' && !quote) {
+        line = line.substr(0, k);
+        break;
+      }
+    }
     line = substitute(line);
-    std::istringstream ss(line);
+    // words split on whitespace; quoted text is one word with the quotes
+    // removed (src/input.cpp:365-400)
     std::vector<std::string> w;
     std::string t;
-    while (ss >> t) w.push_back(t);
+    bool in_word = false;
+    quote = 0;
+    for (char ch : line) {
+      if (quote) {
+        if (ch == quote) quote = 0;
+        else t += ch;
+      } else if (ch == '"' || ch == '\'') {
+        quote = ch;
+        in_word = true;
+      } else if (std::isspace((unsigned char)ch)) {
+        if (in_word) w.push_back(t);
+        t.clear();
+        in_word = false;
+      } else {
+        t += ch;
+        in_word = true;
+      }
+    }
+    if (quote) throw Error("Unbalanced quotes in input line");
+    if (in_word) w.push_back(t);
     if (w.empty()) continue;
     std::printf("%s\n", line.c_str());
     command(w);
@@ -210,7 +251,12 @@ void Lammps::command(const std::vector<std::string> &w) {
     if (args.at(0) != "atomic") throw Error("this build supports atom_style atomic");
   } else if (c == "boundary") {
     if (args.size() != 3) throw Error("Illegal boundary command");
-    for (int d = 0; d < 3; d++) periodic_[d] = args[d] == "p" ? 1 : 0;
+    boundstr_.clear();
+    for (int d = 0; d < 3; d++) {
+      periodic_[d] = args[d] == "p" ? 1 : 0;
+      // src/domain.cpp:1554-1567: two letters per dimension
+      boundstr_ += (args[d].size() == 1 ? args[d] + args[d] : args[d]) + (d < 2 ? " " : "");
+    }
   } else if (c == "lattice") {
     if (args.size() != 2) throw Error("Illegal lattice command");
     lattice_style_ = args[0];
@@ -299,6 +345,25 @@ void Lammps::command(const std::vector<std::string> &w) {
     dt_ = std::strtod(args.at(0).c_str(), nullptr);
   } else if (c == "thermo") {
     thermo_every_ = std::atoi(args.at(0).c_str());
+  } else if (c == "compute") {
+    if (args.empty()) throw Error("Illegal compute command");
+    if (computes_.count(args[0])) throw Error("Reuse of compute ID");
+    computes_[args[0]] = ComputePerAtom::create(args);
+  } else if (c == "dump") {
+    if (args.empty()) throw Error("Illegal dump command");
+    if (find_dump(args[0])) throw Error("Reuse of dump ID");
+    dumps_.emplace_back(new DumpCustom(args, computes_));
+  } else if (c == "dump_modify") {
+    if (args.size() < 2) throw Error("Illegal dump_modify command");
+    DumpCustom *d = find_dump(args[0]);
+    if (!d) throw Error("Could not find dump_modify ID");
+    d->modify(std::vector<std::string>(args.begin() + 1, args.end()));
+  } else if (c == "undump") {
+    if (args.size() != 1) throw Error("Illegal undump command");
+    size_t k = 0;
+    while (k < dumps_.size() && dumps_[k]->id != args[0]) k++;
+    if (k == dumps_.size()) throw Error("Could not find undump ID");
+    dumps_.erase(dumps_.begin() + k);
   } else if (c == "mdx") {
     // mdx precision mixed|double ; mdx tpa N ; mdx list gather|brick ; mdx fused yes|no
     for (size_t k = 0; k + 1 < args.size(); k += 2) {
@@ -340,10 +405,15 @@ void Lammps::upload() {
 struct ThermoState {
   const Units *u;
   double natoms, vol;
+  long first, last;  // the run's first and last step
+  int every;
 };
 
 static void thermo_row(void *user, long step, const double *raw) {
   const ThermoState *s = static_cast<const ThermoState *>(user);
+  // a run split at dump steps reports the end of every piece: print only
+  // the rows of Output (first, last and multiples of thermo N)
+  if (!(step == s->first || step == s->last || (s->every > 0 && step % s->every == 0))) return;
   const double dof = 3.0 * s->natoms - 3.0;
   const double t = raw[0] * s->u->mvv2e / (dof * s->u->boltz);
   const bool norm = s->u->style == "lj";
@@ -354,18 +424,82 @@ static void thermo_row(void *user, long step, const double *raw) {
   std::printf("%8ld %12.8g %12.8g %12.8g %12.8g %12.8g \n", step, t, epair, 0.0, epair + ke, press);
 }
 
+DumpCustom *Lammps::find_dump(const std::string &id) {
+  for (auto &d : dumps_)
+    if (d->id == id) return d.get();
+  return nullptr;
+}
+
+// Output::setup / Output::write (src/output.cpp:171-214,291-320): a dump is
+// written on steps that are multiples of its N and not yet written, and at
+// setup also when `dump_modify first yes` asks for it.
+void Lammps::write_dumps(bool setup) {
+  std::vector<DumpCustom *> due;
+  bool peratom = false;
+  for (auto &d : dumps_) {
+    const bool w = (step_ % d->every == 0 && d->last != step_) || (setup && d->last < 0 && d->first_flag);
+    if (w) {
+      due.push_back(d.get());
+      peratom = peratom || d->needs_peratom();
+    }
+  }
+  if (due.empty()) return;
+  const int n = (int)type_.size();
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), f(3 * (size_t)n), eatom, vatom;
+  std::vector<int> image(3 * (size_t)n);
+  check(mdx_get_atoms(ctx_, x.data(), v.data(), f.data(), image.data()));
+  if (peratom) {
+    // the per-atom energy and virial of this step's positions (eflag_atom,
+    // vflag_atom of Integrate::ev_set, src/integrate.cpp:99-112)
+    eatom.resize(n);
+    vatom.resize(6 * (size_t)n);
+    pair_->compute(ctx_, 2, 4, nullptr, eatom.data(), vatom.data());
+  }
+  Snapshot snap;
+  snap.step = step_;
+  snap.n = n;
+  snap.type = type_.data();
+  snap.mass = mass_.data();
+  snap.x = x.data();
+  snap.v = v.data();
+  snap.f = f.data();
+  snap.image = image.data();
+  snap.eatom = peratom ? eatom.data() : nullptr;
+  snap.vatom = peratom ? vatom.data() : nullptr;
+  for (int d = 0; d < 3; d++) {
+    snap.boxlo[d] = boxlo_[d];
+    snap.boxhi[d] = boxhi_[d];
+  }
+  snap.boundstr = boundstr_;
+  snap.mvv2e = units_.mvv2e;
+  snap.nktv2p = units_.nktv2p;
+  for (DumpCustom *d : due) d->write(snap);
+}
+
 void Lammps::run(int nsteps) {
   if (!uploaded_) upload();
   const double dt = dt_ > 0 ? dt_ : units_.dt;
   ThermoState ts{&units_, (double)type_.size(),
-                 (boxhi_[0] - boxlo_[0]) * (boxhi_[1] - boxlo_[1]) * (boxhi_[2] - boxlo_[2])};
+                 (boxhi_[0] - boxlo_[0]) * (boxhi_[1] - boxlo_[1]) * (boxhi_[2] - boxlo_[2]), step_,
+                 step_ + nsteps, thermo_every_};
   std::printf("Step Temp E_pair E_mol TotEng Press \n");
   // Verlet::setup for every run command, then the timed loop
   check(mdx_setup(ctx_, thermo_row, &ts));
+  write_dumps(true);
   const auto t0 = std::chrono::steady_clock::now();
-  check(mdx_run(ctx_, nsteps, dt, units_.ftm2v, thermo_every_, thermo_row, &ts));
+  // the device runs uninterrupted between dump steps; each piece ends with
+  // x, v, f of its last step in place (mdx_run's final step is unfused)
+  const long end = step_ + nsteps;
+  while (step_ < end) {
+    long stop = end;
+    for (auto &d : dumps_) stop = std::min(stop, (step_ / d->every + 1) * d->every);
+    check(mdx_run(ctx_, (int)(stop - step_), dt, units_.ftm2v, thermo_every_, thermo_row, &ts));
+    step_ = stop;
+    write_dumps(false);
+  }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  step_ += nsteps;
+  // host copy of the state, so a later re-upload continues from here
+  if (!type_.empty()) check(mdx_get_atoms(ctx_, x_.data(), v_.data(), nullptr, nullptr));
   mdx_stats st;
   check(mdx_get_stats(ctx_, &st));
   std::printf("Loop time of %g on 1 procs for %d steps with %d atoms\n\n", secs, nsteps, (int)type_.size());
