@@ -119,6 +119,13 @@ void free_atoms(mdx_ctx *c) {
   c->cap_partials = 0;
 }
 
+// the fixed-point mirror xq is kept for mixed-precision gathers
+void update_qpos(mdx_ctx *c) {
+  const bool on = c->precision == MDX_MIXED && c->list_mode != MDX_LIST_BRICK && c->pos_format == MDX_POS_Q32;
+  if (on && !c->qpos) c->xq_stale = true;
+  c->qpos = on;
+}
+
 // Pair::init + Neighbor::init cutoffs (src/pair.cpp:219-231, src/neighbor.cpp:239-279)
 int prepare(mdx_ctx *c) {
   if (!c->box_set) {
@@ -145,6 +152,17 @@ int prepare(mdx_ctx *c) {
       c->cutneighmax = std::max(c->cutneighmax, cut);
       c->cutforce = std::max(c->cutforce, cutoff);
     }
+  update_qpos(c);
+  if (c->qpos) {
+    // the mirror's rounding (half a step per coordinate) must stay well
+    // inside the cutoff band of the fixed-point tests: relative rsq error
+    // 2*sqrt(3)*step/r below 5e-7 at the smallest cutoff
+    double rmin = 0.0, smax = 0.0;
+    for (int q = 0; q < nt1 * nt1; q++)
+      if (c->cutsq[q] > 0.0) rmin = rmin > 0.0 ? std::min(rmin, std::sqrt(c->cutsq[q])) : std::sqrt(c->cutsq[q]);
+    for (int d = 0; d < 3; d++) smax = std::max(smax, c->qs.step[d]);
+    if (2.0 * std::sqrt(3.0) * smax > 5e-7 * rmin) c->qpos = false;
+  }
   int rc = neigh_setup_grid(c);
   if (rc) return rc;
   // first-guess list capacity from the mean density; neigh_build regrows
@@ -244,11 +262,6 @@ int mdx_destroy(mdx_ctx *c) {
 }
 
 // the fixed-point mirror is kept while mixed precision gathers it
-static void update_qpos(mdx_ctx *c) {
-  const bool on = c->precision == MDX_MIXED && c->list_mode != MDX_LIST_BRICK && c->pos_format == MDX_POS_Q32;
-  if (on && !c->qpos) c->xq_stale = true;
-  c->qpos = on;
-}
 
 int mdx_set_box(mdx_ctx *c, const double lo[3], const double hi[3], const int periodic[3]) {
   int rc = check_ctx(c);

@@ -70,6 +70,44 @@ __device__ __forceinline__ double rsq_exact(double a0, double a1, double a2, dou
   return dx * dx + dy * dy + dz * dz;
 }
 
+// The FP64 keep test of one candidate with the image shift (shx, shy, shz):
+// rsq on the ghost coordinate as CommBrick::borders forms it, and for an
+// image pair within a hair of the cutoff, the reference's side rule.
+__device__ __forceinline__ bool keep_exact(const NeighArgs &a, const double4 &xi, const double4 &xj, double shx,
+                                           double shy, double shz, int code, double c) {
+  const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
+  const double r = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2);
+  bool keep = r <= c;
+  // the mirrored rsq of an image pair differs from r by a few ulps, so the
+  // reference's side can only matter within a hair of c
+  if (code != MDX_NOSHIFT && fabs(r - c) <= 1e-10 * c) {
+    const bool keep_b = rsq_exact(xj.x, xj.y, xj.z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
+    if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+  }
+  return keep;
+}
+
+// Fixed-point candidate test (mixed precision): rsq from the wrapping
+// integer differences of the 32-bit position mirror, in FP32.  Its error
+// (quantisation <= 5e-7 relative, enforced where the mirror is enabled, plus
+// FP32 rounding < 7e-7) is well inside kBuildBand, so a candidate outside
+// [c(1-band), c(1+band)] is decided exactly as the FP64 test decides it; the
+// rare one inside is re-tested by keep_exact on the FP64 positions.  The
+// wrapping difference is the minimum image, which is the candidate's own
+// image whenever that image can be within the cutoff, provided no atom
+// appears in the stencil under two images (2S+1 <= nbin in every periodic
+// dimension, checked on the host).
+constexpr double kBuildBand = 2e-6;
+struct QStep {
+  float s[3];
+};
+
+#ifndef MDX_BUILD_PRUNE
+#define MDX_BUILD_PRUNE 1
+#endif
+// 0: whole stencil, 1: planes, rows and row ends, 2: planes and rows
+constexpr int kPrune = MDX_BUILD_PRUNE;
+
 #ifndef MDX_BUILD_MIN_WAVES
 #define MDX_BUILD_MIN_WAVES 1
 #endif
@@ -82,6 +120,24 @@ __device__ __forceinline__ void stage_cutneighsq(const NeighArgs &a, double *s) 
   const int nt2 = a.ntypes1 * a.ntypes1;
   for (int q = threadIdx.x; q < nt2; q += blockDim.x) s[q] = a.cutneighsq[q];
   __syncthreads();
+}
+
+// the largest squared neighbour cutoff of each type (row maxima of the table)
+__device__ __forceinline__ void stage_cmax(const NeighArgs &a, double *s) {
+  for (int t = threadIdx.x; t < a.ntypes1; t += blockDim.x) {
+    double m = 0.0;
+    for (int u = 0; u < a.ntypes1; u++) m = fmax(m, a.cutneighsq[t * a.ntypes1 + u]);
+    s[t] = m;
+  }
+}
+
+// the FP32 decision bounds c(1 -/+ kBuildBand) of the fixed-point test
+__device__ __forceinline__ void stage_band(const NeighArgs &a, float *lo, float *hi) {
+  const int nt2 = a.ntypes1 * a.ntypes1;
+  for (int q = threadIdx.x; q < nt2; q += blockDim.x) {
+    lo[q] = static_cast<float>(a.cutneighsq[q] * (1.0 - kBuildBand));
+    hi[q] = static_cast<float>(a.cutneighsq[q] * (1.0 + kBuildBand));
+  }
 }
 
 // Domain::pbc + bin index + histogram; one thread per atom
@@ -258,18 +314,38 @@ __global__ MDX_BUILD_BOUNDS void k_build_full(NeighArgs a, const double4 *__rest
 // taken four at a time (four gathers in flight), and accepted entries are
 // stored four at a time as one 16-B store into the row.  Candidate order,
 // keep test and tie rule are those of k_build_full, so the rows are
-// identical entry for entry.
-__global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__restrict__ x,
+// identical entry for entry.  Q: candidates are read from the fixed-point
+// mirror (16 B instead of 32) and tested in FP32 (see kBuildBand); that
+// variant is held to 5 wavefronts per SIMD (96 VGPRs), which measured 9%
+// faster on 2 M atoms than the 4 its register count would otherwise give.
+//
+// Per-atom stencil pruning skips only candidates the keep test rejects, so
+// the rows do not change: a z-plane, a row or the end bins of a row whose
+// closest point lies beyond the largest cutoff of type ti hold no neighbour
+// of i (about a third of the 5x5x5 stencil for an atom inside its bin).
+// Distances are taken in bin units from t = (x - lo) * bininv, the
+// coordinate the binning used, in FP32: its rounding (below 1e-5 bin for
+// up to 10^4 bins per dimension) is dwarfed by the margins (1e-3 bin on
+// every gap and reach, 1e-4 relative on the squared cutoff).
+template <bool Q>
+__global__ __launch_bounds__(256, Q ? 5 : MDX_BUILD_MIN_WAVES) void k_build_lane(NeighArgs a, const double4 *__restrict__ x,
+                                                   const int4 *__restrict__ xq, QStep qs,
                                                    const unsigned *__restrict__ bkey,
                                                    const int *__restrict__ bin_start, int *__restrict__ nbr,
                                                    int *__restrict__ numneigh, int *__restrict__ dflags,
                                                    unsigned long long *__restrict__ nsum) {
   __shared__ double s_cut[MDX_NT1 * MDX_NT1];
+  __shared__ float s_lo[Q ? MDX_NT1 * MDX_NT1 : 1], s_hi[Q ? MDX_NT1 * MDX_NT1 : 1];
+  __shared__ double s_cmax[MDX_NT1];
+  if constexpr (Q) stage_band(a, s_lo, s_hi);
+  stage_cmax(a, s_cmax);
   stage_cutneighsq(a, s_cut);
   const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const double4 xi = x[i];
   const int ti = static_cast<int>(xi.w);
+  int4 qi = make_int4(0, 0, 0, 0);
+  if constexpr (Q) qi = xq[i];
   const unsigned b = bkey[i];
   const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
   int ix, iy, iz;
@@ -277,7 +353,18 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
   int *row = nbr + (size_t)i * a.maxneigh;  // maxneigh is a multiple of 4: rows are 16-B aligned
   int cnt = 0;
   int q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // pending entries cnt&~3 .. cnt-1
+  const float tx = static_cast<float>((xi.x - a.lo[0]) * a.bininv[0]),
+              ty = static_cast<float>((xi.y - a.lo[1]) * a.bininv[1]),
+              tz = static_cast<float>((xi.z - a.lo[2]) * a.bininv[2]);
+  const float bsy = static_cast<float>(1.0 / a.bininv[1]), bsz = static_cast<float>(1.0 / a.bininv[2]);
+  const float reach = static_cast<float>(s_cmax[ti] * (1.0 + 1e-4));
+  // FP32 bin coordinates stay within the margins up to 8192 bins a side
+  const int prune = (a.nbin[0] <= 8192 && a.nbin[1] <= 8192 && a.nbin[2] <= 8192) ? kPrune : 0;
   for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
+    // gap between atom i and the plane of bins iz+dz, in bins
+    const float gz = (dz > 0 ? (iz + dz) - tz : dz < 0 ? tz - (iz + dz + 1) : 0.0f) - 1e-3f;
+    const float dz2 = gz > 0.0f ? gz * gz * bsz * bsz : 0.0f;
+    if (prune && dz2 > reach) continue;
     int jz = iz + dz, sz = 0;
     if (jz < 0) {
       if (!a.periodic[2]) continue;
@@ -289,6 +376,9 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
       sz = 1;
     }
     for (int dy = -a.S[1]; dy <= a.S[1]; dy++) {
+      const float gy = (dy > 0 ? (iy + dy) - ty : dy < 0 ? ty - (iy + dy + 1) : 0.0f) - 1e-3f;
+      const float dyz2 = dz2 + (gy > 0.0f ? gy * gy * bsy * bsy : 0.0f);
+      if (prune && dyz2 > reach) continue;
       int jy = iy + dy, sy = 0;
       if (jy < 0) {
         if (!a.periodic[1]) continue;
@@ -300,7 +390,13 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
         sy = 1;
       }
       const int rowbase = bin_row(a, jy, jz) * nbx;
-      const int x0 = ix - a.S[0], x1 = ix + a.S[0];
+      // bins of this row within reach in x: gap max(0, jx - tx, tx - jx - 1) <= wb
+      int x0 = ix - a.S[0], x1 = ix + a.S[0];
+      if (prune == 1) {
+        const float wb = sqrtf(reach - dyz2) * static_cast<float>(a.bininv[0]) + 1e-3f;
+        x0 = max(x0, static_cast<int>(ceilf(tx - 1.0f - wb)));
+        x1 = min(x1, static_cast<int>(floorf(tx + wb)));
+      }
       const double shy = sy * a.prd[1], shz = sz * a.prd[2];
       for (int s = 0; s < 3; s++) {
         int lo, hi, sx;
@@ -325,25 +421,31 @@ __global__ MDX_BUILD_BOUNDS void k_build_lane(NeighArgs a, const double4 *__rest
         const int j0 = bin_start[rowbase + lo];
         const int j1 = bin_start[rowbase + hi + 1];
         for (int base = j0; base < j1; base += 4) {
-          double4 xj[4];
+          double4 xj[Q ? 1 : 4];
+          int4 qj[Q ? 4 : 1];
 #pragma unroll
-          for (int u = 0; u < 4; u++) xj[u] = x[min(base + u, j1 - 1)];  // in range; unused tail ignored
+          for (int u = 0; u < 4; u++) {  // in range; unused tail ignored
+            if constexpr (Q) qj[u] = xq[min(base + u, j1 - 1)];
+            else xj[u] = x[min(base + u, j1 - 1)];
+          }
 #pragma unroll
           for (int u = 0; u < 4; u++) {
             // no early exit: every loaded candidate is used, so the four
             // loads stay in flight together
             const int j = base + u;
             const bool valid = j < j1 && !(code == MDX_NOSHIFT && j == i);
-            const int tj = static_cast<int>(xj[u].w);
-            const double g0 = xj[u].x + shx, g1 = xj[u].y + shy, g2 = xj[u].z + shz;
-            const double c = s_cut[ti * a.ntypes1 + tj];
-            const double r = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2);
-            bool keep = valid && r <= c;
-            // the mirrored rsq of an image pair differs from r by a few ulps,
-            // so the reference's side can only matter within a hair of c
-            if (valid && code != MDX_NOSHIFT && fabs(r - c) <= 1e-10 * c) {
-              const bool keep_b = rsq_exact(xj[u].x, xj[u].y, xj[u].z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
-              if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
+            bool keep;
+            if constexpr (Q) {
+              const float dx = static_cast<float>(static_cast<int>((unsigned)qi.x - (unsigned)qj[u].x)) * qs.s[0];
+              const float dy = static_cast<float>(static_cast<int>((unsigned)qi.y - (unsigned)qj[u].y)) * qs.s[1];
+              const float dz = static_cast<float>(static_cast<int>((unsigned)qi.z - (unsigned)qj[u].z)) * qs.s[2];
+              const float rq = dx * dx + dy * dy + dz * dz;
+              const int t = ti * a.ntypes1 + qj[u].w;
+              keep = valid && rq <= s_lo[t];
+              if (valid && !keep && rq <= s_hi[t]) keep = keep_exact(a, xi, x[j], shx, shy, shz, code, s_cut[t]);
+            } else {
+              const int tj = static_cast<int>(xj[u].w);
+              keep = valid && keep_exact(a, xi, xj[u], shx, shy, shz, code, s_cut[ti * a.ntypes1 + tj]);
             }
             if (keep) {
               const int e = j | (code << MDX_IDX_BITS);
@@ -456,13 +558,30 @@ __global__ MDX_BUILD_BOUNDS void k_build_mask(NeighArgs a, const double4 *__rest
   atomicAdd(nsum, (unsigned long long)cnt);
 }
 
+// The fixed-point build needs the mirror (kept for mixed-precision gathers,
+// written by k_permute just before the build) and a stencil that never holds
+// one atom under two periodic images.
+bool q_build_ok(const mdx_ctx *c) {
+  if (!c->qpos || !c->xq) return false;
+  for (int d = 0; d < 3; d++)
+    if (c->na.periodic[d] && 2 * c->na.S[d] + 1 > c->na.nbin[d]) return false;
+  return true;
+}
+
 template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   if (TPA == 1) {
-    hipLaunchKernelGGL(k_build_lane, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start, c->nbr,
-                       c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
+    QStep qs{{(float)c->qs.step[0], (float)c->qs.step[1], (float)c->qs.step[2]}};
+    if (q_build_ok(c))
+      hipLaunchKernelGGL(k_build_lane<true>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->xq, qs, c->bkey_s,
+                         c->bin_start, c->nbr, c->numneigh, c->dflags,
+                         reinterpret_cast<unsigned long long *>(c->dflags + 8));
+    else
+      hipLaunchKernelGGL(k_build_lane<false>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, nullptr, qs, c->bkey_s,
+                         c->bin_start, c->nbr, c->numneigh, c->dflags,
+                         reinterpret_cast<unsigned long long *>(c->dflags + 8));
     return;
   }
   hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,

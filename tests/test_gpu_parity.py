@@ -349,6 +349,77 @@ def test_build_variants_agree_with_walls(lj_state, periodic, layout):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 0, 0), (1, 0, 1)])
+def test_fixed_point_build_gives_the_fp64_pair_set(lj_state, periodic):
+    """Mixed precision builds the list from the 32-bit fixed-point position
+    mirror (FP32 test, FP64 re-test within 2e-6 of the cutoff): pair set,
+    total and largest neighbour count equal those of the FP64-position build."""
+    from lammps_amd import mdx
+    x, _, _, t, _ = lj_state.state()
+    L = 20 * A_LJ
+    x = np.mod(x, L)
+    lo = tuple(0.0 if p else -1.0 for p in periodic)
+    hi = tuple(L if p else L + 1.0 for p in periodic)
+    out = []
+    for fmt in (mdx.POS_Q32, mdx.POS_FP64):
+        ctx = mdx.Context(0)
+        ctx.set_box(lo, hi, periodic)
+        ctx.set_atoms(x, t, np.array([0.0, 1.0]), tags=np.arange(1, len(t) + 1))
+        ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+        ctx.precision(mdx.MIXED)
+        ctx.position_format(fmt)
+        ctx.neigh_modify(0.3)
+        ctx.set_build_launch(1)
+        ctx.neigh_build()
+        st = ctx.stats()
+        out.append(({(int(a), int(b)) for a, b in ctx.neigh_pairs()}, st["total_neighbors"], st["max_neighbors"]))
+        ctx.close()
+    assert len(out[0][0]) > 0
+    assert out[0] == out[1]
+
+
+def test_fixed_point_build_near_the_cutoff_matches_the_oracle():
+    """Pairs placed at cutneigh*(1+d) for d from -3e-6 to 3e-6 (inside and
+    outside the FP64 re-test band), some through periodic images: the
+    mixed-precision build keeps exactly the oracle's FP64 pair set."""
+    from lammps_amd import mdx
+    L, cut = 21.0, 2.8
+    deltas = [-3e-6, -1e-6, -1e-7, -1e-9, -1e-12, 0.0, 1e-12, 1e-9, 1e-7, 1e-6, 3e-6]
+    rng = np.random.default_rng(77)
+    g = np.arange(3) * 7.0 + 0.5
+    centers = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = []
+    for k, c in enumerate(centers):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        if k % 2:
+            u = -np.abs(u)  # towards the low faces: partners of the 0.5 row cross the boundary
+        pts += [c, np.mod(c + cut * (1.0 + deltas[k % len(deltas)]) * u, L)]
+    x = np.array(pts)
+    n = len(x)
+    ref = M.Ref(M.LJ)
+    ref.box((0, 0, 0), (L, L, L))
+    ref.add_atoms(x, np.ones(n, dtype=np.int32))
+    ref.mass(1, 1.0)
+    ref.pair_lj(2.5)
+    ref.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ref.neighbor(0.3)
+    ref.setup()
+    want = {(min(a, b), max(a, b)) for a, b in ref.list_pairs()}
+    ctx = mdx.Context(0)
+    ctx.set_box((0, 0, 0), (L, L, L), (1, 1, 1))
+    ctx.set_atoms(x, np.ones(n, dtype=np.int32), np.array([0.0, 1.0]), tags=np.arange(1, n + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    ctx.precision(mdx.MIXED)
+    ctx.neigh_modify(0.3)
+    ctx.set_build_launch(1)
+    ctx.neigh_build()
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    ctx.close()
+    assert 0 < len(want) < len(centers)  # about half of the placed pairs are kept
+    assert got == want
+
+
 # ---------------------------------------------------------------- mask lists
 @pytest.mark.parametrize("style", ["lj", "eam"])
 def test_mask_list_pair_set_identical(lj_state, eam_state, style):
