@@ -7,8 +7,8 @@
 // Design: no ghost atoms.  At each build the owned atoms are wrapped into the
 // box (Domain::pbc, src/domain.cpp:478), binned on the reference's own bin
 // grid (binsize = cutneigh/2, src/neighbor.cpp:1580-1620), and stably
-// radix-sorted by bin so that every x-row of bins is one contiguous atom
-// range.  A group of TPA lanes per atom then scans the 5x5 rows of its
+// sorted by bin (counting sort: arrival slots, then ranks by atom index)
+// so that every x-row of bins is one contiguous atom range.  A group of TPA lanes per atom then scans the 5x5 rows of its
 // stencil (each row at most 3 contiguous segments: wrapped-low, main,
 // wrapped-high) and keeps j iff rsq <= cutneighsq, with rsq evaluated in
 // FP64 without contraction on the periodic image x_j + s*L, the value the
@@ -140,9 +140,11 @@ __device__ __forceinline__ void stage_band(const NeighArgs &a, float *lo, float 
   }
 }
 
-// Domain::pbc + bin index + histogram; one thread per atom
+// Domain::pbc + bin index + histogram; one thread per atom.  slot[i] is the
+// atom's arrival order in its bin (not deterministic; k_bin_rank orders
+// each bin by atom index).
 __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict__ image,
-                           unsigned *__restrict__ key, int *__restrict__ idx, int *__restrict__ count,
+                           unsigned *__restrict__ key, int *__restrict__ slot, int *__restrict__ count,
                            int *__restrict__ dflags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -179,8 +181,32 @@ __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict
   image[i] = (im[2] << 20) | (im[1] << 10) | im[0];
   unsigned b = (unsigned)bin_index(a, ib[0], ib[1], ib[2]);
   key[i] = b;
-  idx[i] = i;
-  atomicAdd(&count[b], 1);
+  slot[i] = atomicAdd(&count[b], 1);
+}
+
+// Stable counting sort by bin, in two passes after the scan of the bin
+// counts: each atom lands in its bin's range at its arrival slot, then each
+// entry of a bin takes the rank of its atom index among the bin's entries.
+// The result is the order a stable sort of (bin, current index) gives:
+// bins ascending, current index ascending within a bin.
+__global__ void k_bin_scatter(int n, const unsigned *__restrict__ key, const int *__restrict__ slot,
+                              const int *__restrict__ bin_start, int *__restrict__ tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tmp[bin_start[key[i]] + slot[i]] = i;
+}
+
+__global__ void k_bin_rank(int n, const unsigned *__restrict__ key, const int *__restrict__ bin_start,
+                           const int *__restrict__ tmp, int *__restrict__ src, unsigned *__restrict__ key_s) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = tmp[p];
+  const unsigned b = key[i];
+  const int s = bin_start[b], e = bin_start[b + 1];
+  int r = 0;
+  for (int k = s; k < e; k++) r += tmp[k] < i;
+  src[s + r] = i;
+  key_s[s + r] = b;
 }
 
 __global__ void k_permute(int n, const int *__restrict__ src, const double4 *__restrict__ x,
@@ -677,12 +703,6 @@ int alloc_mask(mdx_ctx *c, int words) {
   return MDX_OK;
 }
 
-static int end_bit_for(int nbins) {
-  int bits = 1;
-  while ((1L << bits) < nbins) bits++;
-  return bits;
-}
-
 int neigh_build(mdx_ctx *c) {
   NeighArgs &a = c->na;
   a.n = c->n;
@@ -701,23 +721,9 @@ int neigh_build(mdx_ctx *c) {
   if ((e = hipMemsetAsync(c->dflags, 0, 64, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
   hipLaunchKernelGGL(k_wrap_bin, dim3(G), dim3(B), 0, c->stream, a, c->x, c->image, c->bkey, c->bidx,
                      c->bin_count, c->dflags);
-  // stable radix sort of (bin, current index): deterministic order in a bin
-  const int eb = end_bit_for(c->nbins);
+  // stable sort of (bin, current index): bin_start from the counts, then
+  // scatter by arrival slot and rank within each bin (deterministic order)
   size_t need = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need, c->bkey, c->bkey_s, c->bidx, c->bidx_s, c->n, 0, eb,
-                                     c->stream);
-  if (need > c->sort_tmp_bytes) {
-    (void)hipFree(c->sort_tmp);
-    c->sort_tmp_bytes = need;
-    if (hipMalloc(&c->sort_tmp, need) != hipSuccess) {
-      set_error("Insufficient memory on accelerator (sort)");
-      return MDX_ERR_NOMEM;
-    }
-  }
-  e = hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, need, c->bkey, c->bkey_s, c->bidx, c->bidx_s, c->n, 0,
-                                         eb, c->stream);
-  if (e != hipSuccess) return hip_fail(e, "radix sort");
-  need = 0;
   (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
   if (need > c->scan_tmp_bytes) {
     (void)hipFree(c->scan_tmp);
@@ -729,7 +735,10 @@ int neigh_build(mdx_ctx *c) {
   }
   e = hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
   if (e != hipSuccess) return hip_fail(e, "bin scan");
-  hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx_s, c->x, c->v, c->tag, c->orig,
+  hipLaunchKernelGGL(k_bin_scatter, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bidx, c->bin_start, c->bidx_s);
+  hipLaunchKernelGGL(k_bin_rank, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bin_start, c->bidx_s, c->bidx,
+                     c->bkey_s);
+  hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx, c->x, c->v, c->tag, c->orig,
                      c->image, c->xb, c->vb, c->tagb, c->origb, c->imageb, c->inv, c->xhold,
                      c->qpos ? c->xq : nullptr, c->qs);
   if (c->qpos) c->xq_stale = false;
