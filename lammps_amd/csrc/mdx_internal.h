@@ -96,6 +96,9 @@ __device__ __forceinline__ void eam_spline_index(T r, T rdr, int nr, int &m, T &
   p = p < T(1) ? p : T(1);
 }
 
+// FP32 rho spline value from the knot's c3..c6 vector at fraction p
+__device__ __forceinline__ float eam_rho_eval(const float4 &c, float p) { return ((c.x * p + c.y) * p + c.z) * p + c.w; }
+
 // rho contribution of a neighbour at rsq through table `tab` (pair_eam.cpp:193-196)
 template <bool MIXED>
 __device__ __forceinline__ double eam_rho_term(const EAMArgs &a, int tab, double rsq) {
@@ -103,13 +106,30 @@ __device__ __forceinline__ double eam_rho_term(const EAMArgs &a, int tab, double
   if (MIXED) {
     float p;
     eam_spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
-    const float4 c = a.rhor_pf[2 * ((size_t)tab * (a.nr + 1) + m) + 1];  // c3..c6
-    return (double)(((c.x * p + c.y) * p + c.z) * p + c.w);
+    return (double)eam_rho_eval(a.rhor_pf[2 * ((size_t)tab * (a.nr + 1) + m) + 1], p);  // c3..c6
   }
   double p;
   eam_spline_index<double>(sqrt(rsq), a.rdr, a.nr, m, p);
   const double4 c = a.rhor_p[2 * ((size_t)tab * (a.nr + 1) + m) + 1];
   return ((c.x * p + c.y) * p + c.z) * p + c.w;
+}
+
+// FP32 force-pass arithmetic of one pair from its knot vectors: rho'_i
+// (ci: c0..c2 of type2rhor[ti][tj]), rho'_j (cj), z2' (zd: c0..c2) and z2
+// (zv: c3..c6) at r, fraction p
+__device__ __forceinline__ void eam_pair_eval(const float4 &ci, const float4 &cj, const float4 &zd,
+                                              const float4 &zv, float r, float p, double fpi, double fpj,
+                                              double &fpair, double &phi) {
+  const float rhoip = (ci.x * p + ci.y) * p + ci.z;
+  const float rhojp = (cj.x * p + cj.y) * p + cj.z;
+  const float z2p = (zd.x * p + zd.y) * p + zd.z;
+  const float z2 = ((zv.x * p + zv.y) * p + zv.z) * p + zv.w;
+  const float recip = __builtin_amdgcn_rcpf(r);  // hardware reciprocal, 1 ulp
+  const float phif = z2 * recip;
+  const float phip = z2p * recip - phif * recip;
+  const float psip = (float)fpi * rhojp + (float)fpj * rhoip + phip;
+  fpair = (double)(-psip * recip);
+  phi = (double)phif;
 }
 
 // pair force factor fpair (f_i += del * fpair) and phi of the force pass
@@ -127,17 +147,7 @@ __device__ __forceinline__ void eam_pair_term(const EAMArgs &a, int ri, int rj, 
     const float4 ci = a.rhor_pf[2 * (ri * kn + m)];  // c0..c2
     float4 cj = ci;  // (a conditional lvalue here would put ci in scratch memory)
     if (ri != rj) cj = a.rhor_pf[2 * (rj * kn + m)];
-    const float4 zd = a.z2r_pf[2 * (zi * kn + m)], zv = a.z2r_pf[2 * (zi * kn + m) + 1];
-    const float rhoip = (ci.x * p + ci.y) * p + ci.z;
-    const float rhojp = (cj.x * p + cj.y) * p + cj.z;
-    const float z2p = (zd.x * p + zd.y) * p + zd.z;
-    const float z2 = ((zv.x * p + zv.y) * p + zv.z) * p + zv.w;
-    const float recip = __builtin_amdgcn_rcpf(r);  // hardware reciprocal, 1 ulp
-    const float phif = z2 * recip;
-    const float phip = z2p * recip - phif * recip;
-    const float psip = (float)fpi * rhojp + (float)fpj * rhoip + phip;
-    fpair = (double)(-psip * recip);
-    phi = (double)phif;
+    eam_pair_eval(ci, cj, a.z2r_pf[2 * (zi * kn + m)], a.z2r_pf[2 * (zi * kn + m) + 1], r, p, fpi, fpj, fpair, phi);
     return;
   }
   const double r = sqrt(rsq);

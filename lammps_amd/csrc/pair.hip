@@ -344,15 +344,24 @@ __device__ __forceinline__ bool pair_inside(const double4 &xi, const int4 &qi, c
   return rsq < cutsq;
 }
 
-template <bool MIXED, bool EV, int TPA, bool Q, bool ONE>
+// TL (mixed, one type): the spline knots the kernel reads are copied into
+// LDS first (dynamic shared memory, (nr+1) float4 per vector kind), so a
+// lookup is an LDS read instead of a cached global gather; same arithmetic.
+template <bool MIXED, bool EV, int TPA, bool Q, bool ONE, bool TL>
 __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, double py, double pz, EAMArgs a,
                                                 const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                                 const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                                 double *__restrict__ rho_out, double *__restrict__ fp_out,
                                                 double *__restrict__ eemb) {
+  static_assert(!TL || (MIXED && ONE), "LDS tables hold the one FP32 table of a single-type system");
   // type map in LDS: a kernel-argument array indexed per lane is a memory read
   __shared__ int s_t2r[kNT2];
+  extern __shared__ float4 s_eam_tab[];  // TL: c3..c6 of knots 0..nr
   for (int q = threadIdx.x; q < a.ntypes1 * a.ntypes1; q += blockDim.x) s_t2r[q] = a.type2rhor[q];
+  if constexpr (TL) {
+    const size_t kn = (size_t)(a.nr + 1), t = a.type2rhor[a.ntypes1 + 1];
+    for (int m = threadIdx.x; m < (int)kn; m += blockDim.x) s_eam_tab[m] = a.rhor_pf[2 * (t * kn + m) + 1];
+  }
   __syncthreads();
   const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   const int lane = threadIdx.x % TPA;
@@ -394,7 +403,14 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
         if (pair_inside<Q>(xi, qi, x, jc[u], xj[Q ? 0 : u], qj[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
                            delx, dely, delz, rsq)) {
           const int tj = Q ? qj[Q ? u : 0].w : static_cast<int>(xj[Q ? 0 : u].w);
-          rho += eam_rho_term<MIXED>(a, s_t2r[ONE ? a.ntypes1 + 1 : tj * a.ntypes1 + ti], rsq);
+          if constexpr (TL) {
+            int m;
+            float p;
+            eam_spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
+            rho += (double)eam_rho_eval(s_eam_tab[m], p);
+          } else {
+            rho += eam_rho_term<MIXED>(a, s_t2r[ONE ? a.ntypes1 + 1 : tj * a.ntypes1 + ti], rsq);
+          }
         }
       }
     }
@@ -419,7 +435,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
   }
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE, bool TL>
 __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px, double py, double pz,
                                                   EAMArgs a, const double4 *__restrict__ x,
                                                   const int4 *__restrict__ xq, QScale qs,
@@ -427,11 +443,23 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
                                                   const double *__restrict__ fp, const double *__restrict__ eemb,
                                                   double4 *__restrict__ f, double *__restrict__ vatom,
                                                   double *__restrict__ partials, IntegArgs ig) {
+  static_assert(!TL || (MIXED && ONE), "LDS tables hold the one FP32 table of a single-type system");
   // type maps in LDS: kernel-argument arrays indexed per lane are memory reads
   __shared__ int s_t2r[kNT2], s_t2z[kNT2];
+  // TL: rho' c0..c2, z2' c0..c2 and z2 c3..c6 of knots 0..nr, one after the other
+  extern __shared__ float4 s_eam_tab[];
   for (int q = threadIdx.x; q < a.ntypes1 * a.ntypes1; q += blockDim.x) {
     s_t2r[q] = a.type2rhor[q];
     s_t2z[q] = a.type2z2r[q];
+  }
+  const int kn = a.nr + 1;
+  if constexpr (TL) {
+    const size_t t = a.type2rhor[a.ntypes1 + 1], z = a.type2z2r[a.ntypes1 + 1];
+    for (int m = threadIdx.x; m < kn; m += blockDim.x) {
+      s_eam_tab[m] = a.rhor_pf[2 * (t * kn + m)];
+      s_eam_tab[kn + m] = a.z2r_pf[2 * (z * kn + m)];
+      s_eam_tab[2 * kn + m] = a.z2r_pf[2 * (z * kn + m) + 1];
+    }
   }
   __syncthreads();
   const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
@@ -486,7 +514,16 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
         const int tj = Q ? qjv[Q ? u : 0].w : static_cast<int>(xjv[Q ? 0 : u].w);
         const int tij = ONE ? a.ntypes1 + 1 : ti * a.ntypes1 + tj, tji = ONE ? tij : tj * a.ntypes1 + ti;
         double fpair, phi;
-        eam_pair_term<MIXED>(a, s_t2r[tij], s_t2r[tji], s_t2z[tij], rsq, fpi, fpjv[u], fpair, phi);
+        if constexpr (TL) {
+          const float r = sqrtf((float)rsq);
+          int m;
+          float p;
+          eam_spline_index<float>(r, (float)a.rdr, a.nr, m, p);
+          const float4 ci = s_eam_tab[m];
+          eam_pair_eval(ci, ci, s_eam_tab[kn + m], s_eam_tab[2 * kn + m], r, p, fpi, fpjv[u], fpair, phi);
+        } else {
+          eam_pair_term<MIXED>(a, s_t2r[tij], s_t2r[tji], s_t2z[tij], rsq, fpi, fpjv[u], fpair, phi);
+        }
         fx += delx * fpair;
         fy += dely * fpair;
         fz += delz * fpair;
@@ -870,18 +907,24 @@ void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   c->last_grid = G;
 }
 
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
+// knots (nr+1) up to which the single-type FP32 tables go to LDS: the force
+// pass holds three float4 per knot (48 KiB at this limit)
+constexpr int kEamLdsKnots = 1024;
+
+template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE, bool TL>
 void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
+  const size_t kn = (size_t)c->eam.nr + 1;
   prof_begin(c, MDX_K_EAM_RHO);
-  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
-                     c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->rho, c->fp,
-                     c->eemb);
+  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>), dim3(G), dim3(B), TL ? kn * sizeof(float4) : 0,
+                     c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
+                     c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
-  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
-                     c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh,
-                     c->fp, c->eemb, c->f, c->vatom, c->partials, ig);
+  hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE, TL>), dim3(G), dim3(B),
+                     TL ? 3 * kn * sizeof(float4) : 0, c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1],
+                     c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
+                     c->partials, ig);
   prof_end(c);
   c->last_grid = G;
 }
@@ -897,9 +940,14 @@ void launch_lj(mdx_ctx *c, int B, const IntegArgs &ig) {
 template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
 void launch_eam(mdx_ctx *c, int B, const IntegArgs &ig) {
   if constexpr (TPA == 4 || TPA == 8 || TPA == 16) {
-    if (c->ntypes == 1) return launch_eam_t<MIXED, EV, TPA, INTEG, Q, true>(c, B, ig);
+    if (c->ntypes == 1) {
+      if constexpr (MIXED) {
+        if (c->eam.nr + 1 <= kEamLdsKnots) return launch_eam_t<MIXED, EV, TPA, INTEG, Q, true, true>(c, B, ig);
+      }
+      return launch_eam_t<MIXED, EV, TPA, INTEG, Q, true, false>(c, B, ig);
+    }
   }
-  launch_eam_t<MIXED, EV, TPA, INTEG, Q, false>(c, B, ig);
+  launch_eam_t<MIXED, EV, TPA, INTEG, Q, false, false>(c, B, ig);
 }
 
 template <bool MIXED, bool EV, bool INTEG, bool Q>
