@@ -102,6 +102,14 @@ struct QStep {
   float s[3];
 };
 
+#ifndef MDX_BUILD_Q_WAVES
+#define MDX_BUILD_Q_WAVES 5
+#endif
+#ifndef MDX_BUILD_LANE_U
+#define MDX_BUILD_LANE_U 4
+#endif
+constexpr int kLaneU = MDX_BUILD_LANE_U;  // candidates loaded together per lane
+
 #ifndef MDX_BUILD_PRUNE
 #define MDX_BUILD_PRUNE 1
 #endif
@@ -354,7 +362,7 @@ __global__ MDX_BUILD_BOUNDS void k_build_full(NeighArgs a, const double4 *__rest
 // up to 10^4 bins per dimension) is dwarfed by the margins (1e-3 bin on
 // every gap and reach, 1e-4 relative on the squared cutoff).
 template <bool Q>
-__global__ __launch_bounds__(256, Q ? 5 : MDX_BUILD_MIN_WAVES) void k_build_lane(NeighArgs a, const double4 *__restrict__ x,
+__global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) void k_build_lane(NeighArgs a, const double4 *__restrict__ x,
                                                    const int4 *__restrict__ xq, QStep qs,
                                                    const unsigned *__restrict__ bkey,
                                                    const int *__restrict__ bin_start, int *__restrict__ nbr,
@@ -446,16 +454,16 @@ __global__ __launch_bounds__(256, Q ? 5 : MDX_BUILD_MIN_WAVES) void k_build_lane
         const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
         const int j0 = bin_start[rowbase + lo];
         const int j1 = bin_start[rowbase + hi + 1];
-        for (int base = j0; base < j1; base += 4) {
-          double4 xj[Q ? 1 : 4];
-          int4 qj[Q ? 4 : 1];
+        for (int base = j0; base < j1; base += kLaneU) {
+          double4 xj[Q ? 1 : kLaneU];
+          int4 qj[Q ? kLaneU : 1];
 #pragma unroll
-          for (int u = 0; u < 4; u++) {  // in range; unused tail ignored
+          for (int u = 0; u < kLaneU; u++) {  // in range; unused tail ignored
             if constexpr (Q) qj[u] = xq[min(base + u, j1 - 1)];
             else xj[u] = x[min(base + u, j1 - 1)];
           }
 #pragma unroll
-          for (int u = 0; u < 4; u++) {
+          for (int u = 0; u < kLaneU; u++) {
             // no early exit: every loaded candidate is used, so the four
             // loads stay in flight together
             const int j = base + u;
