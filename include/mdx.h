@@ -64,6 +64,7 @@ enum {
   MDX_K_NVE_INITIAL = 4, /* fix nve initial integrate (+ rebuild trigger) */
   MDX_K_NVE_FINAL = 5,   /* fix nve final integrate */
   MDX_K_FUSED = 6,       /* force + final + next initial integrate, fused */
+  MDX_K_HALO = 7,        /* ghost-shell pack / unpack (mdx_halo_pack, mdx_halo_unpack) */
   MDX_NKERNEL = 8
 };
 
@@ -209,6 +210,34 @@ int mdx_set_list_mode(mdx_ctx *ctx, int mode, int rx, int ry, int rz);
    MDX_POS_FP64: neighbours are read as FP64 positions.  FP64 precision
    always reads FP64 positions. */
 int mdx_set_position_format(mdx_ctx *ctx, int format);
+
+/* Ghost-shell exchange of a spatially decomposed run (one context per GPU
+   sub-box; the packed buffers travel between GPUs by RCCL send/recv, which
+   the caller issues).  Replaces CommBrick's per-step forward_comm of ghost
+   positions (src/comm_brick.cpp:398-460) with the send lists that borders
+   builds at each reneighbouring (src/comm_brick.cpp:696-864), and
+   AtomVecAtomic::pack_comm / unpack_comm with the pbc shift
+   (src/atom_vec_atomic.cpp:117-190).
+
+   mdx_halo_sendlist: swap in [0, 27) selects the atoms of this context with
+   lo <= x < hi (each dimension), in device order, and records `shift` (the
+   image shift prd * pbc a ghost of them carries).  *count gets the number
+   selected.  The list stays valid until the next list build of this
+   context; rebuild it after each one, as borders does.
+   mdx_halo_pack: writes the listed atoms' x + shift, 3 doubles each, into
+   buf (device memory of this context's GPU).
+   mdx_halo_unpack: writes count received positions (3 doubles each, device
+   memory) into this context's atoms first .. first+count-1 (caller order):
+   the ghost atoms, which the caller appended after the owned ones.  Ghost
+   atoms are ordinary atoms of the context to the list build and the force
+   kernels (mdx_compute), and their forces are not used.  mdx_run
+   integrates every atom of a context, so the step loop of a sharded run
+   (owned-only integration, ghost refresh between the initial integrate and
+   the force kernel) is not part of this library. */
+int mdx_halo_sendlist(mdx_ctx *ctx, int swap, const double lo[3], const double hi[3], const double shift[3],
+                      int *count);
+int mdx_halo_pack(mdx_ctx *ctx, int swap, double *buf);
+int mdx_halo_unpack(mdx_ctx *ctx, int first, int count, const double *buf);
 
 #ifdef __cplusplus
 }

@@ -251,6 +251,7 @@ int mdx_destroy(mdx_ctx *c) {
                   c->sort_tmp, c->scan_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  halo_free(c);
   if (c->red_host) (void)hipHostFree(c->red_host);
   for (auto &r : c->evpool) {
     (void)hipEventDestroy(r.a);
@@ -260,8 +261,6 @@ int mdx_destroy(mdx_ctx *c) {
   delete c;
   return MDX_OK;
 }
-
-// the fixed-point mirror is kept while mixed precision gathers it
 
 int mdx_set_box(mdx_ctx *c, const double lo[3], const double hi[3], const int periodic[3]) {
   int rc = check_ctx(c);
@@ -312,6 +311,7 @@ int mdx_set_atoms(mdx_ctx *c, int n, int ntypes, const double *x, const double *
       return MDX_ERR_ARG;
     }
   free_atoms(c);
+  for (int k = 0; k < MDX_HALO_SWAPS; k++) c->halo_count[k] = 0;  // send lists name old atoms
   c->n = n;
   c->ntypes = ntypes;
   for (int t = 0; t < MDX_NT1; t++) c->mass[t] = t <= ntypes ? mass[t] : 0.0;

@@ -1,0 +1,220 @@
+// halo.hip - device side of the ghost-shell exchange of a spatially
+// decomposed run: send lists, pack with the periodic image shift, unpack
+// into the receiving context's ghost atoms.
+//
+// Replaces the per-step forward communication of CommBrick
+// (src/comm_brick.cpp:398-460 forward_comm; send lists built by borders,
+// src/comm_brick.cpp:696-864) and AtomVecAtomic's pack_comm / unpack_comm
+// with the pbc shift (src/atom_vec_atomic.cpp:117-190).  A send list is the
+// set of atoms whose position lies in a box [lo, hi), chosen on the device
+// by a flag pass, a scan and a stable compaction (device order, so the list
+// is deterministic).  Pack writes x + shift for each listed atom, the value
+// the reference stores for a ghost image (x[j] + prd * pbc); unpack writes
+// received positions into consecutive caller-order atoms of the receiving
+// context.  Between ranks the packed buffer travels by RCCL send/recv; this
+// file does not move data between devices.
+#include <hipcub/hipcub.hpp>
+
+#include "mdx_internal.h"
+
+namespace mdx {
+
+namespace {
+
+__global__ void k_halo_flag(int n, const double4 *__restrict__ x, double lo0, double lo1, double lo2, double hi0,
+                            double hi1, double hi2, int *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {  // the scan's last element: the total lands in flag[n]
+    flag[n] = 0;
+    return;
+  }
+  const double4 p = x[i];
+  flag[i] = (p.x >= lo0 && p.x < hi0 && p.y >= lo1 && p.y < hi1 && p.z >= lo2 && p.z < hi2) ? 1 : 0;
+}
+
+// pos: exclusive scan of the flags (recomputed here from flag to keep one
+// buffer: the scan is written in place after the flags were consumed)
+__global__ void k_halo_compact(int n, const double4 *__restrict__ x, double lo0, double lo1, double lo2, double hi0,
+                               double hi1, double hi2, const int *__restrict__ pos, int *__restrict__ list) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 p = x[i];
+  if (p.x >= lo0 && p.x < hi0 && p.y >= lo1 && p.y < hi1 && p.z >= lo2 && p.z < hi2) list[pos[i]] = i;
+}
+
+__global__ void k_halo_pack(int count, const int *__restrict__ list, const double4 *__restrict__ x, double s0,
+                            double s1, double s2, double *__restrict__ buf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const double4 p = x[list[k]];
+  buf[3 * k] = p.x + s0;
+  buf[3 * k + 1] = p.y + s1;
+  buf[3 * k + 2] = p.z + s2;
+}
+
+__global__ void k_halo_unpack(int count, int first, const int *__restrict__ inv, const double *__restrict__ buf,
+                              double4 *__restrict__ x, int4 *__restrict__ xq, QScale qs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int i = inv[first + k];
+  double4 p = x[i];  // keeps the type in w
+  p.x = buf[3 * k];
+  p.y = buf[3 * k + 1];
+  p.z = buf[3 * k + 2];
+  x[i] = p;
+  if (xq) xq[i] = quantize(qs, p);
+}
+
+int check_ctx(mdx_ctx *c) {
+  if (!c) {
+    set_error("null context");
+    return MDX_ERR_ARG;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "hipSetDevice");
+}
+
+// [p, p + bytes) lies inside one device allocation of `device`
+bool device_range(const void *p, size_t bytes, int device) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (a.type != hipMemoryTypeDevice || a.device != device) return false;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const char *b = static_cast<const char *>(base), *q = static_cast<const char *>(p);
+  return q >= b && q + bytes <= b + size;
+}
+
+}  // namespace
+
+void halo_free(mdx_ctx *c) {
+  for (int s = 0; s < MDX_HALO_SWAPS; s++) {
+    if (c->halo_list[s]) (void)hipFree(c->halo_list[s]);
+    c->halo_list[s] = nullptr;
+    c->halo_cap[s] = c->halo_count[s] = 0;
+  }
+  if (c->halo_flag) (void)hipFree(c->halo_flag);
+  if (c->halo_scan_tmp) (void)hipFree(c->halo_scan_tmp);
+  c->halo_flag = nullptr;
+  c->halo_scan_tmp = nullptr;
+  c->halo_flag_cap = 0;
+  c->halo_scan_bytes = 0;
+}
+
+}  // namespace mdx
+
+using namespace mdx;
+
+extern "C" {
+
+int mdx_halo_sendlist(mdx_ctx *c, int swap, const double lo[3], const double hi[3], const double shift[3],
+                      int *count) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (swap < 0 || swap >= MDX_HALO_SWAPS || !lo || !hi || !shift || !count) {
+    set_error("mdx_halo_sendlist: swap in [0, 27) and non-null lo, hi, shift, count required");
+    return MDX_ERR_ARG;
+  }
+  const int n = c->n;
+  *count = 0;
+  for (int d = 0; d < 3; d++) c->halo_shift[swap][d] = shift[d];
+  c->halo_count[swap] = 0;
+  if (n == 0) return MDX_OK;
+  hipError_t e;
+  if (n + 1 > c->halo_flag_cap) {
+    if (c->halo_flag) (void)hipFree(c->halo_flag);
+    c->halo_flag = nullptr;
+    c->halo_flag_cap = 0;
+    if ((e = hipMalloc(&c->halo_flag, sizeof(int) * 2 * (size_t)(n + 1))) != hipSuccess)
+      return hip_fail(e, "halo flags");
+    c->halo_flag_cap = n + 1;
+  }
+  int *flag = c->halo_flag, *pos = c->halo_flag + c->halo_flag_cap;
+  size_t need = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, pos, n + 1, c->stream);
+  if (need > c->halo_scan_bytes) {
+    if (c->halo_scan_tmp) (void)hipFree(c->halo_scan_tmp);
+    c->halo_scan_tmp = nullptr;
+    c->halo_scan_bytes = 0;
+    if ((e = hipMalloc(&c->halo_scan_tmp, need)) != hipSuccess) return hip_fail(e, "halo scan");
+    c->halo_scan_bytes = need;
+  }
+  const int B = 256;
+  hipLaunchKernelGGL(k_halo_flag, dim3((n + 1 + B - 1) / B), dim3(B), 0, c->stream, n, c->x, lo[0], lo[1], lo[2],
+                     hi[0], hi[1], hi[2], flag);
+  if ((e = hipcub::DeviceScan::ExclusiveSum(c->halo_scan_tmp, need, flag, pos, n + 1, c->stream)) != hipSuccess)
+    return hip_fail(e, "halo scan");
+  int total = 0;
+  if ((e = hipMemcpyAsync(&total, pos + n, sizeof(int), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return hip_fail(e, "halo count");
+  if (total > c->halo_cap[swap]) {
+    if (c->halo_list[swap]) (void)hipFree(c->halo_list[swap]);
+    c->halo_list[swap] = nullptr;
+    c->halo_cap[swap] = 0;
+    const int cap = total + total / 8 + 64;
+    if ((e = hipMalloc(&c->halo_list[swap], sizeof(int) * (size_t)cap)) != hipSuccess)
+      return hip_fail(e, "halo send list");
+    c->halo_cap[swap] = cap;
+  }
+  if (total > 0)
+    hipLaunchKernelGGL(k_halo_compact, dim3((n + B - 1) / B), dim3(B), 0, c->stream, n, c->x, lo[0], lo[1], lo[2],
+                       hi[0], hi[1], hi[2], pos, c->halo_list[swap]);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "halo send list");
+  c->halo_count[swap] = total;
+  *count = total;
+  return MDX_OK;
+}
+
+int mdx_halo_pack(mdx_ctx *c, int swap, double *buf) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (swap < 0 || swap >= MDX_HALO_SWAPS) {
+    set_error("mdx_halo_pack: swap in [0, 27) required");
+    return MDX_ERR_ARG;
+  }
+  const int count = c->halo_count[swap];
+  if (count == 0) return MDX_OK;
+  if (!buf || !device_range(buf, sizeof(double) * 3 * (size_t)count, c->device)) {
+    set_error("mdx_halo_pack: buf must be device memory of the context's GPU holding 3 doubles per listed atom");
+    return MDX_ERR_ARG;
+  }
+  prof_begin(c, MDX_K_HALO);
+  const int B = 256;
+  hipLaunchKernelGGL(k_halo_pack, dim3((count + B - 1) / B), dim3(B), 0, c->stream, count, c->halo_list[swap], c->x,
+                     c->halo_shift[swap][0], c->halo_shift[swap][1], c->halo_shift[swap][2], buf);
+  prof_end(c);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "halo pack");
+}
+
+int mdx_halo_unpack(mdx_ctx *c, int first, int count, const double *buf) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (first < 0 || count < 0 || (long)first + count > c->n) {
+    set_error("mdx_halo_unpack: atoms [first, first + count) must exist");
+    return MDX_ERR_ARG;
+  }
+  if (count == 0) return MDX_OK;
+  if (!buf || !device_range(buf, sizeof(double) * 3 * (size_t)count, c->device)) {
+    set_error("mdx_halo_unpack: buf must be device memory of the context's GPU holding 3 doubles per atom");
+    return MDX_ERR_ARG;
+  }
+  prof_begin(c, MDX_K_HALO);
+  const int B = 256;
+  hipLaunchKernelGGL(k_halo_unpack, dim3((count + B - 1) / B), dim3(B), 0, c->stream, count, first, c->inv, buf, c->x,
+                     c->qpos ? c->xq : nullptr, c->qs);
+  prof_end(c);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? MDX_OK : hip_fail(e, "halo unpack");
+}
+
+}  // extern "C"

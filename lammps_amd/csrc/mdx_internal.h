@@ -30,6 +30,7 @@
 #define MDX_NT1 (MDX_MAXT + 1)
 #define MDX_IDX_BITS 27
 #define MDX_IDX_MASK ((1 << MDX_IDX_BITS) - 1)
+#define MDX_HALO_SWAPS 27  // ghost-shell send lists per context (26 neighbour regions of a sub-box, + 1)
 #define MDX_NOSHIFT 13
 #define MDX_IMGMAX 512
 #define MDX_IMGMASK 1023
@@ -356,6 +357,15 @@ struct mdx_ctx {
   // overflow, [6] brick count scratch, [8..9] 64-bit neighbour total
   int *nbr = nullptr, *numneigh = nullptr, *dflags = nullptr;
   unsigned short *nbr16 = nullptr;  // brick mode: LDS slot numbers, [npad][cap]
+  // ghost-shell exchange (halo.hip): per swap, the device indices of the
+  // atoms to send (valid until the next list build) and their image shift
+  int *halo_list[MDX_HALO_SWAPS] = {};
+  int halo_cap[MDX_HALO_SWAPS] = {}, halo_count[MDX_HALO_SWAPS] = {};
+  double halo_shift[MDX_HALO_SWAPS][3] = {};
+  int *halo_flag = nullptr;  // [npad + 1] selection flags, then their exclusive scan
+  int halo_flag_cap = 0;
+  void *halo_scan_tmp = nullptr;
+  size_t halo_scan_bytes = 0;
   // mask lists: one bit per stencil candidate of each atom, in the build's
   // enumeration order, [npad][mwords] (mwords a multiple of 4)
   unsigned *nmask = nullptr;
@@ -425,6 +435,7 @@ inline int xcd_grid(long blocks) { return (int)(((blocks + kXCD - 1) / kXCD) * k
 
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
+void halo_free(mdx_ctx *c);
 
 // kernel timing with HIP events (no-ops unless profiling is on)
 void prof_begin(mdx_ctx *c, int kind);

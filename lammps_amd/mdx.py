@@ -51,7 +51,7 @@ class Stats(ctypes.Structure):
                 ("kernel_ms", ctypes.c_double * 8), ("kernel_launches", ctypes.c_long * 8)]
 
 
-K_BUILD, K_LJ, K_EAM_RHO, K_EAM_FORCE, K_NVE_INITIAL, K_NVE_FINAL, K_FUSED = range(7)
+K_BUILD, K_LJ, K_EAM_RHO, K_EAM_FORCE, K_NVE_INITIAL, K_NVE_FINAL, K_FUSED, K_HALO = range(8)
 
 
 class MdxError(RuntimeError):
@@ -101,6 +101,9 @@ def lib():
             "mdx_set_position_format": [_p, _i],
             "mdx_get_launch": [_p, _ip, _ip, _ip],
             "mdx_autotune": [_p, _i],
+            "mdx_halo_sendlist": [_p, _i, _dp, _dp, _dp, _ip],
+            "mdx_halo_pack": [_p, _i, ctypes.c_void_p],
+            "mdx_halo_unpack": [_p, _i, _i, ctypes.c_void_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -233,6 +236,23 @@ class Context:
     def position_format(self, fmt):
         """POS_Q32 (default) or POS_FP64 neighbour positions in the mixed gathers."""
         self._ck(self._L.mdx_set_position_format(self.h, fmt))
+
+    def halo_sendlist(self, swap, lo, hi, shift):
+        """Atoms with lo <= x < hi as ghost-shell send list `swap`, carrying the
+        image shift `shift`; returns their count."""
+        n = ctypes.c_int(0)
+        lo = np.asarray(lo, dtype=np.float64); hi = np.asarray(hi, dtype=np.float64)
+        sh = np.asarray(shift, dtype=np.float64)
+        self._ck(self._L.mdx_halo_sendlist(self.h, swap, _ptr(lo), _ptr(hi), _ptr(sh), ctypes.byref(n)))
+        return n.value
+
+    def halo_pack(self, swap, dev_ptr):
+        """x + shift of send list `swap` into device memory at dev_ptr (3 doubles per atom)."""
+        self._ck(self._L.mdx_halo_pack(self.h, swap, ctypes.c_void_p(dev_ptr)))
+
+    def halo_unpack(self, first, count, dev_ptr):
+        """Positions at device address dev_ptr into atoms first .. first+count-1."""
+        self._ck(self._L.mdx_halo_unpack(self.h, first, count, ctypes.c_void_p(dev_ptr)))
 
     def neigh_modify(self, skin, every=1, delay=10, check=True, one=2000):
         self._ck(self._L.mdx_neigh_modify(self.h, skin, every, delay, 1 if check else 0, one))

@@ -1,0 +1,138 @@
+"""Ghost-shell exchange of a spatially decomposed run on one GPU's share:
+send lists, pack with the image shift and unpack (mdx_halo_*; the
+reference's CommBrick::borders / forward_comm, src/comm_brick.cpp:398-460,
+696-864, and AtomVecAtomic::pack_comm with pbc, src/atom_vec_atomic.cpp).
+
+A periodic system G plays every neighbouring rank.  One octant of its box
+becomes a sub-domain D: the octant's atoms are D's owned atoms, and the atoms
+of G within rc+skin outside the octant, packed from G with their image shift
+and unpacked into D, are D's ghosts.  D's box is closed (no periodic images),
+so every neighbour of an owned atom that lies across a face reaches it only
+as a ghost.
+
+Bars (stated here, in the test):
+  * the packed ghost positions equal x + shift bit for bit (the reference's
+    ghost coordinate);
+  * forces and per-atom energies of D's owned atoms equal G's for the same
+    atoms: FP64 max_rel <= 1e-12 (summation order only), mixed <= 1e-5
+    against G in FP64;
+  * bad arguments (host buffer, out-of-range atoms or swap) are errors that
+    launch nothing.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+A_LJ = (4.0 / 0.8442) ** (1.0 / 3.0)
+CUT = 2.5 + 0.3  # neighbour cutoff: the ghost shell width
+
+
+def perturbed_lattice(ncell, seed):
+    from lammps_amd import mdx
+    _, hi, x = mdx.lattice("fcc", 0.8442, "lj", (0, 0, 0), (ncell,) * 3)
+    L = hi[0]
+    rng = np.random.default_rng(seed)
+    x = np.mod(x + rng.uniform(-0.12, 0.12, x.shape), L)
+    return x, L
+
+
+def lj_ctx(lo, hi, periodic, x, prec):
+    from lammps_amd import mdx
+    ctx = mdx.Context(0)
+    ctx.set_box(lo, hi, periodic)
+    ctx.set_atoms(x, np.ones(len(x), dtype=np.int32), np.array([0.0, 1.0]), tags=np.arange(1, len(x) + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    ctx.precision(mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    ctx.neigh_modify(0.3)
+    return ctx
+
+
+def octant_regions(L):
+    """the 26 regions of G's box whose atoms (with the given image shift) form
+    the ghost shell of the octant [0, L/2)^3"""
+    h = L / 2.0
+    regions = []
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for d in (-1, 0, 1):
+                if (a, b, d) == (0, 0, 0):
+                    continue
+                lo, hi, sh = [], [], []
+                for side in (a, b, d):
+                    if side < 0:
+                        lo.append(L - CUT); hi.append(L); sh.append(-L)
+                    elif side == 0:
+                        lo.append(0.0); hi.append(h); sh.append(0.0)
+                    else:
+                        lo.append(h); hi.append(h + CUT); sh.append(0.0)
+                regions.append((lo, hi, sh))
+    return regions
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
+    import torch
+    x, L = perturbed_lattice(12, seed=5)
+    n = len(x)
+    G = lj_ctx((0, 0, 0), (L, L, L), (1, 1, 1), x, "fp64")
+    ref = G.compute(1, 1, peratom=True)
+    xg = G.get_atoms()[0]  # positions as G holds them after its build (wrapped into the box)
+    h = L / 2.0
+    owned = np.where(np.all((xg >= 0.0) & (xg < h), axis=1))[0]
+    regions = octant_regions(L)
+    counts = [G.halo_sendlist(k, lo, hi, sh) for k, (lo, hi, sh) in enumerate(regions)]
+    ng = sum(counts)
+    expect = []
+    for (lo, hi, sh) in regions:
+        sel = np.all((xg >= lo) & (xg < hi), axis=1)
+        expect.append(xg[sel] + np.array(sh))
+    expect = np.concatenate(expect)
+    assert len(expect) == ng and ng > 0
+    buf = torch.zeros(3 * ng, dtype=torch.float64, device="cuda")
+    off = 0
+    for k, c in enumerate(counts):
+        G.halo_pack(k, buf.data_ptr() + 8 * 3 * off)
+        off += c
+    torch.cuda.synchronize()
+    got = buf.cpu().numpy().reshape(-1, 3)
+    # the same rows bit for bit (pack order is G's device order)
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    assert np.array_equal(key(got), key(expect))
+    # D: owned atoms first, then placeholders for the ghosts, filled by unpack
+    lo_d, hi_d = (-CUT - 0.5,) * 3, (h + CUT + 0.5,) * 3
+    place = np.full((ng, 3), h / 2.0) + np.linspace(0.0, 1.0, ng)[:, None] * 0.01
+    D = lj_ctx(lo_d, hi_d, (0, 0, 0), np.vstack([xg[owned], place]), prec)
+    D.halo_unpack(len(owned), ng, buf.data_ptr())
+    out = D.compute(1, 1, peratom=True)
+    xd = D.get_atoms()[0]
+    assert np.array_equal(xd[len(owned):], got)
+    fo, fr = out["f"][: len(owned)], ref["f"][owned]
+    e = np.linalg.norm(fo - fr, axis=1)
+    scale = np.sqrt(np.mean(np.sum(fr * fr, axis=1)))
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert e.max() / scale <= tol
+    eo, er = out["eatom"][: len(owned)], ref["eatom"][owned]
+    assert np.max(np.abs(eo - er)) <= tol * np.max(np.abs(er))
+    D.close()
+    G.close()
+
+
+def test_halo_argument_errors_launch_nothing():
+    from lammps_amd import mdx
+    x, L = perturbed_lattice(6, seed=2)
+    G = lj_ctx((0, 0, 0), (L, L, L), (1, 1, 1), x, "fp64")
+    G.neigh_build()
+    c = G.halo_sendlist(0, (0, 0, 0), (L / 2, L / 2, L / 2), (0, 0, 0))
+    assert c > 0
+    host = np.zeros(3 * c)
+    with pytest.raises(mdx.MdxError) as e:
+        G.halo_pack(0, host.ctypes.data)
+    assert e.value.code == mdx.ERR_ARG and "device memory" in str(e.value)
+    with pytest.raises(mdx.MdxError) as e:
+        G.halo_sendlist(27, (0, 0, 0), (1, 1, 1), (0, 0, 0))
+    assert e.value.code == mdx.ERR_ARG
+    with pytest.raises(mdx.MdxError) as e:
+        G.halo_unpack(len(x) - 1, 2, host.ctypes.data)
+    assert e.value.code == mdx.ERR_ARG and "must exist" in str(e.value)
+    G.close()
