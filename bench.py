@@ -205,6 +205,60 @@ def cpu_baseline_and_errors(kind, inp, ctxs, sample_steps):
     return out, base, t_force
 
 
+def halo_share(G, L, reps=20):
+    """One GPU's share of the 2.048 M-atom LJ run split 2x2x2 over eight GPUs:
+    the octant's 256 k owned atoms in their own context with the ghost shell
+    (rc + skin wide, about 70 k atoms) packed from the full system, which
+    stands in for the seven neighbouring ranks, and unpacked into the octant.
+    Device times from the library's HIP events: the pack of one exchange (the
+    26 swaps in one launch), the unpack, and the octant's force kernel; the send lists (built once per
+    reneighbouring) are timed on the host."""
+    import torch
+    from lammps_amd import mdx
+    cut = 2.5 + 0.3
+    G.neigh_build()
+    xg = G.get_atoms()[0]
+    regions = mdx.octant_shell_regions(L, cut)
+    t0 = time.perf_counter()
+    counts = [G.halo_sendlist(k, lo, hi, sh) for k, (lo, hi, sh) in enumerate(regions)]
+    t_list = (time.perf_counter() - t0) * 1e3
+    ng = sum(counts)
+    buf = torch.empty(3 * ng, dtype=torch.float64, device="cuda")
+    G.profiling(True)
+    for _ in range(reps):
+        G.halo_pack(-1, buf.data_ptr())  # the 26 swaps in one launch, in swap order
+    st = G.stats()
+    G.profiling(False)
+    pack_ms = st["kernel_ms"][mdx.K_HALO] / reps
+    h = L / 2.0
+    owned = np.where(np.all((xg >= 0.0) & (xg < h), axis=1))[0]
+    place = np.full((ng, 3), h / 2.0)
+    D = mdx.Context(0)
+    D.set_box((-cut - 0.5,) * 3, (h + cut + 0.5,) * 3, (0, 0, 0))
+    D.set_atoms(np.vstack([xg[owned], place]), np.ones(len(owned) + ng, dtype=np.int32), np.array([0.0, 1.0]))
+    D.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    D.precision(mdx.MIXED)
+    D.neigh_modify(0.3)
+    D.profiling(True)
+    for _ in range(reps):
+        D.halo_unpack(len(owned), ng, buf.data_ptr())
+    D.neigh_build()
+    D.set_launch(0, 4)
+    for _ in range(reps):
+        fo = D.compute(0, 0)["f"][: len(owned)]
+    st = D.stats()
+    D.profiling(False)
+    fr = G.compute(0, 0)["f"][owned]
+    D.close()
+    err = float(np.max(np.linalg.norm(fo - fr, axis=1)) / np.sqrt(np.mean(np.sum(fr * fr, axis=1))))
+    unpack_ms = st["kernel_ms"][mdx.K_HALO] / reps
+    return dict(owned=int(len(owned)), ghosts=int(ng), ghost_bytes=24 * ng, send_lists_ms=t_list,
+                pack_ms=pack_ms, unpack_ms=unpack_ms,
+                pack_gbs=(24.0 * ng * 2 + 4.0 * ng) / (pack_ms * 1e-3) / 1e9 if pack_ms > 0 else None,
+                force_kernel_ms=st["kernel_ms"][mdx.K_LJ] / max(1, st["kernel_launches"][mdx.K_LJ]),
+                force_max_rel_vs_full_system=err)
+
+
 def fit_lj_ka(reps=10, check_oracle=True):
     """Differentiable lj/cut force matching on the KA mixture: 64 snapshots of
     8,000 atoms (device MD preparation), forward forces in mixed precision,
@@ -490,6 +544,11 @@ def main():
         log("  clusters forward %.2f ms, adjoint %.2f ms (mixed), grad vs FD %.2e"
             % (replicas["mixed"]["ms_forward"], replicas["mixed"]["ms_adjoint"], replicas["mixed_grad_vs_fd_max_rel"]))
 
+    log("one GPU's share of a 2x2x2 split: octant + ghost shell, pack / unpack")
+    share = halo_share(res["mixed"]["ctx"], lj["hi"][0] - lj["lo"][0])
+    log("  %d owned + %d ghosts: pack %.3f ms, unpack %.3f ms, force kernel %.3f ms"
+        % (share["owned"], share["ghosts"], share["pack_ms"], share["unpack_ms"], share["force_kernel_ms"]))
+
     out = {
         "metric": METRIC,
         "value": res["mixed"]["value"],
@@ -516,6 +575,7 @@ def main():
         "eam_256k": eam,
         "force_matching_ka": fit_ka,
         "eam_cluster_replicas": replicas,
+        "sharded_share": share,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -225,7 +225,9 @@ int mdx_set_position_format(mdx_ctx *ctx, int format);
    selected.  The list stays valid until the next list build of this
    context; rebuild it after each one, as borders does.
    mdx_halo_pack: writes the listed atoms' x + shift, 3 doubles each, into
-   buf (device memory of this context's GPU).
+   buf (device memory of this context's GPU).  swap = -1 packs every swap
+   in one launch, swap 0's atoms first, then swap 1's, ... (each swap's rows
+   start at the sum of the earlier swaps' counts).
    mdx_halo_unpack: writes count received positions (3 doubles each, device
    memory) into this context's atoms first .. first+count-1 (caller order):
    the ghost atoms, which the caller appended after the owned ones.  Ghost

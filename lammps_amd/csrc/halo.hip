@@ -53,6 +53,24 @@ __global__ void k_halo_pack(int count, const int *__restrict__ list, const doubl
   buf[3 * k + 2] = p.z + s2;
 }
 
+// every swap in one launch: swap s fills buf rows off[s] .. off[s+1]-1
+struct HaloAll {
+  const int *list[MDX_HALO_SWAPS];
+  double shift[MDX_HALO_SWAPS][3];
+  int off[MDX_HALO_SWAPS + 1];
+};
+
+__global__ void k_halo_pack_all(HaloAll h, const double4 *__restrict__ x, double *__restrict__ buf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= h.off[MDX_HALO_SWAPS]) return;
+  int s = 0;
+  while (h.off[s + 1] <= k) s++;  // 27 entries: a short scan of kernel arguments
+  const double4 p = x[h.list[s][k - h.off[s]]];
+  buf[3 * k] = p.x + h.shift[s][0];
+  buf[3 * k + 1] = p.y + h.shift[s][1];
+  buf[3 * k + 2] = p.z + h.shift[s][2];
+}
+
 __global__ void k_halo_unpack(int count, int first, const int *__restrict__ inv, const double *__restrict__ buf,
                               double4 *__restrict__ x, int4 *__restrict__ xq, QScale qs) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,9 +195,30 @@ int mdx_halo_sendlist(mdx_ctx *c, int swap, const double lo[3], const double hi[
 int mdx_halo_pack(mdx_ctx *c, int swap, double *buf) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (swap < 0 || swap >= MDX_HALO_SWAPS) {
-    set_error("mdx_halo_pack: swap in [0, 27) required");
+  if (swap < -1 || swap >= MDX_HALO_SWAPS) {
+    set_error("mdx_halo_pack: swap in [0, 27), or -1 for all, required");
     return MDX_ERR_ARG;
+  }
+  if (swap == -1) {
+    HaloAll h{};
+    h.off[0] = 0;
+    for (int s = 0; s < MDX_HALO_SWAPS; s++) {
+      h.list[s] = c->halo_list[s];
+      for (int d = 0; d < 3; d++) h.shift[s][d] = c->halo_shift[s][d];
+      h.off[s + 1] = h.off[s] + c->halo_count[s];
+    }
+    const int total = h.off[MDX_HALO_SWAPS];
+    if (total == 0) return MDX_OK;
+    if (!buf || !device_range(buf, sizeof(double) * 3 * (size_t)total, c->device)) {
+      set_error("mdx_halo_pack: buf must be device memory of the context's GPU holding 3 doubles per listed atom");
+      return MDX_ERR_ARG;
+    }
+    prof_begin(c, MDX_K_HALO);
+    const int B = 256;
+    hipLaunchKernelGGL(k_halo_pack_all, dim3((total + B - 1) / B), dim3(B), 0, c->stream, h, c->x, buf);
+    prof_end(c);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? MDX_OK : hip_fail(e, "halo pack");
   }
   const int count = c->halo_count[swap];
   if (count == 0) return MDX_OK;

@@ -247,7 +247,8 @@ class Context:
         return n.value
 
     def halo_pack(self, swap, dev_ptr):
-        """x + shift of send list `swap` into device memory at dev_ptr (3 doubles per atom)."""
+        """x + shift of send list `swap` (-1: all, in swap order) into device
+        memory at dev_ptr (3 doubles per atom)."""
         self._ck(self._L.mdx_halo_pack(self.h, swap, ctypes.c_void_p(dev_ptr)))
 
     def halo_unpack(self, first, count, dev_ptr):
@@ -459,6 +460,34 @@ def eam_context_tables(ctx, path, ntypes=1):
                  t["rhor_spline"], t["z2r_spline"], np.zeros(nt1, dtype=np.int32),
                  np.zeros(nt1 * nt1, dtype=np.int32), np.zeros(nt1 * nt1, dtype=np.int32))
     return t
+
+
+def octant_shell_regions(L, cut):
+    """Send regions of a periodic cubic box [0, L)^3 for the ghost shell of its
+    octant [0, L/2)^3 (one GPU's share when the box is split 2x2x2): for each
+    of the 26 neighbouring directions, (lo, hi, shift) such that the atoms
+    with lo <= x < hi, shifted by `shift`, are the ghosts within `cut` of that
+    face, edge or corner (CommBrick::borders' selection, done in one pass per
+    direction).  Requires L > 4 cut."""
+    h = L / 2.0
+    if not L > 4.0 * cut:
+        raise ValueError("octant shell regions need L > 4 cut")
+    regions = []
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for d in (-1, 0, 1):
+                if (a, b, d) == (0, 0, 0):
+                    continue
+                lo, hi, sh = [], [], []
+                for side in (a, b, d):
+                    if side < 0:
+                        lo.append(L - cut); hi.append(L); sh.append(-L)
+                    elif side == 0:
+                        lo.append(0.0); hi.append(h); sh.append(0.0)
+                    else:
+                        lo.append(h); hi.append(h + cut); sh.append(0.0)
+                regions.append((lo, hi, sh))
+    return regions
 
 
 def thermo_columns(raw, natoms, prd, units="lj"):

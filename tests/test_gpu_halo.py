@@ -48,28 +48,6 @@ def lj_ctx(lo, hi, periodic, x, prec):
     return ctx
 
 
-def octant_regions(L):
-    """the 26 regions of G's box whose atoms (with the given image shift) form
-    the ghost shell of the octant [0, L/2)^3"""
-    h = L / 2.0
-    regions = []
-    for a in (-1, 0, 1):
-        for b in (-1, 0, 1):
-            for d in (-1, 0, 1):
-                if (a, b, d) == (0, 0, 0):
-                    continue
-                lo, hi, sh = [], [], []
-                for side in (a, b, d):
-                    if side < 0:
-                        lo.append(L - CUT); hi.append(L); sh.append(-L)
-                    elif side == 0:
-                        lo.append(0.0); hi.append(h); sh.append(0.0)
-                    else:
-                        lo.append(h); hi.append(h + CUT); sh.append(0.0)
-                regions.append((lo, hi, sh))
-    return regions
-
-
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
     import torch
@@ -80,7 +58,8 @@ def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
     xg = G.get_atoms()[0]  # positions as G holds them after its build (wrapped into the box)
     h = L / 2.0
     owned = np.where(np.all((xg >= 0.0) & (xg < h), axis=1))[0]
-    regions = octant_regions(L)
+    from lammps_amd import mdx
+    regions = mdx.octant_shell_regions(L, CUT)
     counts = [G.halo_sendlist(k, lo, hi, sh) for k, (lo, hi, sh) in enumerate(regions)]
     ng = sum(counts)
     expect = []
@@ -96,6 +75,11 @@ def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
         off += c
     torch.cuda.synchronize()
     got = buf.cpu().numpy().reshape(-1, 3)
+    # all swaps in one launch: the same rows in the same order
+    buf_all = torch.zeros(3 * ng, dtype=torch.float64, device="cuda")
+    G.halo_pack(-1, buf_all.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(buf_all.cpu().numpy().reshape(-1, 3), got)
     # the same rows bit for bit (pack order is G's device order)
     key = lambda a: a[np.lexsort(a.T[::-1])]
     assert np.array_equal(key(got), key(expect))
@@ -128,6 +112,9 @@ def test_halo_argument_errors_launch_nothing():
     host = np.zeros(3 * c)
     with pytest.raises(mdx.MdxError) as e:
         G.halo_pack(0, host.ctypes.data)
+    assert e.value.code == mdx.ERR_ARG and "device memory" in str(e.value)
+    with pytest.raises(mdx.MdxError) as e:
+        G.halo_pack(-1, host.ctypes.data)
     assert e.value.code == mdx.ERR_ARG and "device memory" in str(e.value)
     with pytest.raises(mdx.MdxError) as e:
         G.halo_sendlist(27, (0, 0, 0), (1, 1, 1), (0, 0, 0))
