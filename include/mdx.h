@@ -227,7 +227,9 @@ int mdx_set_position_format(mdx_ctx *ctx, int format);
    mdx_halo_pack: writes the listed atoms' x + shift, 3 doubles each, into
    buf (device memory of this context's GPU).  swap = -1 packs every swap
    in one launch, swap 0's atoms first, then swap 1's, ... (each swap's rows
-   start at the sum of the earlier swaps' counts).
+   start at the sum of the earlier swaps' counts).  Pack and unpack run on
+   the context's stream and return when done; work that the caller queued
+   on other streams for the same buffers must be complete before the call.
    mdx_halo_unpack: writes count received positions (3 doubles each, device
    memory) into this context's atoms first .. first+count-1 (caller order):
    the ghost atoms, which the caller appended after the owned ones.  Ghost
@@ -238,6 +240,11 @@ int mdx_set_position_format(mdx_ctx *ctx, int format);
    the force kernel) is not part of this library. */
 int mdx_halo_sendlist(mdx_ctx *ctx, int swap, const double lo[3], const double hi[3], const double shift[3],
                       int *count);
+/* Send lists 0 .. nswap-1 in one pass over the atoms (regions may overlap):
+   lo, hi, shift are [nswap][3], counts[nswap] gets each list's size.  The
+   lists equal what nswap calls of mdx_halo_sendlist would build. */
+int mdx_halo_sendlists(mdx_ctx *ctx, int nswap, const double *lo, const double *hi, const double *shift,
+                       int *counts);
 int mdx_halo_pack(mdx_ctx *ctx, int swap, double *buf);
 int mdx_halo_unpack(mdx_ctx *ctx, int first, int count, const double *buf);
 

@@ -13,6 +13,9 @@
 // received positions into consecutive caller-order atoms of the receiving
 // context.  Between ranks the packed buffer travels by RCCL send/recv; this
 // file does not move data between devices.
+#include <algorithm>
+#include <vector>
+
 #include <hipcub/hipcub.hpp>
 
 #include "mdx_internal.h"
@@ -51,6 +54,68 @@ __global__ void k_halo_pack(int count, const int *__restrict__ list, const doubl
   buf[3 * k] = p.x + s0;
   buf[3 * k + 1] = p.y + s1;
   buf[3 * k + 2] = p.z + s2;
+}
+
+// Send lists of many regions in one pass over the atoms.  Regions may
+// overlap (an atom can go to several neighbours).  Pass 1 counts, per block
+// of 256 atoms and region, the members; a scan over the (region, block)
+// counts, region-major, gives each block its first row in each region's
+// list; pass 2 recomputes the memberships and writes each member at that
+// row plus its rank among the block's members (wavefront ballots), so every
+// list is in device order, as the one-region compaction writes it.
+struct HaloRegions {
+  double lo[MDX_HALO_SWAPS][3], hi[MDX_HALO_SWAPS][3];
+  int *list[MDX_HALO_SWAPS];
+  int nswap;
+};
+
+__device__ __forceinline__ bool in_region(const HaloRegions &r, int s, const double4 &p) {
+  return p.x >= r.lo[s][0] && p.x < r.hi[s][0] && p.y >= r.lo[s][1] && p.y < r.hi[s][1] && p.z >= r.lo[s][2] &&
+         p.z < r.hi[s][2];
+}
+
+__global__ __launch_bounds__(256) void k_halo_count(int n, HaloRegions r, const double4 *__restrict__ x,
+                                                    int *__restrict__ counts) {
+  __shared__ int s_cnt[MDX_HALO_SWAPS];
+  if (threadIdx.x < MDX_HALO_SWAPS) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double4 p = x[i];
+    for (int s = 0; s < r.nswap; s++)
+      if (in_region(r, s, p)) atomicAdd(&s_cnt[s], 1);
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < r.nswap) counts[threadIdx.x * gridDim.x + blockIdx.x] = s_cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_halo_fill(int n, HaloRegions r, const double4 *__restrict__ x,
+                                                   const int *__restrict__ start) {
+  __shared__ int s_wave[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (i < n) p = x[i];
+  for (int s = 0; s < r.nswap; s++) {  // uniform trip count: every thread reaches the barriers
+    const bool m = i < n && in_region(r, s, p);
+    const unsigned long long b = __ballot(m);
+    if (lane == 0) s_wave[wave] = __popcll(b);
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; w++) before += s_wave[w];
+    if (m) {
+      const int row = start[s * gridDim.x + blockIdx.x] + before + __popcll(b & ((1ull << lane) - 1ull));
+      r.list[s][row - start[s * gridDim.x]] = i;
+    }
+    __syncthreads();
+  }
+}
+
+// the scan rows where each region's list starts (and the total), gathered
+// for one copy to the host
+__global__ void k_halo_segments(int nswap, int nblock, const int *__restrict__ start, int *__restrict__ seg) {
+  const int s = threadIdx.x;
+  if (s <= nswap) seg[s] = start[(size_t)s * nblock];
 }
 
 // every swap in one launch: swap s fills buf rows off[s] .. off[s+1]-1
@@ -189,6 +254,82 @@ int mdx_halo_sendlist(mdx_ctx *c, int swap, const double lo[3], const double hi[
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "halo send list");
   c->halo_count[swap] = total;
   *count = total;
+  return MDX_OK;
+}
+
+int mdx_halo_sendlists(mdx_ctx *c, int nswap, const double *lo, const double *hi, const double *shift, int *counts) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (nswap < 1 || nswap > MDX_HALO_SWAPS || !lo || !hi || !shift || !counts) {
+    set_error("mdx_halo_sendlists: 1 <= nswap <= 27 and non-null lo, hi, shift, counts required");
+    return MDX_ERR_ARG;
+  }
+  const int n = c->n;
+  for (int s = 0; s < nswap; s++) {
+    for (int d = 0; d < 3; d++) c->halo_shift[s][d] = shift[3 * s + d];
+    c->halo_count[s] = 0;
+    counts[s] = 0;
+  }
+  if (n == 0) return MDX_OK;
+  const int B = 256, G = (n + B - 1) / B;
+  const size_t cells = (size_t)nswap * G + 1;  // + 1: the scan's total
+  hipError_t e;
+  if (cells > (size_t)c->halo_flag_cap) {
+    if (c->halo_flag) (void)hipFree(c->halo_flag);
+    c->halo_flag = nullptr;
+    c->halo_flag_cap = 0;
+    const int cap = (int)std::max<size_t>(cells, (size_t)n + 1);
+    if ((e = hipMalloc(&c->halo_flag, sizeof(int) * 2 * (size_t)cap)) != hipSuccess) return hip_fail(e, "halo counts");
+    c->halo_flag_cap = cap;
+  }
+  int *cnt = c->halo_flag, *start = c->halo_flag + c->halo_flag_cap;
+  size_t need = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, start, (int)cells, c->stream);
+  if (need > c->halo_scan_bytes) {
+    if (c->halo_scan_tmp) (void)hipFree(c->halo_scan_tmp);
+    c->halo_scan_tmp = nullptr;
+    c->halo_scan_bytes = 0;
+    if ((e = hipMalloc(&c->halo_scan_tmp, need)) != hipSuccess) return hip_fail(e, "halo scan");
+    c->halo_scan_bytes = need;
+  }
+  HaloRegions r{};
+  r.nswap = nswap;
+  for (int s = 0; s < nswap; s++)
+    for (int d = 0; d < 3; d++) {
+      r.lo[s][d] = lo[3 * s + d];
+      r.hi[s][d] = hi[3 * s + d];
+    }
+  if ((e = hipMemsetAsync(cnt + cells - 1, 0, sizeof(int), c->stream)) != hipSuccess) return hip_fail(e, "halo counts");
+  hipLaunchKernelGGL(k_halo_count, dim3(G), dim3(B), 0, c->stream, n, r, c->x, cnt);
+  if ((e = hipcub::DeviceScan::ExclusiveSum(c->halo_scan_tmp, need, cnt, start, (int)cells, c->stream)) != hipSuccess)
+    return hip_fail(e, "halo scan");
+  // region s owns rows start[s*G] .. start[(s+1)*G] - 1 of the scan; the
+  // counts' cells are free now and take the gathered segment starts
+  std::vector<int> seg(nswap + 1);
+  hipLaunchKernelGGL(k_halo_segments, dim3(1), dim3(64), 0, c->stream, nswap, G, start, cnt);
+  if ((e = hipMemcpyAsync(seg.data(), cnt, sizeof(int) * (nswap + 1), hipMemcpyDeviceToHost, c->stream)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return hip_fail(e, "halo counts");
+  for (int s = 0; s < nswap; s++) {
+    const int total = seg[s + 1] - seg[s];
+    if (total > c->halo_cap[s]) {
+      if (c->halo_list[s]) (void)hipFree(c->halo_list[s]);
+      c->halo_list[s] = nullptr;
+      c->halo_cap[s] = 0;
+      const int cap = total + total / 8 + 64;
+      if ((e = hipMalloc(&c->halo_list[s], sizeof(int) * (size_t)cap)) != hipSuccess)
+        return hip_fail(e, "halo send list");
+      c->halo_cap[s] = cap;
+    }
+    r.list[s] = c->halo_list[s];
+  }
+  hipLaunchKernelGGL(k_halo_fill, dim3(G), dim3(B), 0, c->stream, n, r, c->x, start);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "halo send lists");
+  for (int s = 0; s < nswap; s++) {
+    c->halo_count[s] = seg[s + 1] - seg[s];
+    counts[s] = c->halo_count[s];
+  }
   return MDX_OK;
 }
 

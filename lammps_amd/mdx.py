@@ -102,6 +102,7 @@ def lib():
             "mdx_get_launch": [_p, _ip, _ip, _ip],
             "mdx_autotune": [_p, _i],
             "mdx_halo_sendlist": [_p, _i, _dp, _dp, _dp, _ip],
+            "mdx_halo_sendlists": [_p, _i, _dp, _dp, _dp, _ip],
             "mdx_halo_pack": [_p, _i, ctypes.c_void_p],
             "mdx_halo_unpack": [_p, _i, _i, ctypes.c_void_p],
         }
@@ -245,6 +246,15 @@ class Context:
         sh = np.asarray(shift, dtype=np.float64)
         self._ck(self._L.mdx_halo_sendlist(self.h, swap, _ptr(lo), _ptr(hi), _ptr(sh), ctypes.byref(n)))
         return n.value
+
+    def halo_sendlists(self, regions):
+        """Send lists 0 .. len(regions)-1 from [(lo, hi, shift), ...] in one pass; returns the counts."""
+        lo = np.ascontiguousarray([r[0] for r in regions], dtype=np.float64)
+        hi = np.ascontiguousarray([r[1] for r in regions], dtype=np.float64)
+        sh = np.ascontiguousarray([r[2] for r in regions], dtype=np.float64)
+        counts = np.zeros(len(regions), dtype=np.int32)
+        self._ck(self._L.mdx_halo_sendlists(self.h, len(regions), _ptr(lo), _ptr(hi), _ptr(sh), _ptr(counts, _ip)))
+        return counts.tolist()
 
     def halo_pack(self, swap, dev_ptr):
         """x + shift of send list `swap` (-1: all, in swap order) into device

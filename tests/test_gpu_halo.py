@@ -69,6 +69,7 @@ def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
     expect = np.concatenate(expect)
     assert len(expect) == ng and ng > 0
     buf = torch.zeros(3 * ng, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()  # torch fills on its own stream; the library runs on another
     off = 0
     for k, c in enumerate(counts):
         G.halo_pack(k, buf.data_ptr() + 8 * 3 * off)
@@ -77,6 +78,14 @@ def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
     got = buf.cpu().numpy().reshape(-1, 3)
     # all swaps in one launch: the same rows in the same order
     buf_all = torch.zeros(3 * ng, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    G.halo_pack(-1, buf_all.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(buf_all.cpu().numpy().reshape(-1, 3), got)
+    # all send lists in one pass: the same lists (the packed rows match in order)
+    assert G.halo_sendlists(regions) == counts
+    buf_all.zero_()
+    torch.cuda.synchronize()
     G.halo_pack(-1, buf_all.data_ptr())
     torch.cuda.synchronize()
     assert np.array_equal(buf_all.cpu().numpy().reshape(-1, 3), got)
@@ -99,6 +108,33 @@ def test_octant_with_packed_ghosts_reproduces_the_periodic_forces(prec):
     eo, er = out["eatom"][: len(owned)], ref["eatom"][owned]
     assert np.max(np.abs(eo - er)) <= tol * np.max(np.abs(er))
     D.close()
+    G.close()
+
+
+def test_overlapping_send_regions_in_one_pass():
+    """regions that overlap (an atom sent to several neighbours): the one-pass
+    lists equal the per-region ones"""
+    import torch
+    from lammps_amd import mdx
+    x, L = perturbed_lattice(8, seed=9)
+    G = lj_ctx((0, 0, 0), (L, L, L), (1, 1, 1), x, "fp64")
+    G.neigh_build()
+    rng = np.random.default_rng(4)
+    regions = []
+    for _ in range(27):
+        a = rng.uniform(0, L, 3); b = rng.uniform(0, L, 3)
+        regions.append((np.minimum(a, b), np.maximum(a, b), rng.choice([-L, 0.0, L], 3)))
+    one = [G.halo_sendlist(k, lo, hi, sh) for k, (lo, hi, sh) in enumerate(regions)]
+    tot = sum(one)
+    b1 = torch.zeros(3 * max(tot, 1), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    G.halo_pack(-1, b1.data_ptr())
+    assert G.halo_sendlists(regions) == one
+    b2 = torch.zeros(3 * max(tot, 1), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    G.halo_pack(-1, b2.data_ptr())
+    torch.cuda.synchronize()
+    assert tot > len(x) // 4 and np.array_equal(b1.cpu().numpy(), b2.cpu().numpy())
     G.close()
 
 
