@@ -248,7 +248,7 @@ int mdx_destroy(mdx_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_atoms(c);
   void *ptrs[] = {c->dflags, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->eam_pk, c->eam_pk_f, c->bin_count, c->bin_start,
-                  c->sort_tmp, c->scan_tmp};
+                  c->scan_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   halo_free(c);

@@ -345,8 +345,6 @@ struct mdx_ctx {
   mdx::NeighArgs na{};
   unsigned *bkey = nullptr, *bkey_s = nullptr;
   int *bidx = nullptr, *bidx_s = nullptr, *bin_count = nullptr, *bin_start = nullptr;
-  void *sort_tmp = nullptr;
-  size_t sort_tmp_bytes = 0;
   void *scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   // neighbour settings and state

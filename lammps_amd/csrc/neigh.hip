@@ -606,7 +606,7 @@ template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
-  if (TPA == 1) {
+  if constexpr (TPA == 1) {
     QStep qs{{(float)c->qs.step[0], (float)c->qs.step[1], (float)c->qs.step[2]}};
     if (q_build_ok(c))
       hipLaunchKernelGGL(k_build_lane<true>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->xq, qs, c->bkey_s,
@@ -616,10 +616,10 @@ void launch_build(mdx_ctx *c) {
       hipLaunchKernelGGL(k_build_lane<false>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, nullptr, qs, c->bkey_s,
                          c->bin_start, c->nbr, c->numneigh, c->dflags,
                          reinterpret_cast<unsigned long long *>(c->dflags + 8));
-    return;
+  } else {
+    hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
+                       c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
   }
-  hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
-                     c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
 }
 
 }  // namespace
