@@ -361,7 +361,12 @@ __global__ MDX_BUILD_BOUNDS void k_build_full(NeighArgs a, const double4 *__rest
 // coordinate the binning used, in FP32: its rounding (below 1e-5 bin for
 // up to 10^4 bins per dimension) is dwarfed by the margins (1e-3 bin on
 // every gap and reach, 1e-4 relative on the squared cutoff).
-template <bool Q>
+//
+// ONE: a single atom type, so the cutoff and its band are loop-invariant
+// values instead of per-candidate table reads.  Accepted entries go through
+// a four-entry shift register (q0 oldest) that is stored as one 16-B write
+// whenever it holds a group of four.
+template <bool Q, bool ONE>
 __global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) void k_build_lane(NeighArgs a, const double4 *__restrict__ x,
                                                    const int4 *__restrict__ xq, QStep qs,
                                                    const unsigned *__restrict__ bkey,
@@ -386,7 +391,10 @@ __global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) v
   bin_coords(a, (int)b, ix, iy, iz);
   int *row = nbr + (size_t)i * a.maxneigh;  // maxneigh is a multiple of 4: rows are 16-B aligned
   int cnt = 0;
-  int q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // pending entries cnt&~3 .. cnt-1
+  int q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // the last four accepted entries, q3 the newest
+  const int t11 = a.ntypes1 + 1;       // (1, 1) in the type tables
+  const float c_lo = (ONE && Q) ? s_lo[Q ? t11 : 0] : 0.0f, c_hi = (ONE && Q) ? s_hi[Q ? t11 : 0] : 0.0f;
+  const double c_cut = ONE ? s_cut[t11] : 0.0;
   const float tx = static_cast<float>((xi.x - a.lo[0]) * a.bininv[0]),
               ty = static_cast<float>((xi.y - a.lo[1]) * a.bininv[1]),
               tz = static_cast<float>((xi.z - a.lo[2]) * a.bininv[2]);
@@ -474,33 +482,37 @@ __global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) v
               const float dy = static_cast<float>(static_cast<int>((unsigned)qi.y - (unsigned)qj[u].y)) * qs.s[1];
               const float dz = static_cast<float>(static_cast<int>((unsigned)qi.z - (unsigned)qj[u].z)) * qs.s[2];
               const float rq = dx * dx + dy * dy + dz * dz;
-              const int t = ti * a.ntypes1 + qj[u].w;
-              keep = valid && rq <= s_lo[t];
-              if (valid && !keep && rq <= s_hi[t]) keep = keep_exact(a, xi, x[j], shx, shy, shz, code, s_cut[t]);
+              if constexpr (ONE) {
+                keep = valid && rq <= c_lo;
+                if (valid && !keep && rq <= c_hi) keep = keep_exact(a, xi, x[j], shx, shy, shz, code, c_cut);
+              } else {
+                const int t = ti * a.ntypes1 + qj[u].w;
+                keep = valid && rq <= s_lo[t];
+                if (valid && !keep && rq <= s_hi[t]) keep = keep_exact(a, xi, x[j], shx, shy, shz, code, s_cut[t]);
+              }
             } else {
-              const int tj = static_cast<int>(xj[u].w);
-              keep = valid && keep_exact(a, xi, xj[u], shx, shy, shz, code, s_cut[ti * a.ntypes1 + tj]);
+              const double c = ONE ? c_cut : s_cut[ti * a.ntypes1 + static_cast<int>(xj[u].w)];
+              keep = valid && keep_exact(a, xi, xj[u], shx, shy, shz, code, c);
             }
             if (keep) {
-              const int e = j | (code << MDX_IDX_BITS);
-              const int slot = cnt & 3;
-              q0 = slot == 0 ? e : q0;
-              q1 = slot == 1 ? e : q1;
-              q2 = slot == 2 ? e : q2;
-              q3 = slot == 3 ? e : q3;
+              q0 = q1;
+              q1 = q2;
+              q2 = q3;
+              q3 = j | (code << MDX_IDX_BITS);
               cnt++;
-              if (slot == 3 && cnt <= a.maxneigh) *reinterpret_cast<int4 *>(row + cnt - 4) = make_int4(q0, q1, q2, q3);
+              if ((cnt & 3) == 0 && cnt <= a.maxneigh) *reinterpret_cast<int4 *>(row + cnt - 4) = make_int4(q0, q1, q2, q3);
             }
           }
         }
       }
     }
   }
-  // entries after the last full group of four
+  // entries after the last full group of four: the newest `tail` of the register
   const int tail = cnt & 3, t0 = cnt - tail;
-  if (tail > 0 && t0 < a.maxneigh) row[t0] = q0;
-  if (tail > 1 && t0 + 1 < a.maxneigh) row[t0 + 1] = q1;
-  if (tail > 2 && t0 + 2 < a.maxneigh) row[t0 + 2] = q2;
+  const int r0 = tail == 1 ? q3 : tail == 2 ? q2 : q1, r1 = tail == 2 ? q3 : q2;
+  if (tail > 0 && t0 < a.maxneigh) row[t0] = r0;
+  if (tail > 1 && t0 + 1 < a.maxneigh) row[t0 + 1] = r1;
+  if (tail > 2 && t0 + 2 < a.maxneigh) row[t0 + 2] = q3;
   numneigh[i] = cnt;
   atomicMax(&dflags[0], cnt);
   atomicAdd(nsum, (unsigned long long)cnt);
@@ -608,14 +620,16 @@ void launch_build(mdx_ctx *c) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   if constexpr (TPA == 1) {
     QStep qs{{(float)c->qs.step[0], (float)c->qs.step[1], (float)c->qs.step[2]}};
-    if (q_build_ok(c))
-      hipLaunchKernelGGL(k_build_lane<true>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->xq, qs, c->bkey_s,
-                         c->bin_start, c->nbr, c->numneigh, c->dflags,
-                         reinterpret_cast<unsigned long long *>(c->dflags + 8));
-    else
-      hipLaunchKernelGGL(k_build_lane<false>, dim3(G), dim3(B), 0, c->stream, c->na, c->x, nullptr, qs, c->bkey_s,
-                         c->bin_start, c->nbr, c->numneigh, c->dflags,
-                         reinterpret_cast<unsigned long long *>(c->dflags + 8));
+    unsigned long long *nsum = reinterpret_cast<unsigned long long *>(c->dflags + 8);
+    const bool q = q_build_ok(c), one = c->ntypes == 1;
+    auto go = [&](auto kernel, const int4 *xq) {
+      hipLaunchKernelGGL(kernel, dim3(G), dim3(B), 0, c->stream, c->na, c->x, xq, qs, c->bkey_s, c->bin_start, c->nbr,
+                         c->numneigh, c->dflags, nsum);
+    };
+    if (q && one) go(k_build_lane<true, true>, c->xq);
+    else if (q) go(k_build_lane<true, false>, c->xq);
+    else if (one) go(k_build_lane<false, true>, nullptr);
+    else go(k_build_lane<false, false>, nullptr);
   } else {
     hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
                        c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
