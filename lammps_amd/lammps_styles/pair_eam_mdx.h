@@ -24,6 +24,7 @@ class PairEAMMDX : public PairEAM {
   void compute(int, int);
   void init_style();
   double init_one(int, int);
+  mdx_ctx *context() { return ctx; }  // the device state run_style verlet/mdx drives
 
  private:
   mdx_ctx *ctx;

@@ -25,6 +25,7 @@ class PairLJCutMDX : public PairLJCut {
   void compute(int, int);
   void init_style();
   double init_one(int, int);
+  mdx_ctx *context() { return ctx; }  // the device state run_style verlet/mdx drives
 
  private:
   mdx_ctx *ctx;
