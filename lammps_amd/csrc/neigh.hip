@@ -150,12 +150,17 @@ __device__ __forceinline__ void stage_band(const NeighArgs &a, float *lo, float 
 
 // Domain::pbc + bin index + histogram; one thread per atom.  slot[i] is the
 // atom's arrival order in its bin (not deterministic; k_bin_rank orders
-// each bin by atom index).
+// each bin by atom index).  Atoms arrive nearly bin-sorted (the previous
+// build's order), so the lanes of a wavefront with equal bins form runs: one
+// returning atomic per run reserves the run's slots, and no two lanes of a
+// wavefront hit one counter.
 __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict__ image,
                            unsigned *__restrict__ key, int *__restrict__ slot, int *__restrict__ count,
                            int *__restrict__ dflags) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  unsigned b = 0xffffffffu;  // past the end: its own run, no atomic
+  if (i < a.n) {
   double4 p = x[i];
   double c[3] = {p.x, p.y, p.z};
   int img = image[i];
@@ -187,9 +192,21 @@ __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict
   p.z = c[2];
   x[i] = p;
   image[i] = (im[2] << 20) | (im[1] << 10) | im[0];
-  unsigned b = (unsigned)bin_index(a, ib[0], ib[1], ib[2]);
+  b = (unsigned)bin_index(a, ib[0], ib[1], ib[2]);
   key[i] = b;
-  slot[i] = atomicAdd(&count[b], 1);
+  }
+  // runs of equal bins across the wavefront's lanes (all 64 lanes take part)
+  const unsigned prev = __shfl_up(b, 1);
+  const bool head = lane == 0 || prev != b;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+  const int start = 63 - __clzll(heads & upto);
+  const unsigned long long above = heads & ~upto;
+  const int len = (above ? __ffsll((long long)above) - 1 : 64) - lane;  // run length at a head
+  int base = 0;
+  if (head && b != 0xffffffffu) base = atomicAdd(&count[b], len);
+  base = __shfl(base, start);
+  if (i < a.n) slot[i] = base + (lane - start);
 }
 
 // Stable counting sort by bin, in two passes after the scan of the bin
