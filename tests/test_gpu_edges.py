@@ -227,3 +227,45 @@ def test_two_type_lj_matches_the_oracle(prec, tpa):
     assert mx <= (1e-11 if prec == "fp64" else 1e-5)
     assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_shifted_two_type_lj_energies_match_the_oracle(prec):
+    """pair_modify shift yes (offset = the pair energy at the cutoff,
+    pair_lj_cut.cpp:566-571) with per-pair cutoffs: total and per-atom
+    energies, forces and virial against the oracle's"""
+    from lammps_amd import mdx
+    L, n = 9.0, 600
+    x = random_gas(n, L, 0.85, seed=21)
+    t = np.where(np.random.default_rng(22).random(n) < 0.3, 2, 1).astype(np.int32)
+    coeffs = {(1, 1): (1.0, 1.0, 2.5), (1, 2): (1.5, 0.8, 2.0), (2, 2): (0.5, 0.88, 2.2)}
+    ref = M.Ref(M.LJ)
+    ref.box((0, 0, 0), (L, L, L), ntypes=2)
+    ref.add_atoms(x, t)
+    ref.mass(1, 1.0)
+    ref.mass(2, 1.0)
+    ref.pair_lj(2.5, shift=True)
+    for (i, j), (e, s, c) in coeffs.items():
+        ref.pair_coeff(i, j, e, s, c)
+    ref.neighbor(0.3)
+    ref.setup()
+    ref.compute(3, 5)
+    fref = ref.state()[2]
+    eref, wref = ref.energy_virial()
+    ea_ref, va_ref = ref.peratom()
+    ctx = mdx.Context(0)
+    ctx.set_box((0, 0, 0), (L, L, L), (1, 1, 1))
+    ctx.set_atoms(x, t, np.array([0.0, 1.0, 1.0]), tags=np.arange(1, n + 1))
+    ctx.pair_lj_from_coeffs(2, coeffs, shift=True)
+    ctx.precision(mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    ctx.neigh_modify(0.3)
+    out = ctx.compute(3, 5, peratom=True)
+    tol = 1e-11 if prec == "fp64" else 1e-5
+    etol = 1e-11 if prec == "fp64" else 1e-6
+    mx, _ = force_errors(out["f"], fref)
+    assert mx <= tol
+    assert abs(out["eng"] - eref) <= etol * abs(eref)
+    np.testing.assert_allclose(out["virial"], wref, rtol=etol, atol=etol * np.abs(wref).max())
+    np.testing.assert_allclose(out["eatom"], ea_ref, rtol=0, atol=100 * etol * np.abs(ea_ref).max())
+    np.testing.assert_allclose(out["vatom"], va_ref, rtol=0, atol=100 * etol * np.abs(va_ref).max())
+    ctx.close()
