@@ -262,6 +262,30 @@ def halo_share(G, L, reps=20):
                 force_max_rel_vs_full_system=err)
 
 
+def plugin_mode(ctx, reps=10):
+    """The per-call pair-style path (pair_style lj/cut/mdx: positions in from
+    the host, forces back, every step) on the same 2.048 M-atom state: the
+    PCIe-inclusive rate, reported beside the resident `value`."""
+    import ctypes
+    from lammps_amd import mdx
+    x = np.ascontiguousarray(ctx.get_atoms()[0])
+    f = np.zeros_like(x)  # the caller's force array, reused every step as atom->f is
+    L = ctx._L
+    xp = x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    fp = f.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for _ in range(2):
+        ctx._ck(L.mdx_set_positions(ctx.h, xp))
+        ctx._ck(L.mdx_compute(ctx.h, 0, 0, fp, None, None, None, None))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx._ck(L.mdx_set_positions(ctx.h, xp))
+        ctx._ck(L.mdx_compute(ctx.h, 0, 0, fp, None, None, None, None))
+    secs = (time.perf_counter() - t0) / reps
+    n = len(x)
+    return dict(ms_per_step=secs * 1e3, value=n / secs / 1e6, unit="Matom-steps/s",
+                host_bytes_per_step=48 * n, note="set_positions (24 B/atom H2D) + compute + forces (24 B/atom D2H)")
+
+
 def fit_lj_ka(reps=10, check_oracle=True):
     """Differentiable lj/cut force matching on the KA mixture: 64 snapshots of
     8,000 atoms (device MD preparation), forward forces in mixed precision,
@@ -547,6 +571,9 @@ def main():
         log("  clusters forward %.2f ms, adjoint %.2f ms (mixed), grad vs FD %.2e"
             % (replicas["mixed"]["ms_forward"], replicas["mixed"]["ms_adjoint"], replicas["mixed_grad_vs_fd_max_rel"]))
 
+    log("per-call pair-style path (host positions in, forces out, every step)")
+    plugin = plugin_mode(res["mixed"]["ctx"])
+    log("  %.3f ms per step, %.1f Matom-steps/s with PCIe" % (plugin["ms_per_step"], plugin["value"]))
     log("one GPU's share of a 2x2x2 split: octant + ghost shell, pack / unpack")
     share = halo_share(res["mixed"]["ctx"], lj["hi"][0] - lj["lo"][0])
     log("  %d owned + %d ghosts: pack %.3f ms, unpack %.3f ms, force kernel %.3f ms"
@@ -579,6 +606,7 @@ def main():
         "force_matching_ka": fit_ka,
         "eam_cluster_replicas": replicas,
         "sharded_share": share,
+        "plugin_mode_pcie": plugin,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -252,6 +252,8 @@ int mdx_destroy(mdx_ctx *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   halo_free(c);
+  for (hipEvent_t &ev : c->stage_ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (c->red_host) (void)hipHostFree(c->red_host);
   for (auto &r : c->evpool) {
     (void)hipEventDestroy(r.a);

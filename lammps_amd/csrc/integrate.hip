@@ -8,6 +8,10 @@
 // (src/neighbor.cpp:1294-1372) the initial-integrate kernel also tests
 // |x - xhold|^2 > (skin/2)^2 and raises a device flag, so the trigger costs
 // no extra pass over x.
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -228,13 +232,86 @@ int reduce_ke(mdx_ctx *c) {
   return finish_reduction(c, G, 1);
 }
 
+// Host copy between the caller's arrays and the pinned staging buffer.  One
+// thread copies about 10 GB/s, well under the link's rate, so copies above
+// 8 MiB are split over up to eight threads.
+void host_copy(void *dst, const void *src, size_t bytes) {
+  constexpr size_t kSplit = size_t(8) << 20;
+  if (bytes < kSplit) {
+    memcpy(dst, src, bytes);
+    return;
+  }
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<size_t>(std::min(8u, hw), bytes / (size_t(2) << 20));
+  const size_t chunk = ((bytes + nt - 1) / nt + 63) / 64 * 64;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) {
+    const size_t off = chunk * t;
+    if (off >= bytes) break;
+    pool.emplace_back([=] { memcpy((char *)dst + off, (const char *)src + off, std::min(chunk, bytes - off)); });
+  }
+  memcpy(dst, src, std::min(chunk, bytes));
+  for (auto &th : pool) th.join();
+}
+
+// Large transfers through the pinned stage go in chunks so that the host
+// copy of one chunk overlaps the DMA of the previous (host -> device) or the
+// next (device -> host).
+constexpr size_t kStageChunk = size_t(8) << 20;
+
+static int stage_events(mdx_ctx *c) {
+  for (hipEvent_t &ev : c->stage_ev)
+    if (!ev) {
+      hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) {
+        ev = nullptr;
+        return hip_fail(e, "staging events");
+      }
+    }
+  return MDX_OK;
+}
+
+// host -> stage -> device, chunked; returns after the copies are queued
+static int upload_staged(mdx_ctx *c, void *dev, const void *host, size_t bytes) {
+  size_t chunk = std::max(kStageChunk, (bytes + 7) / 8);
+  for (size_t off = 0; off < bytes; off += chunk) {
+    const size_t len = std::min(chunk, bytes - off);
+    host_copy((char *)c->host_stage + off, (const char *)host + off, len);
+    hipError_t e = hipMemcpyAsync((char *)dev + off, (char *)c->host_stage + off, len, hipMemcpyHostToDevice,
+                                  c->stream);
+    if (e != hipSuccess) return hip_fail(e, "staged upload");
+  }
+  return MDX_OK;
+}
+
+// device -> stage -> host, chunked; waits for the last chunk
+static int download_staged(mdx_ctx *c, void *host, const void *dev, size_t bytes) {
+  int rc = stage_events(c);
+  if (rc) return rc;
+  size_t chunk = std::max(kStageChunk, (bytes + 7) / 8);
+  const int nchunk = (int)((bytes + chunk - 1) / chunk);  // at most 8
+  hipError_t e;
+  for (int k = 0; k < nchunk; k++) {
+    const size_t off = chunk * k, len = std::min(chunk, bytes - off);
+    if ((e = hipMemcpyAsync((char *)c->host_stage + off, (const char *)dev + off, len, hipMemcpyDeviceToHost,
+                            c->stream)) != hipSuccess ||
+        (e = hipEventRecord(c->stage_ev[k], c->stream)) != hipSuccess)
+      return hip_fail(e, "staged download");
+  }
+  for (int k = 0; k < nchunk; k++) {
+    const size_t off = chunk * k, len = std::min(chunk, bytes - off);
+    if ((e = hipEventSynchronize(c->stage_ev[k])) != hipSuccess) return hip_fail(e, "staged download");
+    host_copy((char *)host + off, (char *)c->host_stage + off, len);
+  }
+  return MDX_OK;
+}
+
 int scatter_positions(mdx_ctx *c, const double *host_x) {
   const int B = 256, G = (c->n + B - 1) / B;
-  double *stage = c->host_stage;
-  memcpy(stage, host_x, sizeof(double) * 3 * c->n);
   double *dtmp = c->dscratch;
-  hipError_t e = hipMemcpyAsync(dtmp, stage, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) return hip_fail(e, "positions upload");
+  int rc = upload_staged(c, dtmp, host_x, sizeof(double) * 3 * c->n);
+  if (rc) return rc;
+  hipError_t e;
   hipLaunchKernelGGL(k_scatter_x, dim3(G), dim3(B), 0, c->stream, c->n, dtmp, c->orig, c->x,
                      c->qpos ? c->xq : nullptr, c->qs);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "positions scatter");
@@ -254,12 +331,7 @@ int refresh_xq(mdx_ctx *c) {
 int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out) {
   const int B = 256, G = (c->n + B - 1) / B;
   hipLaunchKernelGGL(k_gather3, dim3(G), dim3(B), 0, c->stream, c->n, c->orig, dev, c->dscratch);
-  hipError_t e = hipMemcpyAsync(c->host_stage, c->dscratch, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost,
-                                c->stream);
-  if (e != hipSuccess) return hip_fail(e, "gather copy");
-  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "gather sync");
-  memcpy(host_out, c->host_stage, sizeof(double) * 3 * c->n);
-  return MDX_OK;
+  return download_staged(c, host_out, c->dscratch, sizeof(double) * 3 * c->n);
 }
 
 int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs) {
@@ -300,12 +372,7 @@ int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs) {
 }
 
 static int copy_back(mdx_ctx *c, size_t count, double *host_out) {
-  hipError_t e = hipMemcpyAsync(c->host_stage, c->dscratch, sizeof(double) * count, hipMemcpyDeviceToHost,
-                                c->stream);
-  if (e != hipSuccess) return hip_fail(e, "gather copy");
-  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "gather sync");
-  memcpy(host_out, c->host_stage, sizeof(double) * count);
-  return MDX_OK;
+  return download_staged(c, host_out, c->dscratch, sizeof(double) * count);
 }
 
 int gather_w(mdx_ctx *c, const double4 *dev, double *host_out) {

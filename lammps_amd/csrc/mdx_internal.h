@@ -339,6 +339,7 @@ struct mdx_ctx {
   double *vatom = nullptr;
   double *dscratch = nullptr;    // device, 6*npad doubles (host<->device reordering)
   double *host_stage = nullptr;  // pinned, 6*npad doubles
+  hipEvent_t stage_ev[8] = {};   // chunked staging copies: one event per chunk in flight
   int *host_istage = nullptr;    // pinned, 3*npad ints
   // bins / sort
   int nbins = 0, cap_bins = 0;
