@@ -153,10 +153,10 @@ __device__ __forceinline__ double integrate_atom(int i, const double4 &xi, doubl
 
 // per-atom results of one group: lane 0 stores (and integrates when fused),
 // all lanes join the block sum of {e, virial[6], m v^2}
-template <int TPA, bool EV, bool INTEG>
+template <int TPA, bool EV, bool INTEG, bool LATE = false>
 __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const double4 &xi, double fx, double fy,
                                             double fz, double e, double (&w)[6], const IntegArgs &ig,
-                                            const double4 &vpre, const double4 &hpre, double4 *__restrict__ f,
+                                            double4 vpre, double4 hpre, double4 *__restrict__ f,
                                             double *__restrict__ vatom, double *__restrict__ partials) {
   fx = group_sum<TPA>(fx);
   fy = group_sum<TPA>(fy);
@@ -172,6 +172,10 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
     if (EV)
 #pragma unroll
       for (int q = 0; q < 6; q++) vatom[(size_t)q * npad + i] = w[q];
+    if (INTEG && LATE) {  // v and xhold read here, not held through the neighbour loop
+      vpre = ig.v[i];
+      if (ig.check) hpre = ig.xhold[i];
+    }
     if (INTEG) mv2 = integrate_atom(i, xi, fx, fy, fz, ig, vpre, hpre);
   }
   if (EV) {
@@ -188,6 +192,14 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
 // re-tested on the FP64 positions with its image shift, so the cutoff
 // decision is the FP64 one (the quantised rsq is within ~1e-7 relative).
 constexpr double kBand = 2e-6;
+
+// fused force kernels (lj/cut, eam): read v (and xhold) after the
+// neighbour loop instead of before it, so they do not occupy registers
+// through the loop
+#ifndef MDX_LATE_V
+#define MDX_LATE_V 1
+#endif
+constexpr bool kLateV = MDX_LATE_V;
 
 // ONE: a single atom type, so every pair uses the coefficients of type pair
 // (1,1): the table reads are loop-invariant and leave the neighbour loop.
@@ -233,7 +245,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
   double4 vpre = make_double4(0.0, 0.0, 0.0, 0.0), hpre = vpre;
-  if (INTEG && i < n && lane == 0) {
+  if (INTEG && !kLateV && i < n && lane == 0) {
     vpre = ig.v[i];
     if (ig.check) hpre = ig.xhold[i];
   }
@@ -324,7 +336,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
       }
     }
   }
-  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
+  finish_atom<TPA, EV, INTEG, kLateV>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
                               partials);
 }
 
@@ -476,7 +488,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
   double4 vpre = make_double4(0.0, 0.0, 0.0, 0.0), hpre = vpre;
-  if (INTEG && i < n && lane == 0) {
+  if (INTEG && !kLateV && i < n && lane == 0) {
     vpre = ig.v[i];
     if (ig.check) hpre = ig.xhold[i];
   }
@@ -550,7 +562,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
     }
     if (EV && lane == 0) e += eemb[i];
   }
-  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
+  finish_atom<TPA, EV, INTEG, kLateV>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
                               partials);
 }
 
