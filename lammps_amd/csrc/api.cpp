@@ -225,10 +225,12 @@ int mdx_create(int device, mdx_ctx **out) {
   }
   if (hipMalloc(&c->dflags, 64) != hipSuccess || hipMalloc(&c->red_out, sizeof(double) * MDX_RED_SLOTS) != hipSuccess ||
       hipHostMalloc(&c->red_host, sizeof(double) * MDX_RED_SLOTS, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->trig_host, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipMalloc(&c->lj_dev, sizeof(LJTab)) != hipSuccess) {
     set_error("Insufficient memory on accelerator");
     return MDX_ERR_NOMEM;
   }
+  memset(c->trig_host, 0, 64);
   if (hipMemset(c->dflags, 0, 64) != hipSuccess) {
     set_error("device flags initialisation failed");
     return MDX_ERR_HIP;
@@ -255,6 +257,7 @@ int mdx_destroy(mdx_ctx *c) {
   for (hipEvent_t &ev : c->stage_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->red_host) (void)hipHostFree(c->red_host);
+  if (c->trig_host) (void)hipHostFree(c->trig_host);
   for (auto &r : c->evpool) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -802,11 +805,8 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
       ig.xqn = c->qpos ? c->xqb : nullptr;
       ig.qs = c->qs;
       ig.xhold = c->xhold;
-      ig.dflags = c->dflags;
-      if (ig.check) {
-        hipError_t e = hipMemsetAsync(c->dflags + 1, 0, sizeof(int), c->stream);
-        if (e != hipSuccess) return hip_fail(e, "clear trigger flag");
-      }
+      ig.trig = c->trig_host;
+      ig.stamp = ig.check ? next_trigger_stamp(c) : 0;
       if ((rc = pair_compute(c, out ? 1 : 0, &ig))) return rc;
       if (out) {
         c->eng_vdwl = c->red_host[0];

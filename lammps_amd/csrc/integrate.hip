@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double d
                                                     double4 *__restrict__ x, double4 *__restrict__ v,
                                                     const double4 *__restrict__ f,
                                                     const double4 *__restrict__ xhold, int check,
-                                                    double triggersq, int *__restrict__ dflags,
+                                                    double triggersq, int *__restrict__ trig, int stamp,
                                                     int4 *__restrict__ xq, QScale qs) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double d
     const double4 h = xhold[i];
     const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
     const double rsq = dx * dx + dy * dy + dz * dz;
-    if (rsq > triggersq) atomicOr(&dflags[1], 1);
+    mark_trigger(rsq > triggersq, trig, stamp);
   }
 }
 
@@ -181,13 +181,10 @@ MassTab mass_tab(const mdx_ctx *c) {
 int nve_initial(mdx_ctx *c, double dtv, double dtf, int check) {
   if (c->n == 0) return MDX_OK;
   const int B = 256, G = (c->n + B - 1) / B;
-  if (check) {
-    hipError_t e = hipMemsetAsync(c->dflags + 1, 0, sizeof(int), c->stream);
-    if (e != hipSuccess) return hip_fail(e, "clear trigger flag");
-  }
+  const int stamp = check ? next_trigger_stamp(c) : 0;
   prof_begin(c, MDX_K_NVE_INITIAL);
   hipLaunchKernelGGL(k_nve_initial, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtv, dtf, c->x, c->v, c->f,
-                     c->xhold, check, c->triggersq, c->dflags, c->qpos ? c->xq : nullptr, c->qs);
+                     c->xhold, check, c->triggersq, c->trig_host, stamp, c->qpos ? c->xq : nullptr, c->qs);
   prof_end(c);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDX_OK : hip_fail(e, "nve initial");
@@ -204,13 +201,9 @@ int nve_final(mdx_ctx *c, double dtf) {
 }
 
 int neigh_check_displacement(mdx_ctx *c, int *flag) {
-  hipError_t e = hipMemcpyAsync(c->red_host + MDX_RED_SLOTS - 1, c->dflags + 1, sizeof(int),
-                                hipMemcpyDeviceToHost, c->stream);
-  if (e != hipSuccess) return hip_fail(e, "read trigger flag");
-  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "trigger sync");
-  int v;
-  memcpy(&v, c->red_host + MDX_RED_SLOTS - 1, sizeof(int));
-  *flag = v;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(e, "trigger sync");
+  *flag = __atomic_load_n(c->trig_host, __ATOMIC_ACQUIRE) == c->trig_stamp ? 1 : 0;
   return MDX_OK;
 }
 

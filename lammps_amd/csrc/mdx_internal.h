@@ -292,6 +292,15 @@ __device__ __forceinline__ double qdelta(int qi, int qj, double step) {
   return (double)(int)((unsigned)qi - (unsigned)qj) * step;
 }
 
+// Rebuild trigger of the displacement check (Neighbor::check_distance,
+// src/neighbor.cpp:1322-1372): the first triggering lane of each wavefront
+// stores the check's stamp into the pinned host word, so a wavefront makes
+// at most one host write however many of its atoms moved.
+__device__ __forceinline__ void mark_trigger(bool moved, int *trig, int stamp) {
+  const unsigned long long m = __ballot(moved);
+  if (moved && (int)(__ffsll((long long)m) - 1) == (int)(threadIdx.x & 63)) *trig = stamp;
+}
+
 // Velocity-Verlet work fused into the force kernel (fix nve final integrate
 // of this step, then initial integrate of the next, src/fix_nve.cpp:65-142)
 struct IntegArgs {
@@ -305,7 +314,8 @@ struct IntegArgs {
   int4 *xqn;  // fixed-point copy of xn (mixed gathers), or null
   QScale qs;
   const double4 *xhold;
-  int *dflags;
+  int *trig;   // pinned host word: set to stamp when an atom moved past the trigger
+  int stamp;
 };
 
 struct Launch {
@@ -400,6 +410,11 @@ struct mdx_ctx {
   int last_grid = 0;  // blocks of the last pair launch (partials to reduce)
   double *red_out = nullptr;    // device, MDX_RED_SLOTS
   double *red_host = nullptr;   // pinned, MDX_RED_SLOTS
+  // Rebuild trigger of the displacement check, written by the kernels
+  // straight into pinned host memory: a check stores its own stamp, so the
+  // word needs no clearing and no device-to-host copy before it is read.
+  int *trig_host = nullptr;
+  int trig_stamp = 0;
   // results of the last compute
   double eng_vdwl = 0.0, virial[6] = {0, 0, 0, 0, 0, 0};
   // launch shape and timing (event pairs recorded on `stream`, resolved
@@ -430,6 +445,12 @@ __device__ __forceinline__ int xcd_block() {
   const int per = gridDim.x / kXCD;
   return (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
 }
+// the stamp of the next displacement check (never 0, the word's initial value)
+inline int next_trigger_stamp(mdx_ctx *c) {
+  c->trig_stamp = c->trig_stamp == 0x7fffffff ? 1 : c->trig_stamp + 1;
+  return c->trig_stamp;
+}
+
 inline int xcd_grid(long blocks) { return (int)(((blocks + kXCD - 1) / kXCD) * kXCD); }
 
 void set_error(const std::string &msg);

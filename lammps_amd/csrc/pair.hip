@@ -144,7 +144,7 @@ __device__ __forceinline__ double integrate_atom(int i, const double4 &xi, doubl
     if (ig.xqn) ig.xqn[i] = quantize(ig.qs, p);
     if (ig.check) {
       const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
-      if (dx * dx + dy * dy + dz * dz > ig.triggersq) atomicOr(&ig.dflags[1], 1);
+      mark_trigger(dx * dx + dy * dy + dz * dz > ig.triggersq, ig.trig, ig.stamp);
     }
   }
   ig.v[i] = q;
@@ -916,11 +916,35 @@ int dispatch_mask(mdx_ctx *c, int B, const IntegArgs &ig) {
   return MDX_OK;
 }
 
+// Resident wavefronts per SIMD of the fused force kernels, held by giving
+// each workgroup enough LDS that no more workgroups fit on a CU (160 KiB of
+// LDS, four SIMDs).  By default no cap is set: the fused lj/cut kernel's 66
+// registers already hold it to 7 wavefronts, where it runs fastest (2.048 M
+// atoms, mixed: 0.334 ms per launch; 0.39 ms held to 6 or 5 by LDS, 0.46 ms
+// at 4; 0.40 ms at 8 with the 64-register build).  MDX_PAIR_WAVES sets a cap
+// (0: none).
+int pair_wave_cap(int dflt) {
+  static const int w = [] {
+    const char *s = std::getenv("MDX_PAIR_WAVES");
+    return s ? std::atoi(s) : -1;
+  }();
+  return w >= 0 ? w : dflt;
+}
+
+size_t occupancy_lds(int B, size_t used, int dflt) {
+  const int w = pair_wave_cap(dflt);
+  if (w <= 0) return used;
+  const int per_block = (B + 63) / 64;
+  const int blocks = std::max(1, 4 * w / per_block);
+  const size_t pad = ((size_t)(163840 / (blocks + 1)) + 256) & ~(size_t)255;
+  return std::max(used, pad);
+}
+
 template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad, c->maxneigh,
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), INTEG ? occupancy_lds(B, 0, 0) : 0, c->stream, c->n, c->npad, c->maxneigh,
                      c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->xq, c->qs, c->nbr, c->numneigh,
                      c->lj_dev, c->f, c->vatom, c->partials, ig);
   prof_end(c);
@@ -941,8 +965,9 @@ void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
                      c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
+  const size_t tl_bytes = TL ? 3 * kn * sizeof(float4) : 0;
   hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE, TL>), dim3(G), dim3(B),
-                     TL ? 3 * kn * sizeof(float4) : 0, c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1],
+                     INTEG ? occupancy_lds(B, tl_bytes, 0) : tl_bytes, c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1],
                      c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
                      c->partials, ig);
   prof_end(c);
