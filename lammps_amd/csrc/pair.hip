@@ -922,7 +922,7 @@ int dispatch_mask(mdx_ctx *c, int B, const IntegArgs &ig) {
 // registers already hold it to 7 wavefronts, where it runs fastest (2.048 M
 // atoms, mixed: 0.334 ms per launch; 0.39 ms held to 6 or 5 by LDS, 0.46 ms
 // at 4; 0.40 ms at 8 with the 64-register build).  MDX_PAIR_WAVES sets a cap
-// (0: none).
+// (0: none) on these and on the EAM density pass.
 int pair_wave_cap(int dflt) {
   static const int w = [] {
     const char *s = std::getenv("MDX_PAIR_WAVES");
@@ -960,8 +960,8 @@ void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   const size_t kn = (size_t)c->eam.nr + 1;
   prof_begin(c, MDX_K_EAM_RHO);
-  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>), dim3(G), dim3(B), TL ? kn * sizeof(float4) : 0,
-                     c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
+  hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>), dim3(G), dim3(B),
+                     occupancy_lds(B, TL ? kn * sizeof(float4) : 0, 0), c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
                      c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
