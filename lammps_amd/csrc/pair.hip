@@ -918,11 +918,12 @@ int dispatch_mask(mdx_ctx *c, int B, const IntegArgs &ig) {
 
 // Resident wavefronts per SIMD of the fused force kernels, held by giving
 // each workgroup enough LDS that no more workgroups fit on a CU (160 KiB of
-// LDS, four SIMDs).  By default no cap is set: the fused lj/cut kernel's 66
+// LDS, four SIMDs).  The mixed fused lj/cut kernel gets no cap: its 66
 // registers already hold it to 7 wavefronts, where it runs fastest (2.048 M
-// atoms, mixed: 0.334 ms per launch; 0.39 ms held to 6 or 5 by LDS, 0.46 ms
-// at 4; 0.40 ms at 8 with the 64-register build).  MDX_PAIR_WAVES sets a cap
-// (0: none) on these and on the EAM density pass.
+// atoms: 0.334 ms per launch; 0.347 ms held to 7 by LDS, 0.39 ms to 6 or 5,
+// 0.46 ms to 4; 0.40 ms at 8 with the 64-register build).  The FP64 one runs
+// 2-3% faster held to 6, its default.  MDX_PAIR_WAVES sets a cap (0: none)
+// on both and on the EAM passes.
 int pair_wave_cap(int dflt) {
   static const int w = [] {
     const char *s = std::getenv("MDX_PAIR_WAVES");
@@ -944,7 +945,7 @@ template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), INTEG ? occupancy_lds(B, 0, 0) : 0, c->stream, c->n, c->npad, c->maxneigh,
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), INTEG ? occupancy_lds(B, 0, MIXED ? 0 : 6) : 0, c->stream, c->n, c->npad, c->maxneigh,
                      c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->xq, c->qs, c->nbr, c->numneigh,
                      c->lj_dev, c->f, c->vatom, c->partials, ig);
   prof_end(c);
