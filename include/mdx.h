@@ -38,11 +38,6 @@ enum {
 };
 
 enum { MDX_FP64 = 0, MDX_MIXED = 1 };        /* pair arithmetic precision */
-/* neighbour-list layout: global 32-bit indices read by per-neighbour gathers
-   (the default), 16-bit slots into LDS-staged bricks of bins, or one bit per
-   stencil candidate (the candidates re-enumerated from the bins by the pair
-   kernels; ~40 B per atom instead of 4 B per neighbour) */
-enum { MDX_LIST_GATHER = 0, MDX_LIST_BRICK = 1, MDX_LIST_MASK = 2 };
 enum { MDX_PAIR_NONE = 0, MDX_PAIR_LJ = 1, MDX_PAIR_EAM = 2 };
 /* neighbour positions read by the mixed-precision gather kernels: 32-bit
    fixed point (16 B, the default) or the FP64 positions (32 B) */
@@ -136,8 +131,11 @@ int mdx_neigh_pairs(mdx_ctx *ctx, int *pairs, long cap, long *npairs);
 /* Pair::compute(eflag, vflag) on the current positions (src/pair.h:135;
    eflag/vflag encoding of src/integrate.cpp:99-112: eflag 1 global, 2 per-atom;
    vflag 1|2 global, 4 per-atom).  Builds the list first if none exists.
-   Outputs are optional: f [n][3] (overwritten, caller order), eng_vdwl,
-   virial[6], eatom[n], vatom[n][6].  Replaces ljl_gpu_compute /
+   Outputs are optional: f [n][3] (caller order) receives f += the pair
+   forces, as atom->f accumulates over the force styles of a step
+   (src/pair.h:33-35, src/pair_lj_cut.cpp:115-117); eng_vdwl, virial[6],
+   eatom[n], vatom[n][6] are overwritten, as ev_setup zeroes them at every
+   compute (src/pair.cpp:706-725).  Replaces ljl_gpu_compute /
    eam_gpu_compute + eam_gpu_compute_force + lmp_gpu_forces
    (src/GPU/pair_lj_cut_gpu.cpp:87-122, src/GPU/pair_eam_gpu.cpp:96-140,
    src/GPU/fix_gpu.cpp:236-257). */
@@ -186,23 +184,13 @@ int mdx_copy_bandwidth(mdx_ctx *ctx, long nbytes, int iters, double *gbs);
    threads per atom (1, 2, 4, 8, 16 or 32; the `threads_per_atom` keyword of
    the reference's `package gpu`, src/GPU/fix_gpu.cpp:112-150). */
 int mdx_set_launch(mdx_ctx *ctx, int block, int tpa);
-/* Threads per atom of the list build (1, 4, 8, 16 or 32; 0 = default).
-   Every variant yields the same rows. */
-int mdx_set_build_launch(mdx_ctx *ctx, int tpa);
-int mdx_get_launch(mdx_ctx *ctx, int *block, int *tpa, int *build_tpa);
+int mdx_get_launch(mdx_ctx *ctx, int *block, int *tpa);
 /* Autotuner (the role of `package gpu ... tpa` chosen by hand in the
-   reference, src/GPU/fix_gpu.cpp:112-150): times the pair kernels on the
-   current list for threads per atom {4, 8, 16} x block {128, 256}, and (index
-   lists) the list build for threads per atom {1, 4, 8}, `reps` launches
-   each; keeps the fastest and recomputes the list and forces with them.  The
-   extra builds are at unchanged positions (same rows, same order) but restart
-   the rebuild-trigger reference; call it right after setup. */
+   reference, src/GPU/fix_gpu.cpp:112-150): times the force kernel of the
+   current configuration for threads per atom {4, 8, 16} x block {128, 256},
+   `reps` launches each; keeps the fastest and recomputes the forces with it. */
 int mdx_autotune(mdx_ctx *ctx, int reps);
-/* List layout (MDX_LIST_GATHER or MDX_LIST_BRICK) and, for bricks, the
-   brick size in bins (0 = default 8x4x4; shrunk automatically until the
-   staged halo fits the LDS).  Takes effect at the next list build. */
-int mdx_set_list_mode(mdx_ctx *ctx, int mode, int rx, int ry, int rz);
-/* Position format of the mixed-precision gathers (gather list layout).
+/* Position format of the mixed-precision gathers.
    MDX_POS_Q32: each neighbour is read as 32-bit fixed point per dimension
    (resolution prd * 2^-32 periodic, prd * 2^-30 otherwise) and its
    separation rebuilt in FP64; pairs within 2e-6 relative of the cutoff are

@@ -95,7 +95,7 @@ int check_ctx(mdx_ctx *c) {
 void free_atoms(mdx_ctx *c) {
   void *ptrs[] = {c->x, c->xb, c->v, c->vb, c->f, c->xhold, c->tag, c->tagb, c->orig, c->origb, c->image,
                   c->imageb, c->inv, c->vatom, c->dscratch, c->bkey, c->bkey_s, c->bidx, c->bidx_s,
-                  c->numneigh, c->nbr, c->nbr16, c->rho, c->fp, c->eemb, c->partials, c->xq, c->xqb, c->nmask};
+                  c->numneigh, c->nbr, c->rho, c->fp, c->eemb, c->partials, c->xq, c->xqb};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->x = c->xb = c->v = c->vb = c->f = c->xhold = nullptr;
@@ -103,11 +103,8 @@ void free_atoms(mdx_ctx *c) {
   c->vatom = c->dscratch = nullptr;
   c->bkey = c->bkey_s = nullptr;
   c->bidx = c->bidx_s = c->numneigh = c->nbr = nullptr;
-  c->nbr16 = nullptr;
   c->rho = c->fp = c->eemb = c->partials = nullptr;
   c->xq = c->xqb = nullptr;
-  c->nmask = nullptr;
-  c->mwords = 0;
   c->xq_stale = true;
   if (c->host_stage) (void)hipHostFree(c->host_stage);
   if (c->host_istage) (void)hipHostFree(c->host_istage);
@@ -121,7 +118,7 @@ void free_atoms(mdx_ctx *c) {
 
 // the fixed-point mirror xq is kept for mixed-precision gathers
 void update_qpos(mdx_ctx *c) {
-  const bool on = c->precision == MDX_MIXED && c->list_mode != MDX_LIST_BRICK && c->pos_format == MDX_POS_Q32;
+  const bool on = c->precision == MDX_MIXED && c->pos_format == MDX_POS_Q32;
   if (on && !c->qpos) c->xq_stale = true;
   c->qpos = on;
 }
@@ -166,10 +163,7 @@ int prepare(mdx_ctx *c) {
   int rc = neigh_setup_grid(c);
   if (rc) return rc;
   // first-guess list capacity from the mean density; neigh_build regrows
-  // (mask lists are sized by neigh_build)
-  const bool have_list = c->list_mode == MDX_LIST_MASK ||
-                         (c->list_mode == MDX_LIST_BRICK ? c->nbr16 != nullptr : c->nbr != nullptr);
-  if (!have_list && c->n > 0) {
+  if (!c->nbr && c->n > 0) {
     const double vol = c->prd[0] * c->prd[1] * c->prd[2];
     const double rc3 = c->cutneighmax * c->cutneighmax * c->cutneighmax;
     int guess = (int)(1.5 * 4.18879 * rc3 * c->n / vol) + 16;
@@ -235,11 +229,6 @@ int mdx_create(int device, mdx_ctx **out) {
     set_error("device flags initialisation failed");
     return MDX_ERR_HIP;
   }
-  // LDS available to the brick kernels' dynamic allocation, minus their
-  // static tables and reductions
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 16384)
-    c->lds_budget = prop.sharedMemPerBlock - 8192;
   *out = c;
   return MDX_OK;
 }
@@ -593,82 +582,14 @@ int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
       ((e = hipMemcpy(num.data(), c->numneigh, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
        (e = hipMemcpy(tag.data(), c->tag, sizeof(int) * n, hipMemcpyDeviceToHost)) != hipSuccess))
     return hip_fail(e, "list copy");
-  // global neighbour index of entry k of atom i
-  std::vector<int> nb;
-  std::vector<unsigned short> nb16;
-  std::vector<std::vector<int>> slotmap;  // brick mode: per brick, LDS slot -> global index
-  std::vector<int> brick_of;              // brick mode: brick of each atom
-  if (c->list_mode == MDX_LIST_BRICK) {
-    nb16.resize((size_t)c->maxneigh * c->npad);
-    std::vector<unsigned> bk(n);
-    std::vector<int> bs(c->nbins + 1);
-    if ((e = hipMemcpy(nb16.data(), c->nbr16, sizeof(unsigned short) * nb16.size(), hipMemcpyDeviceToHost)) !=
-            hipSuccess ||
-        (e = hipMemcpy(bk.data(), c->bkey_s, sizeof(unsigned) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
-        (e = hipMemcpy(bs.data(), c->bin_start, sizeof(int) * bs.size(), hipMemcpyDeviceToHost)) != hipSuccess)
-      return hip_fail(e, "list copy");
-    slotmap.resize(c->br.nbricks);
-    brick_of.resize(n);
-    for (int i = 0; i < n; i++) brick_of[i] = brick_of_bin(c->br, c->na, (int)bk[i]);
-    for (int b = 0; b < c->br.nbricks; b++) brick_slot_map(c->na, c->br, b, bs.data(), slotmap[b]);
-  } else if (c->list_mode == MDX_LIST_MASK) {
-    // expand the candidate bits into index rows (host copy of the device walk)
-    std::vector<unsigned> bk(n), mk((size_t)c->mwords * n);
-    std::vector<int> bs(c->nbins + 1);
-    if (n > 0 &&
-        ((e = hipMemcpy(mk.data(), c->nmask, sizeof(unsigned) * mk.size(), hipMemcpyDeviceToHost)) != hipSuccess ||
-         (e = hipMemcpy(bk.data(), c->bkey_s, sizeof(unsigned) * n, hipMemcpyDeviceToHost)) != hipSuccess ||
-         (e = hipMemcpy(bs.data(), c->bin_start, sizeof(int) * bs.size(), hipMemcpyDeviceToHost)) != hipSuccess))
-      return hip_fail(e, "list copy");
-    const int maxn = *std::max_element(num.begin(), num.end());
-    nb.assign((size_t)std::max(maxn, 1) * n, 0);
-    const NeighArgs &a = c->na;
-    const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
-    for (int i = 0; i < n; i++) {
-      const unsigned key = bk[i];
-      int ix, iy, iz;
-      bin_coords(a, (int)key, ix, iy, iz);
-      long bit = 0;
-      int k = 0;
-      for (int r = 0; r < nrow; r++) {
-        RowSegs rs;
-        if (!row_segments(a, bs.data(), ix, iy, iz, r, rs)) continue;
-        for (int sgi = 0; sgi < 3; sgi++)
-          for (int j = rs.j0[sgi]; j < rs.j1[sgi]; j++, bit++)
-            if ((mk[(size_t)i * c->mwords + bit / 32] >> (bit % 32)) & 1u) {
-              if (k < maxn) nb[(size_t)i * maxn + k] = j;
-              k++;
-            }
-      }
-      if (k != num[i]) {
-        set_error("mask list decode does not match the neighbour count (internal error)");
-        return MDX_ERR_ARG;
-      }
-    }
-    long cnt = 0;
-    for (int i = 0; i < n; i++)
-      for (int k = 0; k < num[i]; k++) {
-        const int j = nb[(size_t)i * maxn + k];
-        if (tag[i] < tag[j]) {
-          if (pairs && cnt < cap) {
-            pairs[2 * cnt] = tag[i];
-            pairs[2 * cnt + 1] = tag[j];
-          }
-          cnt++;
-        }
-      }
-    if (npairs) *npairs = cnt;
-    return MDX_OK;
-  } else {
-    nb.resize((size_t)c->maxneigh * c->npad);
-    if ((e = hipMemcpy(nb.data(), c->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost)) != hipSuccess)
-      return hip_fail(e, "list copy");
-  }
+  std::vector<int> nb(n > 0 ? (size_t)c->maxneigh * c->npad : 0);
+  if (n > 0 && (e = hipMemcpy(nb.data(), c->nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_fail(e, "list copy");
   long cnt = 0;
   for (int i = 0; i < n; i++)
     for (int k = 0; k < num[i]; k++) {
       const size_t q = (size_t)i * c->maxneigh + k;
-      const int j = c->list_mode == MDX_LIST_BRICK ? slotmap[brick_of[i]].at(nb16[q]) : (nb[q] & MDX_IDX_MASK);
+      const int j = nb[q] & MDX_IDX_MASK;
       if (tag[i] < tag[j]) {
         if (pairs && cnt < cap) {
           pairs[2 * cnt] = tag[i];
@@ -696,7 +617,11 @@ int mdx_compute(mdx_ctx *c, int eflag, int vflag, double *f, double *eng_vdwl, d
       for (int k = 0; k < 6; k++) virial[k] = c->virial[k];
   }
   if (c->n == 0) return MDX_OK;
-  if (f && (rc = gather_vec3(c, c->f, f))) return rc;
+  // f accumulates, as atom->f does across Pair::compute and the other force
+  // styles of a step (src/pair.h:33-35, src/pair_lj_cut.cpp:115-117);
+  // eatom/vatom are the pair's own arrays, which ev_setup zeroes at every
+  // compute (src/pair.cpp:706-725), so they are overwritten
+  if (f && (rc = gather_vec3(c, c->f, f, true))) return rc;
   if (eatom && ev && (rc = gather_w(c, c->f, eatom))) return rc;
   if (vatom && ev && (rc = gather_soa(c, c->vatom, 6, vatom))) return rc;
   hipError_t e = hipStreamSynchronize(c->stream);
@@ -764,7 +689,7 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
   // integrate of n and the initial integrate of n+1 (writing x(n+1) to the
   // spare position buffer), so only the first step of a call runs a
   // standalone initial integrate and no step runs a standalone final one.
-  const bool fused = c->fuse && c->list_mode != MDX_LIST_BRICK;
+  const bool fused = c->fuse;
   bool pending_initial = true;
   auto is_decide = [c](int ago) { return ago >= c->delay && ago % c->every == 0; };
   for (int s = 0; s < nsteps; s++) {
@@ -871,12 +796,11 @@ int mdx_set_launch(mdx_ctx *c, int block, int tpa) {
   return MDX_OK;
 }
 
-int mdx_get_launch(mdx_ctx *c, int *block, int *tpa, int *build_tpa) {
+int mdx_get_launch(mdx_ctx *c, int *block, int *tpa) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (block) *block = c->launch.block;
   if (tpa) *tpa = c->launch.tpa;
-  if (build_tpa) *build_tpa = c->launch.build_tpa;
   return MDX_OK;
 }
 
@@ -888,7 +812,6 @@ int mdx_autotune(mdx_ctx *c, int reps) {
   if (rc) return rc;
   if (reps < 1) reps = 3;
   if ((rc = ensure_built(c))) return rc;
-  if (c->list_mode == MDX_LIST_BRICK) return MDX_OK;  // the brick kernels have one shape
   hipEvent_t e0, e1;
   hipError_t e = hipEventCreate(&e0);
   if (e != hipSuccess) return hip_fail(e, "autotune events");
@@ -926,89 +849,8 @@ int mdx_autotune(mdx_ctx *c, int reps) {
     return rc ? rc : hip_fail(e, "autotune");
   }
   c->launch = pick;
-  // threads per atom of the list build: rebuilding at unchanged positions
-  // gives the same rows in the same order (the bin sort is stable), so after
-  // setup the extra builds leave the state as it was
-  if (c->list_mode == MDX_LIST_GATHER) {
-    float bbest = 1e30f;
-    int bpick = c->launch.build_tpa;
-    for (int bt : {1, 4, 8}) {
-      c->launch.build_tpa = bt;
-      if ((rc = neigh_build(c))) break;  // warm-up (and capacity)
-      e = hipEventCreate(&e0);
-      if (e == hipSuccess && (e = hipEventCreate(&e1)) != hipSuccess) (void)hipEventDestroy(e0);
-      if (e != hipSuccess) {
-        c->profile = prof;
-        return hip_fail(e, "autotune events");
-      }
-      e = hipEventRecord(e0, c->stream);
-      for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = neigh_build(c);
-      if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
-      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.0f;
-      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      if (rc || e != hipSuccess) break;
-      if (ms < bbest) {
-        bbest = ms;
-        bpick = bt;
-      }
-    }
-    if (rc || e != hipSuccess) {
-      c->profile = prof;
-      return rc ? rc : hip_fail(e, "autotune");
-    }
-    c->launch.build_tpa = bpick;
-    if ((rc = neigh_build(c))) {
-      c->profile = prof;
-      return rc;
-    }
-  }
   c->profile = prof;
   return pair_compute(c, 0);
-}
-
-int mdx_set_list_mode(mdx_ctx *c, int mode, int rx, int ry, int rz) {
-  int rc = check_ctx(c);
-  if (rc) return rc;
-  if (mode != MDX_LIST_GATHER && mode != MDX_LIST_BRICK && mode != MDX_LIST_MASK) {
-    set_error("list mode must be MDX_LIST_GATHER, MDX_LIST_BRICK or MDX_LIST_MASK");
-    return MDX_ERR_ARG;
-  }
-  if (rx < 0 || ry < 0 || rz < 0 || rx > 64 || ry > 64 || rz > 64) {
-    set_error("brick dimensions must be 0 (default) or 1..64 bins");
-    return MDX_ERR_ARG;
-  }
-  if (mode != c->list_mode) {
-    (void)hipFree(c->nbr);
-    (void)hipFree(c->nbr16);
-    (void)hipFree(c->nmask);
-    c->nbr = nullptr;
-    c->nbr16 = nullptr;
-    c->nmask = nullptr;
-    c->mwords = 0;
-    c->maxneigh = 0;
-  }
-  c->list_mode = mode;
-  update_qpos(c);
-  const int def[3] = {8, 4, 4};
-  c->brick_req[0] = rx ? rx : def[0];
-  c->brick_req[1] = ry ? ry : def[1];
-  c->brick_req[2] = rz ? rz : def[2];
-  c->built = false;
-  return MDX_OK;
-}
-
-int mdx_set_build_launch(mdx_ctx *c, int tpa) {
-  int rc = check_ctx(c);
-  if (rc) return rc;
-  if (tpa != 0 && tpa != 1 && tpa != 4 && tpa != 8 && tpa != 16 && tpa != 32) {
-    set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
-    return MDX_ERR_ARG;
-  }
-  c->launch.build_tpa = tpa ? tpa : 1;
-  return MDX_OK;
 }
 
 int mdx_copy_bandwidth(mdx_ctx *c, long nbytes, int iters, double *gbs) {

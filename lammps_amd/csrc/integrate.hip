@@ -266,7 +266,7 @@ static int stage_events(mdx_ctx *c) {
 
 // host -> stage -> device, chunked; returns after the copies are queued
 static int upload_staged(mdx_ctx *c, void *dev, const void *host, size_t bytes) {
-  size_t chunk = std::max(kStageChunk, (bytes + 7) / 8);
+  size_t chunk = std::max(kStageChunk, ((bytes + 7) / 8 + 63) / 64 * 64);  // whole doubles per chunk
   for (size_t off = 0; off < bytes; off += chunk) {
     const size_t len = std::min(chunk, bytes - off);
     host_copy((char *)c->host_stage + off, (const char *)host + off, len);
@@ -277,11 +277,35 @@ static int upload_staged(mdx_ctx *c, void *dev, const void *host, size_t bytes) 
   return MDX_OK;
 }
 
-// device -> stage -> host, chunked; waits for the last chunk
-static int download_staged(mdx_ctx *c, void *host, const void *dev, size_t bytes) {
+// host[k] += stage[k] over `count` doubles, split as host_copy splits
+void host_add(double *dst, const double *src, size_t count) {
+  constexpr size_t kSplit = size_t(1) << 20;
+  auto add = [](double *d, const double *s, size_t n) {
+    for (size_t k = 0; k < n; k++) d[k] += s[k];
+  };
+  if (count < kSplit) {
+    add(dst, src, count);
+    return;
+  }
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<size_t>(std::min(8u, hw), count / (size_t(1) << 18));
+  const size_t chunk = ((count + nt - 1) / nt + 7) / 8 * 8;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) {
+    const size_t off = chunk * t;
+    if (off >= count) break;
+    pool.emplace_back([=] { add(dst + off, src + off, std::min(chunk, count - off)); });
+  }
+  add(dst, src, std::min(chunk, count));
+  for (auto &th : pool) th.join();
+}
+
+// device -> stage -> host, chunked; waits for the last chunk.  With `add`
+// the host array receives host += device (doubles) instead of a copy.
+static int download_staged(mdx_ctx *c, void *host, const void *dev, size_t bytes, bool add = false) {
   int rc = stage_events(c);
   if (rc) return rc;
-  size_t chunk = std::max(kStageChunk, (bytes + 7) / 8);
+  size_t chunk = std::max(kStageChunk, ((bytes + 7) / 8 + 63) / 64 * 64);  // whole doubles per chunk
   const int nchunk = (int)((bytes + chunk - 1) / chunk);  // at most 8
   hipError_t e;
   for (int k = 0; k < nchunk; k++) {
@@ -294,7 +318,11 @@ static int download_staged(mdx_ctx *c, void *host, const void *dev, size_t bytes
   for (int k = 0; k < nchunk; k++) {
     const size_t off = chunk * k, len = std::min(chunk, bytes - off);
     if ((e = hipEventSynchronize(c->stage_ev[k])) != hipSuccess) return hip_fail(e, "staged download");
-    host_copy((char *)host + off, (char *)c->host_stage + off, len);
+    if (add)
+      host_add(reinterpret_cast<double *>((char *)host + off), reinterpret_cast<const double *>((char *)c->host_stage + off),
+               len / sizeof(double));
+    else
+      host_copy((char *)host + off, (char *)c->host_stage + off, len);
   }
   return MDX_OK;
 }
@@ -321,10 +349,10 @@ int refresh_xq(mdx_ctx *c) {
   return MDX_OK;
 }
 
-int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out) {
+int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out, bool add) {
   const int B = 256, G = (c->n + B - 1) / B;
   hipLaunchKernelGGL(k_gather3, dim3(G), dim3(B), 0, c->stream, c->n, c->orig, dev, c->dscratch);
-  return download_staged(c, host_out, c->dscratch, sizeof(double) * 3 * c->n);
+  return download_staged(c, host_out, c->dscratch, sizeof(double) * 3 * c->n, add);
 }
 
 int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs) {

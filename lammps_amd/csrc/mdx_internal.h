@@ -56,15 +56,6 @@ struct NeighArgs {
   RefBins ref;
 };
 
-// LDS-staged bricks of bins (brick.hip)
-struct BrickArgs {
-  int R[3];        // brick size in bins
-  int nbrick[3];   // bricks per dimension
-  int nbricks;
-  int max_slots;   // largest number of staged (halo) atoms over all bricks
-  int cap;         // neighbour-list row capacity (entries)
-};
-
 struct LJTab {  // (ntypes+1)^2 row-major, as Pair::init_one fills them
   double cutsq[MDX_NT1 * MDX_NT1], lj1[MDX_NT1 * MDX_NT1], lj2[MDX_NT1 * MDX_NT1],
       lj3[MDX_NT1 * MDX_NT1], lj4[MDX_NT1 * MDX_NT1], offset[MDX_NT1 * MDX_NT1];
@@ -84,8 +75,9 @@ struct EAMArgs {
   const float4 *rhor_pf, *z2r_pf;
 };
 
-// ---- eam spline evaluation, shared by every list layout (gather, mask,
-// brick) so that the same pair gives the same bits in each.  The spline
+// ---- eam spline evaluation of the pair kernels (every variant of the
+// density and force passes uses these, so a pair gives the same bits in
+// each).  The spline
 // index arithmetic is PairEAM::compute's (pair_eam.cpp:188-192); tables are
 // the per-knot packed rhor/z2r of EAMArgs (MIXED: FP32 coefficients).
 template <typename T>
@@ -170,14 +162,6 @@ __device__ __forceinline__ void eam_pair_term(const EAMArgs &a, int ri, int rj, 
   fpair = -psip * recip;
 }
 
-// Candidate segments of one stencil row of the atom in bin (ix, iy, iz).
-// Rows are numbered r = (dz + Sz)(2Sy + 1) + (dy + Sy), dz outer, the list
-// build's order; a row is at most three contiguous atom ranges (x wrapped
-// low, main, x wrapped high), each with its periodic image code
-// (sx+1) + 3(sy+1) + 9(sz+1).  A row outside a non-periodic box is empty.
-struct RowSegs {
-  int j0[3], j1[3], code[3];
-};
 __host__ __device__ inline int imax2(int a, int b) { return a > b ? a : b; }
 __host__ __device__ inline int imin2(int a, int b) { return a < b ? a : b; }
 
@@ -209,65 +193,6 @@ __host__ __device__ inline void bin_coords(const NeighArgs &a, int b, int &x, in
   const int rem = r - y0 * a.nbin[2];
   z = rem / th;
   y = y0 + rem % th;
-}
-
-__host__ __device__ inline int row_segments(const NeighArgs &a, const int *__restrict__ bin_start, int ix, int iy,
-                                            int iz, int r, RowSegs &rs) {
-  const int wy = 2 * a.S[1] + 1;
-  const int dz = r / wy - a.S[2], dy = r % wy - a.S[1];
-  const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
-#pragma unroll
-  for (int s = 0; s < 3; s++) rs.j0[s] = rs.j1[s] = rs.code[s] = 0;
-  int jz = iz + dz, sz = 0, jy = iy + dy, sy = 0;
-  if (jz < 0) {
-    if (!a.periodic[2]) return 0;
-    jz += nbz;
-    sz = -1;
-  } else if (jz >= nbz) {
-    if (!a.periodic[2]) return 0;
-    jz -= nbz;
-    sz = 1;
-  }
-  if (jy < 0) {
-    if (!a.periodic[1]) return 0;
-    jy += nby;
-    sy = -1;
-  } else if (jy >= nby) {
-    if (!a.periodic[1]) return 0;
-    jy -= nby;
-    sy = 1;
-  }
-  const int rowbase = bin_row(a, jy, jz) * nbx;
-  const int x0 = ix - a.S[0], x1 = ix + a.S[0];
-  int total = 0;
-#pragma unroll
-  for (int s = 0; s < 3; s++) {
-    int lo, hi, sx;
-    bool on;
-    if (s == 0) {
-      on = x0 < 0 && a.periodic[0];
-      lo = imax2(x0 + nbx, 0);
-      hi = nbx - 1;
-      sx = -1;
-    } else if (s == 1) {
-      on = true;
-      lo = imax2(x0, 0);
-      hi = imin2(x1, nbx - 1);
-      sx = 0;
-    } else {
-      on = x1 >= nbx && a.periodic[0];
-      lo = 0;
-      hi = imin2(x1 - nbx, nbx - 1);
-      sx = 1;
-    }
-    if (on && lo <= hi) {
-      rs.j0[s] = bin_start[rowbase + lo];
-      rs.j1[s] = bin_start[rowbase + hi + 1];
-      rs.code[s] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-      total += rs.j1[s] - rs.j0[s];
-    }
-  }
-  return total;
 }
 
 // 32-bit fixed-point positions read by the mixed-precision gathers (16 B
@@ -321,7 +246,6 @@ struct IntegArgs {
 struct Launch {
   int block = 256;     // threads per workgroup of the pair kernels
   int tpa = 8;         // threads per atom in the pair kernels
-  int build_tpa = 1;   // threads per atom in the list build
 };
 
 }  // namespace mdx
@@ -342,7 +266,7 @@ struct mdx_ctx {
   int4 *xq = nullptr, *xqb = nullptr;
   mdx::QScale qs{};
   int pos_format = MDX_POS_Q32;  // mixed precision gathers: MDX_POS_Q32 or MDX_POS_FP64
-  bool qpos = false;             // xq is maintained (mixed precision, gather layout, Q32)
+  bool qpos = false;             // xq is maintained (mixed precision, Q32)
   bool xq_stale = true;          // xq must be rebuilt from x before the next force call
   int *tag = nullptr, *tagb = nullptr, *orig = nullptr, *origb = nullptr, *image = nullptr, *imageb = nullptr;
   int *inv = nullptr;  // caller index -> device index
@@ -362,10 +286,8 @@ struct mdx_ctx {
   double skin = 0.3;
   int every = 1, delay = 10, check = 1, one = 2000;
   double triggersq = 0.0225, cutneighmax = 0.0;
-  // dflags: [0] max count, [1] displaced, [4] lost atom, [5] brick staging
-  // overflow, [6] brick count scratch, [8..9] 64-bit neighbour total
+  // dflags: [0] max count, [4] lost atom, [8..9] 64-bit neighbour total
   int *nbr = nullptr, *numneigh = nullptr, *dflags = nullptr;
-  unsigned short *nbr16 = nullptr;  // brick mode: LDS slot numbers, [npad][cap]
   // ghost-shell exchange (halo.hip): per swap, the device indices of the
   // atoms to send (valid until the next list build) and their image shift
   int *halo_list[MDX_HALO_SWAPS] = {};
@@ -375,14 +297,6 @@ struct mdx_ctx {
   int halo_flag_cap = 0;
   void *halo_scan_tmp = nullptr;
   size_t halo_scan_bytes = 0;
-  // mask lists: one bit per stencil candidate of each atom, in the build's
-  // enumeration order, [npad][mwords] (mwords a multiple of 4)
-  unsigned *nmask = nullptr;
-  int mwords = 0;
-  int list_mode = MDX_LIST_GATHER;
-  int brick_req[3] = {8, 4, 4};
-  mdx::BrickArgs br{};
-  size_t lds_budget = 60 * 1024;
   int maxneigh = 0;
   long nbr_elems = 0;
   bool built = false;
@@ -390,7 +304,7 @@ struct mdx_ctx {
   long ncalls = 0, ndanger = 0;
   long ntimestep = 0;
   bool run_setup_done = false;
-  bool fuse = true;  // fuse fix nve into the force kernel in mdx_run (gather layout)
+  bool fuse = true;  // fuse fix nve into the force kernel in mdx_run
   // pair
   int pair_style = MDX_PAIR_NONE;
   int precision = MDX_FP64;
@@ -466,15 +380,7 @@ int prof_resolve(mdx_ctx *c);
 int neigh_setup_grid(mdx_ctx *c);
 int neigh_build(mdx_ctx *c);
 int alloc_list(mdx_ctx *c, int cap);
-int alloc_mask(mdx_ctx *c, int words);
 int neigh_check_displacement(mdx_ctx *c, int *flag);
-
-// LDS-staged bricks (brick.hip)
-int brick_prepare(mdx_ctx *c);
-int brick_build_list(mdx_ctx *c);
-int brick_pair(mdx_ctx *c, int ev);
-void brick_slot_map(const NeighArgs &a, const BrickArgs &br, int brick, const int *bin_start, std::vector<int> &gidx);
-int brick_of_bin(const BrickArgs &br, const NeighArgs &a, int bin);
 
 // pair kernels (pair.hip); eflag/vflag as Pair::compute.  Writes f (device,
 // sorted order), per-atom energy in f.w, per-atom virial in vatom, and the
@@ -489,6 +395,7 @@ int finish_reduction(mdx_ctx *c, int nblocks, int nslots);
 int scatter_positions(mdx_ctx *c, const double *host_x);
 int refresh_xq(mdx_ctx *c);  // rebuild the fixed-point positions if stale
 int copy_probe(mdx_ctx *c, long nbytes, int iters, double *gbs);
-int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out);
+// device order -> caller order [n][3] on the host; add: host += device
+int gather_vec3(mdx_ctx *c, const double4 *dev, double *host_out, bool add = false);
 
 }  // namespace mdx

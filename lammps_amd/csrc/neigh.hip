@@ -8,14 +8,12 @@
 // box (Domain::pbc, src/domain.cpp:478), binned on the reference's own bin
 // grid (binsize = cutneigh/2, src/neighbor.cpp:1580-1620), and stably
 // sorted by bin (counting sort: arrival slots, then ranks by atom index)
-// so that every x-row of bins is one contiguous atom range.  A group of TPA lanes per atom then scans the 5x5 rows of its
-// stencil (each row at most 3 contiguous segments: wrapped-low, main,
-// wrapped-high) and keeps j iff rsq <= cutneighsq, with rsq evaluated in
-// FP64 without contraction on the periodic image x_j + s*L, the value the
-// reference computes for a ghost atom (atom_vec_atomic.cpp:290-297).  The
-// lanes of a group test consecutive candidates (coalesced gathers); a
-// wavefront ballot and a popcount prefix give each kept candidate its slot,
-// so the list order is the sequential scan order whatever TPA is.
+// so that every x-row of bins is one contiguous atom range.  One lane per
+// atom then scans the 5x5 rows of its stencil (each row at most 3
+// contiguous segments: wrapped-low, main, wrapped-high) and keeps j iff
+// rsq <= cutneighsq, with rsq evaluated in FP64 without contraction on the
+// periodic image x_j + s*L, the value the reference computes for a ghost
+// atom (atom_vec_atomic.cpp:290-297).  Rows list candidates in scan order.
 // The result is a FULL list, one row of `cap` entries per atom (row major);
 // each entry holds the neighbour index (bits 0-26) and its periodic image
 // code (bits 27-31) so pair kernels rebuild the same image between builds.
@@ -255,117 +253,12 @@ __global__ void k_permute(int n, const int *__restrict__ src, const double4 *__r
   inv[oi] = k;
 }
 
-// Full list, TPA lanes per atom.  All lanes of a group run the same loop
-// trip counts (bounds depend only on the atom), so the group's lanes are
-// active together at every ballot.
-template <int TPA>
-__global__ MDX_BUILD_BOUNDS void k_build_full(NeighArgs a, const double4 *__restrict__ x,
-                                                   const unsigned *__restrict__ bkey,
-                                                   const int *__restrict__ bin_start, int *__restrict__ nbr,
-                                                   int *__restrict__ numneigh, int *__restrict__ dflags,
-                                                   unsigned long long *__restrict__ nsum) {
-  __shared__ double s_cut[MDX_NT1 * MDX_NT1];
-  stage_cutneighsq(a, s_cut);
-  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
-  if (i >= a.n) return;  // whole groups leave together (TPA divides 64)
-  const int lane = threadIdx.x % TPA;
-  const int gshift = (threadIdx.x & 63) - lane;  // group's first lane within the wavefront
-  const unsigned long long below = (1ull << lane) - 1ull;
-  const double4 xi = x[i];
-  const int ti = static_cast<int>(xi.w);
-  const unsigned b = bkey[i];
-  const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
-  int ix, iy, iz;
-  bin_coords(a, (int)b, ix, iy, iz);
-  int *row = nbr + (size_t)i * a.maxneigh;
-  int cnt = 0;
-  for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
-    int jz = iz + dz, sz = 0;
-    if (jz < 0) {
-      if (!a.periodic[2]) continue;
-      jz += nbz;
-      sz = -1;
-    } else if (jz >= nbz) {
-      if (!a.periodic[2]) continue;
-      jz -= nbz;
-      sz = 1;
-    }
-    for (int dy = -a.S[1]; dy <= a.S[1]; dy++) {
-      int jy = iy + dy, sy = 0;
-      if (jy < 0) {
-        if (!a.periodic[1]) continue;
-        jy += nby;
-        sy = -1;
-      } else if (jy >= nby) {
-        if (!a.periodic[1]) continue;
-        jy -= nby;
-        sy = 1;
-      }
-      const int rowbase = bin_row(a, jy, jz) * nbx;
-      const int x0 = ix - a.S[0], x1 = ix + a.S[0];
-      const double shy = sy * a.prd[1], shz = sz * a.prd[2];
-      // segments in stencil order: wrapped low part, main part, wrapped high part
-      for (int s = 0; s < 3; s++) {
-        int lo, hi, sx;
-        if (s == 0) {
-          if (!(x0 < 0 && a.periodic[0])) continue;
-          lo = max(x0 + nbx, 0);
-          hi = nbx - 1;
-          sx = -1;
-        } else if (s == 1) {
-          lo = max(x0, 0);
-          hi = min(x1, nbx - 1);
-          sx = 0;
-        } else {
-          if (!(x1 >= nbx && a.periodic[0])) continue;
-          lo = 0;
-          hi = min(x1 - nbx, nbx - 1);
-          sx = 1;
-        }
-        if (lo > hi) continue;
-        const double shx = sx * a.prd[0];
-        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-        const int j0 = bin_start[rowbase + lo];
-        const int j1 = bin_start[rowbase + hi + 1];
-        for (int base = j0; base < j1; base += TPA) {
-          const int j = base + lane;
-          bool keep = false;
-          if (j < j1 && !(code == MDX_NOSHIFT && j == i)) {
-            const double4 xj = x[j];
-            const int tj = static_cast<int>(xj.w);
-            // ghost image coordinate as CommBrick::borders forms it
-            const double g0 = xj.x + shx, g1 = xj.y + shy, g2 = xj.z + shz;
-            const double c = s_cut[ti * a.ntypes1 + tj];
-            keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
-            if (code != MDX_NOSHIFT) {
-              // the reference evaluates an owned-ghost pair once, from one side
-              const bool keep_b = rsq_exact(xj.x, xj.y, xj.z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
-              if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
-            }
-          }
-          const unsigned long long m = (__ballot(keep) >> gshift) & ((TPA == 64) ? ~0ull : ((1ull << TPA) - 1ull));
-          if (keep) {
-            const int pos = cnt + __popcll(m & below);
-            if (pos < a.maxneigh) row[pos] = j | (code << MDX_IDX_BITS);
-          }
-          cnt += __popcll(m);
-        }
-      }
-    }
-  }
-  if (lane == 0) {
-    numneigh[i] = cnt;
-    atomicMax(&dflags[0], cnt);
-    atomicAdd(nsum, (unsigned long long)cnt);
-  }
-}
-
-// One lane per atom (the default build).  A lane walks ~300 candidates, so
+// One lane per atom.  A lane walks ~300 candidates, so
 // one gather in flight at a time leaves it latency bound: candidates are
 // taken four at a time (four gathers in flight), and accepted entries are
 // stored four at a time as one 16-B store into the row.  Candidate order,
-// keep test and tie rule are those of k_build_full, so the rows are
-// identical entry for entry.  Q: candidates are read from the fixed-point
+// keep test and tie rule do not depend on Q or ONE, so every variant gives
+// the same rows entry for entry.  Q: candidates are read from the fixed-point
 // mirror (16 B instead of 32) and tested in FP32 (see kBuildBand); that
 // variant is held to 5 wavefronts per SIMD (96 VGPRs), which measured 9%
 // faster on 2 M atoms than the 4 its register count would otherwise give.
@@ -535,92 +428,6 @@ __global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) v
   atomicAdd(nsum, (unsigned long long)cnt);
 }
 
-// Mask list: the same candidate enumeration and keep test as k_build_lane,
-// one bit per candidate (self included, as a zero bit) in enumeration
-// order; words are stored four at a time.  dflags[2] gets the largest
-// number of words an atom needs (capacity regrowth).
-__global__ MDX_BUILD_BOUNDS void k_build_mask(NeighArgs a, const double4 *__restrict__ x,
-                                                   const unsigned *__restrict__ bkey,
-                                                   const int *__restrict__ bin_start, unsigned *__restrict__ mask,
-                                                   int mwords, int *__restrict__ numneigh, int *__restrict__ dflags,
-                                                   unsigned long long *__restrict__ nsum) {
-  __shared__ double s_cut[MDX_NT1 * MDX_NT1];
-  stage_cutneighsq(a, s_cut);
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const double4 xi = x[i];
-  const int ti = static_cast<int>(xi.w);
-  const unsigned b = bkey[i];
-  int ix, iy, iz;
-  bin_coords(a, (int)b, ix, iy, iz);
-  unsigned *row = mask + (size_t)i * mwords;  // mwords is a multiple of 4: 16-B aligned rows
-  int ncand = 0, cnt = 0, nw = 0;
-  unsigned w = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-  const int nrow = (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
-  for (int r = 0; r < nrow; r++) {
-    RowSegs rs;
-    if (!row_segments(a, bin_start, ix, iy, iz, r, rs)) continue;
-#pragma unroll
-    for (int s = 0; s < 3; s++) {
-      const int j0 = rs.j0[s], j1 = rs.j1[s], code = rs.code[s];
-      if (j0 >= j1) continue;
-      const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
-      const double shx = sx * a.prd[0], shy = sy * a.prd[1], shz = sz * a.prd[2];
-      for (int base = j0; base < j1; base += 4) {
-        double4 xj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) xj[u] = x[min(base + u, j1 - 1)];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int j = base + u;
-          if (j >= j1) continue;
-          bool keep = false;
-          if (!(code == MDX_NOSHIFT && j == i)) {
-            const int tj = static_cast<int>(xj[u].w);
-            const double g0 = xj[u].x + shx, g1 = xj[u].y + shy, g2 = xj[u].z + shz;
-            const double c = s_cut[ti * a.ntypes1 + tj];
-            keep = rsq_exact(xi.x, xi.y, xi.z, g0, g1, g2) <= c;
-            if (code != MDX_NOSHIFT) {
-              const bool keep_b = rsq_exact(xj[u].x, xj[u].y, xj[u].z, xi.x - shx, xi.y - shy, xi.z - shz) <= c;
-              if (keep_b != keep && !ref_owner_side(a.ref, xi.x, xi.y, xi.z, g0, g1, g2)) keep = keep_b;
-            }
-          }
-          w |= (keep ? 1u : 0u) << (ncand & 31);
-          cnt += keep ? 1 : 0;
-          ncand++;
-          if ((ncand & 31) == 0) {
-            const int slot = nw & 3;
-            q0 = slot == 0 ? w : q0;
-            q1 = slot == 1 ? w : q1;
-            q2 = slot == 2 ? w : q2;
-            q3 = slot == 3 ? w : q3;
-            nw++;
-            if (slot == 3 && nw <= mwords) *reinterpret_cast<uint4 *>(row + nw - 4) = make_uint4(q0, q1, q2, q3);
-            w = 0;
-          }
-        }
-      }
-    }
-  }
-  if (ncand & 31) {  // the last, partial word
-    const int slot = nw & 3;
-    q0 = slot == 0 ? w : q0;
-    q1 = slot == 1 ? w : q1;
-    q2 = slot == 2 ? w : q2;
-    q3 = slot == 3 ? w : q3;
-    nw++;
-    if (slot == 3 && nw <= mwords) *reinterpret_cast<uint4 *>(row + nw - 4) = make_uint4(q0, q1, q2, q3);
-  }
-  const int tail = nw & 3, t0 = nw - tail;
-  if (tail > 0 && t0 < mwords) row[t0] = q0;
-  if (tail > 1 && t0 + 1 < mwords) row[t0 + 1] = q1;
-  if (tail > 2 && t0 + 2 < mwords) row[t0 + 2] = q2;
-  numneigh[i] = cnt;
-  atomicMax(&dflags[0], cnt);
-  atomicMax(&dflags[2], nw);
-  atomicAdd(nsum, (unsigned long long)cnt);
-}
-
 // The fixed-point build needs the mirror (kept for mixed-precision gathers,
 // written by k_permute just before the build) and a stencil that never holds
 // one atom under two periodic images.
@@ -631,26 +438,20 @@ bool q_build_ok(const mdx_ctx *c) {
   return true;
 }
 
-template <int TPA>
 void launch_build(mdx_ctx *c) {
   const int B = 256;
-  const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
-  if constexpr (TPA == 1) {
-    QStep qs{{(float)c->qs.step[0], (float)c->qs.step[1], (float)c->qs.step[2]}};
-    unsigned long long *nsum = reinterpret_cast<unsigned long long *>(c->dflags + 8);
-    const bool q = q_build_ok(c), one = c->ntypes == 1;
-    auto go = [&](auto kernel, const int4 *xq) {
-      hipLaunchKernelGGL(kernel, dim3(G), dim3(B), 0, c->stream, c->na, c->x, xq, qs, c->bkey_s, c->bin_start, c->nbr,
-                         c->numneigh, c->dflags, nsum);
-    };
-    if (q && one) go(k_build_lane<true, true>, c->xq);
-    else if (q) go(k_build_lane<true, false>, c->xq);
-    else if (one) go(k_build_lane<false, true>, nullptr);
-    else go(k_build_lane<false, false>, nullptr);
-  } else {
-    hipLaunchKernelGGL((k_build_full<TPA>), dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start,
-                       c->nbr, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
-  }
+  const int G = xcd_grid(((long)c->n + B - 1) / B);
+  QStep qs{{(float)c->qs.step[0], (float)c->qs.step[1], (float)c->qs.step[2]}};
+  unsigned long long *nsum = reinterpret_cast<unsigned long long *>(c->dflags + 8);
+  const bool q = q_build_ok(c), one = c->ntypes == 1;
+  auto go = [&](auto kernel, const int4 *xq) {
+    hipLaunchKernelGGL(kernel, dim3(G), dim3(B), 0, c->stream, c->na, c->x, xq, qs, c->bkey_s, c->bin_start, c->nbr,
+                       c->numneigh, c->dflags, nsum);
+  };
+  if (q && one) go(k_build_lane<true, true>, c->xq);
+  else if (q) go(k_build_lane<true, false>, c->xq);
+  else if (one) go(k_build_lane<false, true>, nullptr);
+  else go(k_build_lane<false, false>, nullptr);
 }
 
 }  // namespace
@@ -707,38 +508,18 @@ int neigh_setup_grid(mdx_ctx *c) {
   return MDX_OK;
 }
 
-// (re)allocates the list of the current layout with `cap` entries per atom
+// (re)allocates the list with `cap` entries per atom
 int alloc_list(mdx_ctx *c, int cap) {
   // a multiple of 4 entries: rows start 16-B aligned (k_build_lane's stores)
   cap = (std::max(cap, 16) + 3) / 4 * 4;
   (void)hipFree(c->nbr);
-  (void)hipFree(c->nbr16);
   c->nbr = nullptr;
-  c->nbr16 = nullptr;
-  hipError_t e;
-  if (c->list_mode == MDX_LIST_BRICK)
-    e = hipMalloc(&c->nbr16, sizeof(unsigned short) * (size_t)cap * c->npad);
-  else
-    e = hipMalloc(&c->nbr, sizeof(int) * (size_t)cap * c->npad);
-  if (e != hipSuccess) {
+  if (hipMalloc(&c->nbr, sizeof(int) * (size_t)cap * c->npad) != hipSuccess) {
     set_error("Insufficient memory on accelerator (neighbour list)");
     return MDX_ERR_NOMEM;
   }
   c->maxneigh = cap;
   c->nbr_elems = (long)cap * c->npad;
-  return MDX_OK;
-}
-
-int alloc_mask(mdx_ctx *c, int words) {
-  words = (std::max(words, 4) + 3) / 4 * 4;  // 16-B aligned rows
-  (void)hipFree(c->nmask);
-  c->nmask = nullptr;
-  c->mwords = 0;
-  if (hipMalloc(&c->nmask, sizeof(unsigned) * (size_t)words * c->npad) != hipSuccess) {
-    set_error("Insufficient memory on accelerator (neighbour masks)");
-    return MDX_ERR_NOMEM;
-  }
-  c->mwords = words;
   return MDX_OK;
 }
 
@@ -786,21 +567,7 @@ int neigh_build(mdx_ctx *c) {
   std::swap(c->tag, c->tagb);
   std::swap(c->orig, c->origb);
   std::swap(c->image, c->imageb);
-  const bool brick = c->list_mode == MDX_LIST_BRICK;
-  const bool mask = c->list_mode == MDX_LIST_MASK;
-  if (brick) {
-    int rc = brick_prepare(c);
-    if (rc) return rc;
-  }
-  if (mask) {
-    if (!c->nmask) {
-      // first guess: the stencil's mean candidate count with 50% margin
-      const int nst = (2 * a.S[0] + 1) * (2 * a.S[1] + 1) * (2 * a.S[2] + 1);
-      const double per_bin = (double)c->n / std::max(1, c->nbins);
-      int rc = alloc_mask(c, (int)(1.5 * nst * per_bin / 32.0) + 4);
-      if (rc) return rc;
-    }
-  } else if (!(brick ? (void *)c->nbr16 : (void *)c->nbr)) {
+  if (!c->nbr) {
     int rc = alloc_list(c, c->maxneigh);
     if (rc) return rc;
   }
@@ -810,35 +577,13 @@ int neigh_build(mdx_ctx *c) {
     if ((e = hipMemsetAsync(c->dflags, 0, 3 * sizeof(int), c->stream)) != hipSuccess)
       return hip_fail(e, "memset flags");
     if ((e = hipMemsetAsync(c->dflags + 8, 0, 8, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
-    if (brick) {
-      int rc = brick_build_list(c);
-      if (rc) return rc;
-    } else if (mask) {
-      const int B = 256, G = xcd_grid(((long)c->n + B - 1) / B);
-      hipLaunchKernelGGL(k_build_mask, dim3(G), dim3(B), 0, c->stream, c->na, c->x, c->bkey_s, c->bin_start, c->nmask,
-                         c->mwords, c->numneigh, c->dflags, reinterpret_cast<unsigned long long *>(c->dflags + 8));
-    } else {
-      switch (c->launch.build_tpa) {
-        case 1: launch_build<1>(c); break;
-        case 4: launch_build<4>(c); break;
-        case 8: launch_build<8>(c); break;
-        case 16: launch_build<16>(c); break;
-        case 32: launch_build<32>(c); break;
-        default:
-          set_error("build threads per atom must be one of 1, 4, 8, 16, 32");
-          return MDX_ERR_ARG;
-      }
-    }
+    launch_build(c);
     int flags[16];
     if ((e = hipMemcpyAsync(flags, c->dflags, 64, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
       return hip_fail(e, "read build flags");
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "neighbour build");
     if (flags[4]) {
       set_error("Lost atoms: an atom left a non-periodic box");
-      return MDX_ERR_ARG;
-    }
-    if (flags[5]) {
-      set_error("brick staging exceeded its LDS sizing (internal error)");
       return MDX_ERR_ARG;
     }
     const int mx = flags[0];
@@ -850,12 +595,6 @@ int neigh_build(mdx_ctx *c) {
     memcpy(&tot, &flags[8], sizeof(tot));
     c->total_neighbors = (long)tot;
     c->max_neighbors = mx;
-    if (mask) {
-      if (flags[2] <= c->mwords) break;
-      int rc = alloc_mask(c, flags[2] + flags[2] / 4 + 4);
-      if (rc) return rc;
-      continue;
-    }
     if (mx <= c->maxneigh) break;
     // regrow: the accelerator library's max_nbors resize (lal_neighbor.cpp:582-595)
     int cap = ((mx + mx / 4 + 15) / 16) * 16;
@@ -863,7 +602,7 @@ int neigh_build(mdx_ctx *c) {
     int rc = alloc_list(c, cap);
     if (rc) return rc;
   }
-  if (!mask && c->max_neighbors > c->maxneigh) {
+  if (c->max_neighbors > c->maxneigh) {
     set_error("neighbour list capacity could not be grown to fit");
     return MDX_ERR_OVERFLOW;
   }

@@ -566,356 +566,6 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
                               partials);
 }
 
-// ------------------------------------------------------------------ mask lists
-// The list holds one bit per stencil candidate (MDX_LIST_MASK).  The TPA
-// lanes of an atom take the stencil rows r = k*TPA + lane; a group prefix
-// scan of the rows' candidate counts gives each row its first bit, and a
-// lane walks the set bits of its rows' segments, four neighbours per batch
-// (four gathers in flight).  Row order, segment order and bit order are the
-// list build's, so the pair set is exactly the index list's.
-
-// exclusive prefix of v over the TPA lanes of a group, and the group total
-template <int TPA>
-__device__ __forceinline__ int group_excl_scan(int v, int lane, int &total) {
-  int incl = v;
-#pragma unroll
-  for (int d = 1; d < TPA; d <<= 1) {
-    const int t = __shfl_up(incl, d, TPA);
-    if (lane >= d) incl += t;
-  }
-  total = __shfl(incl, TPA - 1, TPA);
-  return incl - v;
-}
-
-// calls body(jj, nv, code) for batches of up to four set bits; unused slots
-// of jj hold `self` (a valid index to gather)
-template <int TPA, typename Body>
-__device__ __forceinline__ void mask_neighbors(const NeighArgs &na, const int *__restrict__ bin_start, unsigned key,
-                                               const unsigned *__restrict__ mrow, int lane, int self, Body &&body) {
-  int ix, iy, iz;
-  bin_coords(na, (int)key, ix, iy, iz);
-  const int nrow = (2 * na.S[1] + 1) * (2 * na.S[2] + 1);
-  int base = 0;
-  for (int k = 0; k < nrow; k += TPA) {  // the same trip count on every lane of the group
-    const int r = k + lane;
-    RowSegs rs;
-    int cnt = 0;
-    if (r < nrow) cnt = row_segments(na, bin_start, ix, iy, iz, r, rs);
-    int total;
-    int off = base + group_excl_scan<TPA>(cnt, lane, total);
-    base += total;
-    if (cnt == 0) continue;
-#pragma unroll
-    for (int sgi = 0; sgi < 3; sgi++) {
-      const int j0 = rs.j0[sgi], nseg = rs.j1[sgi] - j0, code = rs.code[sgi];
-      for (int t = 0; t < nseg; t += 32) {
-        const int len = min(32, nseg - t), pos = off + t, wi = pos >> 5, sh = pos & 31;
-        unsigned long long v = mrow[wi];
-        if (sh + len > 32) v |= (unsigned long long)mrow[wi + 1] << 32;
-        unsigned bits = (unsigned)(v >> sh) & (len == 32 ? 0xffffffffu : ((1u << len) - 1u));
-        while (bits) {
-          int jj[4];
-          int nv = 0;
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            jj[u] = self;
-            if (bits) {
-              jj[u] = j0 + t + __builtin_ctz(bits);
-              bits &= bits - 1u;
-              nv++;
-            }
-          }
-          body(jj, nv, code);
-        }
-      }
-      off += nseg > 0 ? nseg : 0;
-    }
-  }
-}
-
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-__global__ __launch_bounds__(256) void k_lj_mask(int n, int npad, int nt1, NeighArgs na, const double4 *__restrict__ x,
-                                                const int4 *__restrict__ xq, QScale qs,
-                                                const int *__restrict__ bin_start, const unsigned *__restrict__ bkey,
-                                                const unsigned *__restrict__ mask, int mwords,
-                                                const LJTab *__restrict__ tab, double4 *__restrict__ f,
-                                                double *__restrict__ vatom, double *__restrict__ partials,
-                                                IntegArgs ig) {
-  __shared__ double s_cutsq[kNT2], s_lj1[kNT2], s_lj2[kNT2], s_lj3[kNT2], s_lj4[kNT2], s_off[kNT2];
-  __shared__ double s_clo[Q ? kNT2 : 1], s_chi[Q ? kNT2 : 1];
-  __shared__ float f_lj1[kNT2], f_lj2[kNT2], f_lj3[kNT2], f_lj4[kNT2], f_off[kNT2];
-  const int nt2 = nt1 * nt1;
-  for (int q = threadIdx.x; q < nt2; q += blockDim.x) {
-    if (Q) {
-      s_clo[q] = tab->cutsq[q] * (1.0 - kBand);
-      s_chi[q] = tab->cutsq[q] * (1.0 + kBand);
-    }
-    s_cutsq[q] = tab->cutsq[q];
-    s_lj1[q] = tab->lj1[q];
-    s_lj2[q] = tab->lj2[q];
-    s_lj3[q] = tab->lj3[q];
-    s_lj4[q] = tab->lj4[q];
-    s_off[q] = tab->offset[q];
-    f_lj1[q] = (float)tab->lj1[q];
-    f_lj2[q] = (float)tab->lj2[q];
-    f_lj3[q] = (float)tab->lj3[q];
-    f_lj4[q] = (float)tab->lj4[q];
-    f_off[q] = (float)tab->offset[q];
-  }
-  __syncthreads();
-  const double px = na.prd[0], py = na.prd[1], pz = na.prd[2];
-  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
-  const int lane = threadIdx.x % TPA;
-  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-  double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
-  double4 vpre = make_double4(0.0, 0.0, 0.0, 0.0), hpre = vpre;
-  if (INTEG && i < n && lane == 0) {
-    vpre = ig.v[i];
-    if (ig.check) hpre = ig.xhold[i];
-  }
-  if (i < n) {
-    xi = x[i];
-    const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
-    const int row = static_cast<int>(xi.w) * nt1;
-    mask_neighbors<TPA>(na, bin_start, bkey[i], mask + (size_t)i * mwords, lane, i,
-                        [&](const int (&jj)[4], int nv, int code) {
-      double4 xj[Q ? 1 : 4];
-      int4 qj[Q ? 4 : 1];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (Q) qj[u] = xq[jj[u]];
-        else xj[Q ? 0 : u] = x[jj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (u >= nv) continue;
-        double delx, dely, delz, rsq;
-        int t;
-        bool inside;
-        if (Q) {
-          delx = qdelta(qi.x, qj[u].x, qs.step[0]);
-          dely = qdelta(qi.y, qj[u].y, qs.step[1]);
-          delz = qdelta(qi.z, qj[u].z, qs.step[2]);
-          rsq = delx * delx + dely * dely + delz * delz;
-          t = row + qj[u].w;
-          inside = rsq < s_clo[t];
-          if (!inside && rsq < s_chi[t]) {
-            double ex, ey, ez;
-            pair_delta(xi, x[jj[u]], code, px, py, pz, ex, ey, ez);
-            inside = ex * ex + ey * ey + ez * ez < s_cutsq[t];
-          }
-        } else {
-          pair_delta(xi, xj[Q ? 0 : u], code, px, py, pz, delx, dely, delz);
-          rsq = delx * delx + dely * dely + delz * delz;
-          t = row + static_cast<int>(xj[Q ? 0 : u].w);
-          inside = rsq < s_cutsq[t];
-        }
-        if (inside) {
-          double fpair, evdwl = 0.0;
-          if (MIXED) {
-            const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
-            const float r6inv = r2inv * r2inv * r2inv;
-            const float forcelj = r6inv * (f_lj1[t] * r6inv - f_lj2[t]);
-            fpair = (double)(forcelj * r2inv);
-            if (EV) evdwl = (double)(r6inv * (f_lj3[t] * r6inv - f_lj4[t]) - f_off[t]);
-          } else {
-            const double r2inv = 1.0 / rsq;
-            const double r6inv = r2inv * r2inv * r2inv;
-            const double forcelj = r6inv * (s_lj1[t] * r6inv - s_lj2[t]);
-            fpair = forcelj * r2inv;
-            if (EV) evdwl = r6inv * (s_lj3[t] * r6inv - s_lj4[t]) - s_off[t];
-          }
-          fx += delx * fpair;
-          fy += dely * fpair;
-          fz += delz * fpair;
-          if (EV) {
-            e += 0.5 * evdwl;
-            const double hf = 0.5 * fpair;
-            w[0] += delx * delx * hf;
-            w[1] += dely * dely * hf;
-            w[2] += delz * delz * hf;
-            w[3] += delx * dely * hf;
-            w[4] += delx * delz * hf;
-            w[5] += dely * delz * hf;
-          }
-        }
-      }
-    });
-  }
-  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
-                              partials);
-}
-
-template <bool MIXED, bool EV, int TPA, bool Q>
-__global__ __launch_bounds__(256) void k_eam_rho_mask(int n, EAMArgs a, NeighArgs na, const double4 *__restrict__ x,
-                                                     const int4 *__restrict__ xq, QScale qs,
-                                                     const int *__restrict__ bin_start,
-                                                     const unsigned *__restrict__ bkey,
-                                                     const unsigned *__restrict__ mask, int mwords,
-                                                     double *__restrict__ rho_out, double *__restrict__ fp_out,
-                                                     double *__restrict__ eemb) {
-  const double px = na.prd[0], py = na.prd[1], pz = na.prd[2];
-  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
-  const int lane = threadIdx.x % TPA;
-  double rho = 0.0;
-  int ti = 0;
-  const double clo = a.cutforcesq * (1.0 - kBand), chi = a.cutforcesq * (1.0 + kBand);
-  if (i < n) {
-    const double4 xi = x[i];
-    const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
-    ti = static_cast<int>(xi.w);
-    mask_neighbors<TPA>(na, bin_start, bkey[i], mask + (size_t)i * mwords, lane, i,
-                        [&](const int (&jj)[4], int nv, int code) {
-      double4 xj[Q ? 1 : 4];
-      int4 qj[Q ? 4 : 1];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (Q) qj[u] = xq[jj[u]];
-        else xj[Q ? 0 : u] = x[jj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (u >= nv) continue;
-        double delx, dely, delz, rsq;
-        const int entry = jj[u] | (code << MDX_IDX_BITS);
-        if (pair_inside<Q>(xi, qi, x, entry, xj[Q ? 0 : u], qj[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
-                           delx, dely, delz, rsq)) {
-          const int tj = Q ? qj[Q ? u : 0].w : static_cast<int>(xj[Q ? 0 : u].w);
-          rho += eam_rho_term<MIXED>(a, a.type2rhor[tj * a.ntypes1 + ti], rsq);
-        }
-      }
-    });
-  }
-  rho = group_sum<TPA>(rho);
-  if (i < n && lane == 0) {
-    // fp pass (pair_eam.cpp:213-228), FP64 in both modes
-    double p = rho * a.rdrho + 1.0;
-    int m = static_cast<int>(p);
-    m = max(1, min(m, a.nrho - 1));
-    p -= m;
-    p = fmin(p, 1.0);
-    const double *c = a.frho + ((size_t)a.type2frho[ti] * (a.nrho + 1) + m) * 7;
-    const double fpi = (c[0] * p + c[1]) * p + c[2];
-    rho_out[i] = rho;
-    fp_out[i] = fpi;
-    if (EV) {
-      double phi = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
-      if (rho > a.rhomax) phi += fpi * (rho - a.rhomax);
-      eemb[i] = phi;
-    }
-  }
-}
-
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-__global__ __launch_bounds__(256) void k_eam_force_mask(int n, int npad, EAMArgs a, NeighArgs na,
-                                                       const double4 *__restrict__ x, const int4 *__restrict__ xq,
-                                                       QScale qs, const int *__restrict__ bin_start,
-                                                       const unsigned *__restrict__ bkey,
-                                                       const unsigned *__restrict__ mask, int mwords,
-                                                       const double *__restrict__ fp, const double *__restrict__ eemb,
-                                                       double4 *__restrict__ f, double *__restrict__ vatom,
-                                                       double *__restrict__ partials, IntegArgs ig) {
-  const double px = na.prd[0], py = na.prd[1], pz = na.prd[2];
-  const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
-  const int lane = threadIdx.x % TPA;
-  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-  double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
-  double4 vpre = make_double4(0.0, 0.0, 0.0, 0.0), hpre = vpre;
-  if (INTEG && i < n && lane == 0) {
-    vpre = ig.v[i];
-    if (ig.check) hpre = ig.xhold[i];
-  }
-  const double clo = a.cutforcesq * (1.0 - kBand), chi = a.cutforcesq * (1.0 + kBand);
-  if (i < n) {
-    xi = x[i];
-    const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
-    const int ti = static_cast<int>(xi.w);
-    const double fpi = fp[i];
-    mask_neighbors<TPA>(na, bin_start, bkey[i], mask + (size_t)i * mwords, lane, i,
-                        [&](const int (&jj)[4], int nv, int code) {
-      double4 xj[Q ? 1 : 4];
-      int4 qj[Q ? 4 : 1];
-      double fpj[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (Q) qj[u] = xq[jj[u]];
-        else xj[Q ? 0 : u] = x[jj[u]];
-        fpj[u] = fp[jj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (u >= nv) continue;
-        double delx, dely, delz, rsq;
-        const int entry = jj[u] | (code << MDX_IDX_BITS);
-        if (!pair_inside<Q>(xi, qi, x, entry, xj[Q ? 0 : u], qj[Q ? u : 0], qs, px, py, pz, a.cutforcesq, clo, chi,
-                            delx, dely, delz, rsq))
-          continue;
-        const int tj = Q ? qj[Q ? u : 0].w : static_cast<int>(xj[Q ? 0 : u].w);
-        const int tij = ti * a.ntypes1 + tj, tji = tj * a.ntypes1 + ti;
-        double fpair, phi;
-        eam_pair_term<MIXED>(a, a.type2rhor[tij], a.type2rhor[tji], a.type2z2r[tij], rsq, fpi, fpj[u], fpair, phi);
-        fx += delx * fpair;
-        fy += dely * fpair;
-        fz += delz * fpair;
-        if (EV) {
-          e += 0.5 * phi;
-          const double hf = 0.5 * fpair;
-          w[0] += delx * delx * hf;
-          w[1] += dely * dely * hf;
-          w[2] += delz * delz * hf;
-          w[3] += delx * dely * hf;
-          w[4] += delx * delz * hf;
-          w[5] += dely * delz * hf;
-        }
-      }
-    });
-    if (EV && lane == 0) e += eemb[i];
-  }
-  finish_atom<TPA, EV, INTEG>(i, i < n && lane == 0, npad, xi, fx, fy, fz, e, w, ig, vpre, hpre, f, vatom,
-                              partials);
-}
-
-template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q>
-void launch_mask(mdx_ctx *c, int B, const IntegArgs &ig) {
-  const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
-  if (c->pair_style == MDX_PAIR_LJ) {
-    prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-    hipLaunchKernelGGL((k_lj_mask<MIXED, EV, TPA, INTEG, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
-                       c->ntypes + 1, c->na, c->x, c->xq, c->qs, c->bin_start, c->bkey_s, c->nmask, c->mwords,
-                       c->lj_dev, c->f, c->vatom, c->partials, ig);
-    prof_end(c);
-  } else {
-    prof_begin(c, MDX_K_EAM_RHO);
-    hipLaunchKernelGGL((k_eam_rho_mask<MIXED, EV, TPA, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->eam, c->na, c->x,
-                       c->xq, c->qs, c->bin_start, c->bkey_s, c->nmask, c->mwords, c->rho, c->fp, c->eemb);
-    prof_end(c);
-    prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
-    hipLaunchKernelGGL((k_eam_force_mask<MIXED, EV, TPA, INTEG, Q>), dim3(G), dim3(B), 0, c->stream, c->n, c->npad,
-                       c->eam, c->na, c->x, c->xq, c->qs, c->bin_start, c->bkey_s, c->nmask, c->mwords, c->fp,
-                       c->eemb, c->f, c->vatom, c->partials, ig);
-    prof_end(c);
-  }
-  c->last_grid = G;
-}
-
-template <bool MIXED, bool EV, bool INTEG, bool Q>
-int dispatch_mask(mdx_ctx *c, int B, const IntegArgs &ig) {
-  switch (c->launch.tpa) {
-    case 1: launch_mask<MIXED, EV, 1, INTEG, Q>(c, B, ig); break;
-    case 2: launch_mask<MIXED, EV, 2, INTEG, Q>(c, B, ig); break;
-    case 4: launch_mask<MIXED, EV, 4, INTEG, Q>(c, B, ig); break;
-    case 8: launch_mask<MIXED, EV, 8, INTEG, Q>(c, B, ig); break;
-    case 16: launch_mask<MIXED, EV, 16, INTEG, Q>(c, B, ig); break;
-    case 32: launch_mask<MIXED, EV, 32, INTEG, Q>(c, B, ig); break;
-    default:
-      set_error("threads per atom must be one of 1, 2, 4, 8, 16, 32");
-      return MDX_ERR_ARG;
-  }
-  return MDX_OK;
-}
-
 // Resident wavefronts per SIMD of the fused force kernels, held by giving
 // each workgroup enough LDS that no more workgroups fit on a CU (160 KiB of
 // LDS, four SIMDs).  The mixed fused lj/cut kernel gets no cap: its 66
@@ -998,7 +648,6 @@ void launch_eam(mdx_ctx *c, int B, const IntegArgs &ig) {
 
 template <bool MIXED, bool EV, bool INTEG, bool Q>
 int dispatch_tpa(mdx_ctx *c, int B, const IntegArgs &ig) {
-  if (c->list_mode == MDX_LIST_MASK) return dispatch_mask<MIXED, EV, INTEG, Q>(c, B, ig);
   const bool lj = c->pair_style == MDX_PAIR_LJ;
   switch (c->launch.tpa) {
     case 1: lj ? launch_lj<MIXED, EV, 1, INTEG, Q>(c, B, ig) : launch_eam<MIXED, EV, 1, INTEG, Q>(c, B, ig); break;
@@ -1030,7 +679,7 @@ int dispatch_prec(mdx_ctx *c, int ev, const IntegArgs &ig) {
 
 }  // namespace
 
-// Pair::compute on the device.  With `ig` (gather layout only) the force
+// Pair::compute on the device.  With `ig` the force
 // kernel also runs fix nve's final integrate of this step and, if ig->next,
 // the initial integrate of the next one (positions go to ig->xn).  With ev
 // the reduction's slots are {eng_vdwl, virial[6], sum m v^2 (fused only)}.
@@ -1043,15 +692,9 @@ int pair_compute(mdx_ctx *c, int ev, const IntegArgs *ig) {
     set_error("no pair style set");
     return MDX_ERR_ARG;
   }
-  int rc, nslots = 8;
-  if (c->list_mode == MDX_LIST_BRICK) {
-    if (ig) {
-      set_error("fused integration needs the gather list layout");
-      return MDX_ERR_ARG;
-    }
-    rc = brick_pair(c, ev);
-    nslots = 7;
-  } else if (ig) {
+  int rc;
+  const int nslots = 8;
+  if (ig) {
     rc = dispatch_prec<true>(c, ev, *ig);
   } else {
     IntegArgs none{};

