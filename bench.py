@@ -46,7 +46,7 @@ def lj_inputs(ncell):
     t = np.ones(len(x), dtype=np.int32)
     mass = np.array([0.0, 1.0])
     v = mdx.velocity_create(x, t, mass, 1.44, 87287, dist="gaussian", loop="geom")
-    return dict(lo=lo, hi=hi, x=x, t=t, mass=mass, v=v)
+    return dict(lo=lo, hi=hi, x=x, t=t, mass=mass, v=v, ncell=ncell)
 
 
 def eam_inputs(ncell, table):
@@ -55,7 +55,7 @@ def eam_inputs(ncell, table):
     t = np.ones(len(x), dtype=np.int32)
     mass = np.array([0.0, 63.55])
     v = mdx.velocity_create(x, t, mass, 1600.0, 376847, dist="gaussian", loop="geom", units="metal")
-    return dict(lo=lo, hi=hi, x=x, t=t, mass=mass, v=v, table=table)
+    return dict(lo=lo, hi=hi, x=x, t=t, mass=mass, v=v, table=table, ncell=ncell)
 
 
 def make_ctx(kind, inp, precision):
@@ -74,8 +74,10 @@ def make_ctx(kind, inp, precision):
 
 
 def run_case(kind, inp, precision, steps, warmup):
-    """Setup, warmup, then exactly `steps` timed steps; returns timing,
-    kernel times and thermo sums at the start/end of the timed region."""
+    """Setup, warmup, then exactly `steps` timed steps with nothing else on
+    the host path (no event records); then the same number of steps again
+    with the library's HIP-event profiling on, for the per-kernel times.
+    Returns the timing of the first, the kernel times of the second."""
     from lammps_amd import mdx
     units = "lj" if kind == "lj" else "metal"
     dt = 0.005
@@ -86,22 +88,31 @@ def run_case(kind, inp, precision, steps, warmup):
     if warmup:
         ctx.run(warmup, dt, ftm2v, 0)
     st0 = ctx.stats()
-    ctx.profiling(True)
     t0 = time.perf_counter()
     rows = ctx.run(steps, dt, ftm2v, steps)
     t1 = time.perf_counter()
     st = ctx.stats()
-    ctx.profiling(False)
     n = len(inp["t"])
     secs = t1 - t0
+    # profiled repeat: kernel durations from HIP events on the library's stream
+    ctx.profiling(True)
+    tp0 = time.perf_counter()
+    ctx.run(steps, dt, ftm2v, steps)
+    tp1 = time.perf_counter()
+    sp = ctx.stats()
+    ctx.profiling(False)
     res = dict(n=n, seconds=secs, ms_per_step=1e3 * secs / steps, value=n * steps / secs / 1e6,
                builds=st["ncalls"] - st0["ncalls"], total_neighbors=st["total_neighbors"],
-               max_neighbors=st["max_neighbors"], kernel_ms=st["kernel_ms"], kernel_launches=st["kernel_launches"],
+               max_neighbors=st["max_neighbors"], kernel_ms=sp["kernel_ms"], kernel_launches=sp["kernel_launches"],
+               profiled_ms_per_step=1e3 * (tp1 - tp0) / steps, profiled_builds=sp["ncalls"] - st["ncalls"],
+               kernel_sum_ms_per_step=sum(sp["kernel_ms"]) / steps,
                end_row=rows[-1][1] if rows else None, ctx=ctx, launch=launch)
     names = ("build", "lj", "eam_rho", "eam_force", "nve_initial", "nve_final", "fused")
-    log("  kernels (ms/step, launches): " + ", ".join(
-        "%s %.4f (%d)" % (nm, st["kernel_ms"][k] / steps, st["kernel_launches"][k])
-        for k, nm in enumerate(names) if st["kernel_launches"][k]))
+    log("  kernels of the profiled repeat (ms/step, launches): " + ", ".join(
+        "%s %.4f (%d)" % (nm, sp["kernel_ms"][k] / steps, sp["kernel_launches"][k])
+        for k, nm in enumerate(names) if sp["kernel_launches"][k]))
+    log("  wall %.4f ms/step unprofiled, kernel sum %.4f ms/step (profiled repeat, %d builds)"
+        % (res["ms_per_step"], res["kernel_sum_ms_per_step"], res["profiled_builds"]))
     return res
 
 
@@ -127,6 +138,24 @@ def energy_per_atom(raw, n, units):
     return (0.5 * mvv2e * raw[0] + raw[1]) / n
 
 
+def oracle_drift(kind, inp, steps):
+    """(E_tot(t) - E_tot(0))/N of the oracle (threaded build, all host
+    threads) over `steps` NVE steps from the workload's seeded state."""
+    from oracle import mdref as M
+    M.set_threads(host_threads())
+    r = _oracle_like(kind, inp, threaded=True)
+    r.thermo(max(1, steps // 10))
+    r.setup()
+    rows = r.run(steps)
+    n = r.natoms
+    del r
+    norm = 1.0 if kind == "lj" else 1.0 / n  # lj thermo energies are per atom already
+    e = [row[4] * norm for row in rows]
+    st = [row[0] for row in rows]
+    slope = float(np.polyfit(st, e, 1)[0]) if len(e) > 2 else 0.0
+    return dict(steps=steps, e0=e[0], e_end=e[-1], drift_per_atom=e[-1] - e[0], slope_per_step=slope)
+
+
 def drift_run(kind, inp, precision, steps):
     """(E_tot(t) - E_tot(0))/N over `steps` NVE steps, thermo every steps/10."""
     from lammps_amd import mdx
@@ -147,19 +176,35 @@ def force_errors(f, fref):
     return dict(max_rel=float(e.max() / s), rms_rel=float(np.sqrt(np.mean(e * e)) / s))
 
 
-def cpu_baseline_and_errors(kind, inp, ctxs, sample_steps):
-    """The FP64 oracle (test infrastructure, used here only as the checker and
-    the CPU baseline): (1) one force evaluation on the device's current
-    configuration -> max/rms force error of each device precision;
-    (2) a bounded NVE sample of the same workload on 1 host core."""
+def _oracle_like(kind, inp, threaded=False):
+    """An oracle system with the workload's commands (same seeded lattice,
+    velocities, potential and neighbour settings)."""
     from oracle import mdref as M
-    from lammps_amd import mdx
-    out = {}
-    # (1) force error on the configuration the device reached
-    ctx = ctxs["mixed"]
-    x, _, _, _ = ctx.get_atoms()
-    units = M.LJ if kind == "lj" else M.METAL
-    r = M.Ref(units)
+    r = M.Ref(M.LJ if kind == "lj" else M.METAL, threaded=threaded)
+    ncell = inp["ncell"]
+    if kind == "lj":
+        r.lattice_box(M.FCC, 0.8442, (0, 0, 0), (ncell,) * 3)
+        r.mass(1, 1.0)
+        r.velocity_create(1.44, 87287, dist="gaussian", loop="geom")
+        r.pair_lj(2.5)
+        r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+        r.neighbor(0.3, every=20, delay=0, check=True)
+    else:
+        r.lattice_box(M.FCC, 3.615, (0, 0, 0), (ncell,) * 3)
+        r.pair_eam(inp["table"])
+        r.velocity_create(1600.0, 376847, dist="gaussian", loop="geom")
+        r.neighbor(1.0, every=1, delay=5, check=True)
+    r.timestep(0.005)
+    return r
+
+
+def force_errors_vs_oracle(kind, inp, ctxs):
+    """The FP64 oracle (test infrastructure, used here only as the checker):
+    one force evaluation on the configuration the mixed run reached, and
+    each device precision's forces on the same positions."""
+    from oracle import mdref as M
+    x, _, _, _ = ctxs["mixed"].get_atoms()
+    r = M.Ref(M.LJ if kind == "lj" else M.METAL)
     r.box(inp["lo"], inp["hi"])
     r.add_atoms(x, inp["t"])
     if kind == "lj":
@@ -175,34 +220,86 @@ def cpu_baseline_and_errors(kind, inp, ctxs, sample_steps):
     r.compute(1, 1)
     t_force = time.perf_counter() - t0
     _, _, fref, _, _ = r.state()
+    out = {}
     for prec, c in ctxs.items():
         c.set_positions(x)
         c.neigh_build()  # the other context's own list belongs to its own trajectory
         out[prec] = force_errors(c.compute(0, 0)["f"], fref)
-    # (2) bounded NVE sample, same commands as the device workload
-    rs = M.Ref(units)
-    if kind == "lj":
-        rs.lattice_box(M.FCC, 0.8442, (0, 0, 0), (80, 80, 80))
-        rs.mass(1, 1.0)
-        rs.velocity_create(1.44, 87287, dist="gaussian", loop="geom")
-        rs.pair_lj(2.5)
-        rs.pair_coeff(1, 1, 1.0, 1.0, 2.5)
-        rs.neighbor(0.3, every=20, delay=0, check=True)
+    out["oracle_force_eval_s"] = t_force
+    return out
+
+
+def host_threads():
+    """The host threads this process may use: OMP_NUM_THREADS when set (the
+    GPU box sets it to its CPU share), else the CPU affinity mask."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else len(os.sched_getaffinity(0))
+
+
+# Oracle-to-reference speed ratios, measured in this repository's build
+# container on identical cores (Xeon @ 2.70 GHz, 8 cores) at the reference's
+# 32k-atom bench inputs (bench/in.lj, bench/in.eam; 100 steps) against the
+# reference's own figures there: LJ 2.57 Matom-steps/s (USER-OMP -O3, 1
+# thread), 15.1 (8 threads); EAM 0.673 (serial), 5.70 (8 threads).  The
+# oracle's threaded -O3 build: LJ 2.40 / 12.81, EAM 0.82 / 4.11 (best of 5,
+# OMP_PROC_BIND=close, OMP_PLACES=cores).  A box figure divided by the ratio
+# is the reference-equivalent rate.
+CALIBRATION = {"lj": {"1": 2.40 / 2.57, "all": 12.81 / 15.1},
+               "eam": {"1": 0.82 / 0.673, "all": 4.11 / 5.70}}
+
+
+def _child(task, kind, ncell, threads, table=None, steps=0):
+    """Runs a CPU-baseline task (cpu_baseline / oracle_drift) in a child
+    process whose OpenMP threads are bound to cores (OMP_PROC_BIND=close,
+    OMP_PLACES=cores): binding also pins the thread that loads the runtime,
+    which this process keeps for its own host work."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", task, kind, str(ncell), str(steps),
+           table or "-"]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError("CPU baseline child failed: %s\n%s" % (" ".join(cmd), r.stderr[-3000:]))
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _child_main(argv):
+    task, kind, ncell, steps, table = argv[0], argv[1], int(argv[2]), int(argv[3]), argv[4]
+    inp = dict(ncell=ncell, table=None if table == "-" else table)
+    if task == "baseline":
+        out = cpu_baseline(kind, inp)
     else:
-        rs.lattice_box(M.FCC, 3.615, (0, 0, 0), (40, 40, 40))
-        rs.pair_eam(inp["table"])
-        rs.velocity_create(1600.0, 376847, dist="gaussian", loop="geom")
-        rs.neighbor(1.0, every=1, delay=5, check=True)
-    rs.timestep(0.005)
-    rs.setup()
-    t0 = time.perf_counter()
-    rs.run(sample_steps)
-    secs = time.perf_counter() - t0
-    n = rs.natoms
-    base = dict(value=n * sample_steps / secs / 1e6, unit="Matom-steps/s", cores=1, kind="port",
-                sample="%s, %d atoms, %d NVE steps after setup (%.1f s); FP64 oracle oracle/mdref.c, gcc -O2, "
-                       "1 thread" % (kind, n, sample_steps, secs))
-    return out, base, t_force
+        out = oracle_drift(kind, inp, steps)
+    print(json.dumps(out))
+
+
+def cpu_baseline(kind, inp, budget_s=12.0):
+    """The reference's CPU path, restated by the oracle's threaded -O3 build
+    (oracle/Makefile: libmdref_omp.so), timed on this host on 1 thread and on
+    every host thread the process may use, each on a bounded NVE sample of
+    the same workload (steps chosen to fill about budget_s)."""
+    from oracle import mdref as M
+    out = {"cores_available": host_threads(), "samples": []}  # the parent passes its count in OMP_NUM_THREADS
+    for label, threads in (("1", 1), ("all", host_threads())):
+        M.set_threads(threads)
+        r = _oracle_like(kind, inp, threaded=True)
+        r.setup()
+        t0 = time.perf_counter()
+        r.run(2)  # rate probe
+        probe = max(1e-6, time.perf_counter() - t0) / 2
+        steps = int(max(2, min(200, budget_s / probe)))
+        t0 = time.perf_counter()
+        r.run(steps)
+        secs = time.perf_counter() - t0
+        n = r.natoms
+        del r
+        v = n * steps / secs / 1e6
+        out["samples"].append(dict(threads=threads, value=v, steps=steps, seconds=secs,
+                                   reference_equivalent=v / CALIBRATION[kind][label]))
+    return out
 
 
 def halo_share(G, L, reps=20):
@@ -465,6 +562,14 @@ def eam_replicas(reps=10, check_oracle=True, fd_knots=6):
     return out
 
 
+def kernel_row(name, ms, launches, bytes_per_launch):
+    """Roofline row of one kernel kind from the profiled repeat."""
+    avg = ms / max(1, launches)
+    ach = bytes_per_launch / (avg * 1e-3) / 1e9 if avg > 0 else None
+    return dict(kernel=name, avg_launch_ms=avg, launches=launches, bytes_per_launch=bytes_per_launch,
+                achieved=ach, frac=None if ach is None else ach / HBM_PEAK_GBS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -474,11 +579,14 @@ def main():
     ap.add_argument("--no-eam", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=1000)
     ap.add_argument("--no-fit", action="store_true", help="skip the force-matching workloads (KA, EAM clusters)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the per-call (PCIe) and sharded-share sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (spatial decomposition over "
               "several GPUs is described in DESIGN.md, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    nthreads = host_threads()
     from lammps_amd import mdx
     K, W = args.steps, args.warmup
 
@@ -491,7 +599,7 @@ def main():
         log("LJ %s: %.1f Matom-steps/s (%.3f ms/step)" % (prec, res[prec]["value"], res[prec]["ms_per_step"]))
     n = res["mixed"]["n"]
 
-    eam = None
+    eam, ein, eres = None, None, None
     if not args.no_eam:
         tmpdir = tempfile.mkdtemp(prefix="mdx_bench_")
         table = os.path.join(tmpdir, "synthetic_cu.eam")
@@ -505,11 +613,12 @@ def main():
         eam = dict(workload="eam fcc 40^3 = 256,000 atoms, a=3.615 A, synthetic funcfl (seed 4951), rc 4.95, "
                             "skin 1.0, every 1 delay 5 check yes, 1600 K, dt 5 fs",
                    params=params,
-                   mixed=dict(value=eres["mixed"]["value"], ms_per_step=eres["mixed"]["ms_per_step"],
-                              builds=eres["mixed"]["builds"]),
-                   fp64=dict(value=eres["fp64"]["value"], ms_per_step=eres["fp64"]["ms_per_step"],
-                             builds=eres["fp64"]["builds"]),
                    mixed_over_fp64=eres["mixed"]["value"] / eres["fp64"]["value"])
+        for prec in ("mixed", "fp64"):
+            r = eres[prec]
+            eam[prec] = dict(value=r["value"], ms_per_step=r["ms_per_step"], builds=r["builds"],
+                             kernel_sum_ms_per_step=r["kernel_sum_ms_per_step"],
+                             profiled_ms_per_step=r["profiled_ms_per_step"])
 
     # measured streaming-copy rate of this device (next to the 8 TB/s spec)
     probe = mdx.Context(0)
@@ -517,48 +626,95 @@ def main():
     probe.close()
     log("HBM copy probe: %.0f GB/s" % copy_gbs)
 
-    # roofline of the dominant kernel: the LJ force kernel, which in the fused
-    # path (the default) also runs fix nve's final integrate of step n and the
-    # initial integrate of n+1.  Algorithmic bytes per atom and launch:
-    # 4*(n_h+1) half-list indices + 32 B x_i (+ 72 B when fused: v read and
-    # write, x(n+1) write; 24 B f_i write otherwise).
+    # Roofline of the dominant kernel: the lj/cut force kernel with fix nve
+    # fused (final integrate of step n, initial integrate of n+1).  Its
+    # algorithmic bytes per atom follow the reference algorithm's compulsory
+    # traffic (half list of n_h entries + i, FP32 positions + type of i,
+    # 24 B f_i, 200 B of velocity Verlet, 16 B fixed-point mirror write):
+    # 4 (n_h + 1) + 16 + 24 + 200 + 16 per atom and launch; the list build
+    # (k (4 n_h + 32)) runs in other kernels.
     km = res["mixed"]["kernel_ms"]
     kl = res["mixed"]["kernel_launches"]
     fused = kl[mdx.K_FUSED] > kl[mdx.K_LJ]
     kk = mdx.K_FUSED if fused else mdx.K_LJ
-    lj_ms = km[kk] / max(1, kl[kk])
     n_h = res["mixed"]["total_neighbors"] / 2.0 / n
-    bytes_per_launch = n * (4.0 * (n_h + 1.0) + 32.0 + (72.0 if fused else 24.0))
-    achieved = bytes_per_launch / (lj_ms * 1e-3) / 1e9 if lj_ms > 0 else None
-    total_ms = res["mixed"]["seconds"] * 1e3
-    # the kernel as rocprofv3 names it: k_lj<MIXED, EV, TPA, INTEG, Q, ONE> with
-    # the autotuned threads per atom, the fixed-point positions of the mixed
-    # path and the single-type specialisation (taken for TPA 4, 8 and 16)
+    per_atom = 4.0 * (n_h + 1.0) + 16.0 + 24.0 + (200.0 + 16.0 if fused else 0.0)
+    narrow = 4.0 * (n_h + 1.0) + 32.0 + (72.0 if fused else 24.0)  # this kernel's own reads and writes
     tpa = res["mixed"]["launch"]["tpa"]
     kname = "k_lj<true, false, %d, %s, true, %s>" % (tpa, "true" if fused else "false",
                                                       "true" if tpa in (4, 8, 16) else "false")
-    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=None if achieved is None else achieved / HBM_PEAK_GBS,
-                    traffic=profiled_traffic(kname), kernel=kname, avg_launch_ms=lj_ms,
-                    bytes_per_launch=bytes_per_launch, n_half_neighbors_per_atom=n_h,
-                    measured_copy_gbs=copy_gbs,
-                    step_algorithmic_frac=(n * (4.0 * (n_h + 1) + 32 + 24 + 200 + 16 + (4.0 * n_h + 32) / 20.0)
-                                           / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9) / HBM_PEAK_GBS)
+    row = kernel_row(kname, km[kk], kl[kk], n * per_atom)
+    steps_per_build = K / max(1, res["mixed"]["profiled_builds"])
+    step_bytes = n * (per_atom + (4.0 * n_h + 32.0) / steps_per_build)
+    roofline = dict(bound="hbm", achieved=row["achieved"], peak=HBM_PEAK_GBS, unit="GB/s", frac=row["frac"],
+                    traffic=profiled_traffic(kname), kernel=kname, avg_launch_ms=row["avg_launch_ms"],
+                    bytes_per_launch=row["bytes_per_launch"], bytes_per_atom=per_atom,
+                    bytes_per_atom_basis="4(n_h+1) + 16 + 24 + 200 + 16 (reference algorithm, half list)",
+                    n_half_neighbors_per_atom=n_h, measured_copy_gbs=copy_gbs,
+                    frac_kernel_own_bytes=(n * narrow / (row["avg_launch_ms"] * 1e-3) / 1e9) / HBM_PEAK_GBS,
+                    kernel_own_bytes_per_atom=narrow,
+                    step_algorithmic_frac=(step_bytes / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9) / HBM_PEAK_GBS)
+    total_ms = res["mixed"]["profiled_ms_per_step"] * K
     shares = {name: km[k] / total_ms for name, k in
               (("neigh_build", mdx.K_BUILD), ("pair", mdx.K_LJ), ("pair_fused_nve", mdx.K_FUSED),
                ("nve_initial", mdx.K_NVE_INITIAL), ("nve_final", mdx.K_NVE_FINAL))}
+    if eres is not None:
+        # EAM passes: density (+ fused fp pass) and force (+ fused fix nve),
+        # reference-algorithm bytes per atom: rho 4(n_h+1) + 16 + 8 + 20,
+        # force 4(n_h+1) + 16 + 8 + 24 + 200 + 16
+        ek, el = eres["mixed"]["kernel_ms"], eres["mixed"]["kernel_launches"]
+        en = eres["mixed"]["n"]
+        enh = eres["mixed"]["total_neighbors"] / 2.0 / en
+        efk = mdx.K_FUSED if el[mdx.K_FUSED] > el[mdx.K_EAM_FORCE] else mdx.K_EAM_FORCE
+        roofline["eam"] = [
+            kernel_row("k_eam_rho (mixed)", ek[mdx.K_EAM_RHO], el[mdx.K_EAM_RHO], en * (4.0 * (enh + 1) + 44.0)),
+            kernel_row("k_eam_force (mixed, fix nve fused)", ek[efk], el[efk],
+                       en * (4.0 * (enh + 1) + 48.0 + 216.0)),
+        ]
 
-    errors, cpu, t_force = None, None, None
+    errors, cpu = None, None
     if not args.no_cpu_baseline:
-        log("oracle force check + CPU baseline sample")
-        errors, cpu, t_force = cpu_baseline_and_errors(
-            "lj", lj, {"mixed": res["mixed"]["ctx"], "fp64": res["fp64"]["ctx"]}, sample_steps=40)
+        log("oracle force check (LJ 2.048M, EAM 256k)")
+        errors = {"lj_2048k": force_errors_vs_oracle("lj", lj, {"mixed": res["mixed"]["ctx"],
+                                                             "fp64": res["fp64"]["ctx"]})}
+        if eres is not None:
+            errors["eam_256k"] = force_errors_vs_oracle("eam", ein, {"mixed": eres["mixed"]["ctx"],
+                                                                     "fp64": eres["fp64"]["ctx"]})
+            eam["force_error_vs_fp64_oracle"] = errors["eam_256k"]
+        log("CPU baseline: the oracle's threaded -O3 build on 1 and %d host threads" % nthreads)
+        lj_cpu = _child("baseline", "lj", lj["ncell"], nthreads)
+        eam_cpu = _child("baseline", "eam", ein["ncell"], nthreads, ein["table"]) if ein is not None else None
+        best = lj_cpu["samples"][-1]
+        cpu = dict(value=best["value"], unit="Matom-steps/s", cores=best["threads"], kind="port",
+                   sample="lj/cut fcc 80^3 = 2,048,000 atoms (the value's workload), %d NVE steps after setup "
+                          "(%.1f s) on %d threads; oracle/mdref.c threaded build (gcc -O3 -fopenmp, half list, "
+                          "per-thread force arrays)" % (best["steps"], best["seconds"], best["threads"]),
+                   one_core=lj_cpu["samples"][0], all_cores=best, nproc=lj_cpu["cores_available"],
+                   eam_256k=eam_cpu, calibration=dict(
+                       ratios=CALIBRATION,
+                       note="oracle / reference speed on identical cores (build container, LJ and EAM 32k bench "
+                            "inputs); reference_equivalent = box value / ratio"))
 
     drift = None
     if args.drift_steps:
-        log("NVE drift over %d steps" % args.drift_steps)
-        drift = {prec: drift_run("lj", lj, p, args.drift_steps) for prec, p in (("mixed", mdx.MIXED),
-                                                                               ("fp64", mdx.FP64))}
+        ds = args.drift_steps
+        log("NVE drift over %d steps" % ds)
+        drift = {"lj_2048k": {prec: drift_run("lj", lj, p, ds) for prec, p in (("mixed", mdx.MIXED),
+                                                                               ("fp64", mdx.FP64))}}
+        if ein is not None:
+            drift["eam_256k"] = {prec: drift_run("eam", ein, p, ds) for prec, p in (("mixed", mdx.MIXED),
+                                                                                   ("fp64", mdx.FP64))}
+        if not args.no_cpu_baseline:
+            # the same trajectory start on the device (both precisions) and the oracle, at 32k atoms
+            lj32 = lj_inputs(20)
+            drift["lj_32k"] = {prec: drift_run("lj", lj32, p, ds) for prec, p in (("mixed", mdx.MIXED),
+                                                                                  ("fp64", mdx.FP64))}
+            drift["lj_32k"]["oracle"] = _child("drift", "lj", 20, nthreads, steps=ds)
+            if ein is not None:
+                e32 = eam_inputs(20, ein["table"])
+                drift["eam_32k"] = {prec: drift_run("eam", e32, p, ds) for prec, p in (("mixed", mdx.MIXED),
+                                                                                       ("fp64", mdx.FP64))}
+                drift["eam_32k"]["oracle"] = _child("drift", "eam", 20, nthreads, ein["table"], steps=ds)
 
     fit_ka, replicas = None, None
     if not args.no_fit:
@@ -571,13 +727,15 @@ def main():
         log("  clusters forward %.2f ms, adjoint %.2f ms (mixed), grad vs FD %.2e"
             % (replicas["mixed"]["ms_forward"], replicas["mixed"]["ms_adjoint"], replicas["mixed_grad_vs_fd_max_rel"]))
 
-    log("per-call pair-style path (host positions in, forces out, every step)")
-    plugin = plugin_mode(res["mixed"]["ctx"])
-    log("  %.3f ms per step, %.1f Matom-steps/s with PCIe" % (plugin["ms_per_step"], plugin["value"]))
-    log("one GPU's share of a 2x2x2 split: octant + ghost shell, pack / unpack")
-    share = halo_share(res["mixed"]["ctx"], lj["hi"][0] - lj["lo"][0])
-    log("  %d owned + %d ghosts: pack %.3f ms, unpack %.3f ms, force kernel %.3f ms"
-        % (share["owned"], share["ghosts"], share["pack_ms"], share["unpack_ms"], share["force_kernel_ms"]))
+    plugin, share = None, None
+    if not args.no_extras:
+        log("per-call pair-style path (host positions in, forces out, every step)")
+        plugin = plugin_mode(res["mixed"]["ctx"])
+        log("  %.3f ms per step, %.1f Matom-steps/s with PCIe" % (plugin["ms_per_step"], plugin["value"]))
+        log("one GPU's share of a 2x2x2 split: octant + ghost shell, pack / unpack")
+        share = halo_share(res["mixed"]["ctx"], lj["hi"][0] - lj["lo"][0])
+        log("  %d owned + %d ghosts: pack %.3f ms, unpack %.3f ms, force kernel %.3f ms"
+            % (share["owned"], share["ghosts"], share["pack_ms"], share["unpack_ms"], share["force_kernel_ms"]))
 
     out = {
         "metric": METRIC,
@@ -599,7 +757,7 @@ def main():
         "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"]},
         "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
         "force_error_vs_fp64_oracle": errors,
-        "oracle_force_eval_s": t_force,
+        "kernel_sum_ms_per_step": {p: res[p]["kernel_sum_ms_per_step"] for p in res},
         "step_time_share": shares,
         "nve_drift": drift,
         "eam_256k": eam,
@@ -612,8 +770,14 @@ def main():
     }
     for r in res.values():
         r["ctx"].close()
+    if eres is not None:
+        for r in eres.values():
+            r["ctx"].close()
     print(json.dumps(out))
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-child":
+        _child_main(sys.argv[2:])
+    else:
+        main()

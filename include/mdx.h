@@ -211,7 +211,9 @@ int mdx_set_position_format(mdx_ctx *ctx, int format);
    lo <= x < hi (each dimension), in device order, and records `shift` (the
    image shift prd * pbc a ghost of them carries).  *count gets the number
    selected.  The list stays valid until the next list build of this
-   context; rebuild it after each one, as borders does.
+   context, which reorders the atoms; rebuild it after each one, as borders
+   does (mdx_halo_pack returns MDX_ERR_ARG for a list older than the last
+   build).
    mdx_halo_pack: writes the listed atoms' x + shift, 3 doubles each, into
    buf (device memory of this context's GPU).  swap = -1 packs every swap
    in one launch, swap 0's atoms first, then swap 1's, ... (each swap's rows
@@ -230,7 +232,8 @@ int mdx_halo_sendlist(mdx_ctx *ctx, int swap, const double lo[3], const double h
                       int *count);
 /* Send lists 0 .. nswap-1 in one pass over the atoms (regions may overlap):
    lo, hi, shift are [nswap][3], counts[nswap] gets each list's size.  The
-   lists equal what nswap calls of mdx_halo_sendlist would build. */
+   lists equal what nswap calls of mdx_halo_sendlist would build; lists
+   nswap .. 26 of earlier calls are dropped. */
 int mdx_halo_sendlists(mdx_ctx *ctx, int nswap, const double *lo, const double *hi, const double *shift,
                        int *counts);
 int mdx_halo_pack(mdx_ctx *ctx, int swap, double *buf);

@@ -405,7 +405,8 @@ __global__ __launch_bounds__(1024) void k_b_eam_rho(int n, const double4 *__rest
       if (rsq < et.cutsq) {
         int m;
         T p;
-        r_spline<T>(et, MIXED ? (T)sqrtf((float)rsq) : (T)sqrt(rsq), m, p);
+        if constexpr (MIXED) (void)eam_index_mixed(rsq, et.rdr, et.nr, m, p);
+        else r_spline<T>(et, (T)sqrt(rsq), m, p);
         const T *c = rhor + m * 7;
         acc += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
       }
@@ -475,9 +476,12 @@ __global__ __launch_bounds__(1024) void k_b_eam_force(int n, const double4 *__re
       if (rsq < et.cutsq) {
         const double fpj = fp[j];
         int m;
-        T p;
-        const T r = MIXED ? (T)sqrtf((float)rsq) : (T)sqrt(rsq);
-        r_spline<T>(et, r, m, p);
+        T p, r;
+        if constexpr (MIXED) r = eam_index_mixed(rsq, et.rdr, et.nr, m, p);
+        else {
+          r = (T)sqrt(rsq);
+          r_spline<T>(et, r, m, p);
+        }
         const T *c = rhor + m * 7;
         const T rhop = (c[0] * p + c[1]) * p + c[2];
         c = z2r + m * 7;

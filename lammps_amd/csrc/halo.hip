@@ -253,6 +253,7 @@ int mdx_halo_sendlist(mdx_ctx *c, int swap, const double lo[3], const double hi[
                        hi[0], hi[1], hi[2], pos, c->halo_list[swap]);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "halo send list");
   c->halo_count[swap] = total;
+  c->halo_serial[swap] = c->build_serial;
   *count = total;
   return MDX_OK;
 }
@@ -265,9 +266,9 @@ int mdx_halo_sendlists(mdx_ctx *c, int nswap, const double *lo, const double *hi
     return MDX_ERR_ARG;
   }
   const int n = c->n;
+  for (int s = 0; s < MDX_HALO_SWAPS; s++) c->halo_count[s] = 0;  // lists 0..nswap-1 replace every earlier one
   for (int s = 0; s < nswap; s++) {
     for (int d = 0; d < 3; d++) c->halo_shift[s][d] = shift[3 * s + d];
-    c->halo_count[s] = 0;
     counts[s] = 0;
   }
   if (n == 0) return MDX_OK;
@@ -328,6 +329,7 @@ int mdx_halo_sendlists(mdx_ctx *c, int nswap, const double *lo, const double *hi
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "halo send lists");
   for (int s = 0; s < nswap; s++) {
     c->halo_count[s] = seg[s + 1] - seg[s];
+    c->halo_serial[s] = c->build_serial;
     counts[s] = c->halo_count[s];
   }
   return MDX_OK;
@@ -340,6 +342,12 @@ int mdx_halo_pack(mdx_ctx *c, int swap, double *buf) {
     set_error("mdx_halo_pack: swap in [0, 27), or -1 for all, required");
     return MDX_ERR_ARG;
   }
+  for (int s = (swap < 0 ? 0 : swap); s < (swap < 0 ? MDX_HALO_SWAPS : swap + 1); s++)
+    if (c->halo_count[s] > 0 && c->halo_serial[s] != c->build_serial) {
+      set_error("mdx_halo_pack: send list " + std::to_string(s) +
+                " was made before the last list build, which reordered the atoms; rebuild it (as borders does)");
+      return MDX_ERR_ARG;
+    }
   if (swap == -1) {
     HaloAll h{};
     h.off[0] = 0;

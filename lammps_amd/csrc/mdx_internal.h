@@ -89,6 +89,27 @@ __device__ __forceinline__ void eam_spline_index(T r, T rdr, int nr, int &m, T &
   p = p < T(1) ? p : T(1);
 }
 
+// Mixed precision: r, the knot index m and the fraction p from the FP64
+// rsq.  r is the FP32 reciprocal square root refined by one FP64 Newton
+// step (relative error ~1e-14; sqrtf of the FP32-rounded rsq is ~1e-7), the
+// index arithmetic of pair_eam.cpp:188-192 runs in FP64, and only r and the
+// fraction are rounded to FP32 for the spline arithmetic.  With r and p in
+// FP32 the spline position carries ~2e-7 relative error, which the steep
+// pair terms turn into force errors of ~1e-5 of the RMS force on large
+// systems; here the FP32 coefficients and arithmetic bound it (~2e-6).
+// Returns r in FP32.
+__device__ __forceinline__ float eam_index_mixed(double rsq, double rdr, int nr, int &m, float &p) {
+  const double y = (double)__builtin_amdgcn_rsqf((float)rsq);
+  const double r0 = rsq * y;
+  const double r = fma(0.5 * y, fma(-r0, r0, rsq), r0);
+  double pp = fma(r, rdr, 1.0);
+  m = static_cast<int>(pp);
+  m = m < 1 ? 1 : (m > nr - 1 ? nr - 1 : m);
+  pp -= m;
+  p = (float)(pp < 1.0 ? pp : 1.0);
+  return (float)r;
+}
+
 // FP32 rho spline value from the knot's c3..c6 vector at fraction p
 __device__ __forceinline__ float eam_rho_eval(const float4 &c, float p) { return ((c.x * p + c.y) * p + c.z) * p + c.w; }
 
@@ -98,7 +119,7 @@ __device__ __forceinline__ double eam_rho_term(const EAMArgs &a, int tab, double
   int m;
   if (MIXED) {
     float p;
-    eam_spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
+    (void)eam_index_mixed(rsq, a.rdr, a.nr, m, p);
     return (double)eam_rho_eval(a.rhor_pf[2 * ((size_t)tab * (a.nr + 1) + m) + 1], p);  // c3..c6
   }
   double p;
@@ -133,10 +154,9 @@ __device__ __forceinline__ void eam_pair_term(const EAMArgs &a, int ri, int rj, 
                                               double fpj, double &fpair, double &phi) {
   const size_t kn = (size_t)(a.nr + 1);
   if (MIXED) {
-    const float r = sqrtf((float)rsq);
     int m;
     float p;
-    eam_spline_index<float>(r, (float)a.rdr, a.nr, m, p);
+    const float r = eam_index_mixed(rsq, a.rdr, a.nr, m, p);
     const float4 ci = a.rhor_pf[2 * (ri * kn + m)];  // c0..c2
     float4 cj = ci;  // (a conditional lvalue here would put ci in scratch memory)
     if (ri != rj) cj = a.rhor_pf[2 * (rj * kn + m)];
@@ -293,6 +313,10 @@ struct mdx_ctx {
   int *halo_list[MDX_HALO_SWAPS] = {};
   int halo_cap[MDX_HALO_SWAPS] = {}, halo_count[MDX_HALO_SWAPS] = {};
   double halo_shift[MDX_HALO_SWAPS][3] = {};
+  // the list build a send list belongs to: every build permutes the atoms,
+  // so a list made before the last build names the wrong ones
+  long build_serial = 0;
+  long halo_serial[MDX_HALO_SWAPS] = {};
   int *halo_flag = nullptr;  // [npad + 1] selection flags, then their exclusive scan
   int halo_flag_cap = 0;
   void *halo_scan_tmp = nullptr;

@@ -219,15 +219,30 @@ __global__ void k_bin_scatter(int n, const unsigned *__restrict__ key, const int
   tmp[bin_start[key[i]] + slot[i]] = i;
 }
 
+// The rank scan is linear in the bin's occupancy, so the pass is quadratic in
+// it.  Every two atoms of one bin are neighbours (the bin diagonal,
+// sqrt(3)/2 of the neighbour cutoff, is inside it), so a bin holding more
+// than `one` + 1 atoms gives each of them more than `one` neighbours: the
+// reference stops there with "Neighbor list overflow" (neigh_half_bin.cpp:
+// 414-415, the first atom of the bin's chain).  Such a bin keeps its arrival
+// order (still a permutation, so the passes after it stay in bounds) and
+// raises the overflow flag, which the build reports; the scan is thus
+// bounded by `one` + 1 entries per atom.
 __global__ void k_bin_rank(int n, const unsigned *__restrict__ key, const int *__restrict__ bin_start,
-                           const int *__restrict__ tmp, int *__restrict__ src, unsigned *__restrict__ key_s) {
+                           const int *__restrict__ tmp, int *__restrict__ src, unsigned *__restrict__ key_s,
+                           int max_occupancy, int *__restrict__ dflags) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int i = tmp[p];
   const unsigned b = key[i];
   const int s = bin_start[b], e = bin_start[b + 1];
-  int r = 0;
-  for (int k = s; k < e; k++) r += tmp[k] < i;
+  int r = p - s;
+  if (e - s <= max_occupancy) {
+    r = 0;
+    for (int k = s; k < e; k++) r += tmp[k] < i;
+  } else if (p == s) {
+    atomicOr(&dflags[5], 1);
+  }
   src[s + r] = i;
   key_s[s + r] = b;
 }
@@ -557,7 +572,7 @@ int neigh_build(mdx_ctx *c) {
   if (e != hipSuccess) return hip_fail(e, "bin scan");
   hipLaunchKernelGGL(k_bin_scatter, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bidx, c->bin_start, c->bidx_s);
   hipLaunchKernelGGL(k_bin_rank, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bin_start, c->bidx_s, c->bidx,
-                     c->bkey_s);
+                     c->bkey_s, c->one + 1, c->dflags);
   hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx, c->x, c->v, c->tag, c->orig,
                      c->image, c->xb, c->vb, c->tagb, c->origb, c->imageb, c->inv, c->xhold,
                      c->qpos ? c->xq : nullptr, c->qs);
@@ -587,7 +602,7 @@ int neigh_build(mdx_ctx *c) {
       return MDX_ERR_ARG;
     }
     const int mx = flags[0];
-    if (mx > c->one) {
+    if (mx > c->one || flags[5]) {  // flags[5]: a bin holds more than one + 1 atoms (k_bin_rank)
       set_error("Neighbor list overflow, boost neigh_modify one");
       return MDX_ERR_OVERFLOW;
     }
@@ -608,6 +623,7 @@ int neigh_build(mdx_ctx *c) {
   }
   prof_end(c);
   c->built = true;
+  c->build_serial++;
   c->ago = 0;
   return MDX_OK;
 }

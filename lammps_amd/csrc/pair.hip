@@ -426,7 +426,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
           if constexpr (TL) {
             int m;
             float p;
-            eam_spline_index<float>(sqrtf((float)rsq), (float)a.rdr, a.nr, m, p);
+            (void)eam_index_mixed(rsq, a.rdr, a.nr, m, p);
             rho += (double)eam_rho_eval(s_eam_tab[m], p);
           } else {
             rho += eam_rho_term<MIXED>(a, s_t2r[ONE ? a.ntypes1 + 1 : tj * a.ntypes1 + ti], rsq);
@@ -535,10 +535,9 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
         const int tij = ONE ? a.ntypes1 + 1 : ti * a.ntypes1 + tj, tji = ONE ? tij : tj * a.ntypes1 + ti;
         double fpair, phi;
         if constexpr (TL) {
-          const float r = sqrtf((float)rsq);
           int m;
           float p;
-          eam_spline_index<float>(r, (float)a.rdr, a.nr, m, p);
+          const float r = eam_index_mixed(rsq, a.rdr, a.nr, m, p);
           const float4 ci = s_eam_tab[m];
           eam_pair_eval(ci, ci, s_eam_tab[kn + m], s_eam_tab[2 * kn + m], r, p, fpi, fpjv[u], fpair, phi);
         } else {
@@ -582,20 +581,34 @@ int pair_wave_cap(int dflt) {
   return w >= 0 ? w : dflt;
 }
 
-size_t occupancy_lds(int B, size_t used, int dflt) {
+// Dynamic LDS of a launch: `used` bytes, or, under a wave cap, enough that
+// the workgroup's whole LDS (the kernel's static LDS, from its attributes,
+// plus this) lets at most 4 * cap / (B/64) workgroups share a CU's 160 KiB.
+template <auto K>
+size_t static_lds() {
+  static const size_t v = [] {
+    hipFuncAttributes attr{};
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(K)) == hipSuccess ? attr.sharedSizeBytes
+                                                                                         : size_t(0);
+  }();
+  return v;
+}
+
+size_t occupancy_lds(size_t stat, int B, size_t used, int dflt) {
   const int w = pair_wave_cap(dflt);
   if (w <= 0) return used;
   const int per_block = (B + 63) / 64;
   const int blocks = std::max(1, 4 * w / per_block);
-  const size_t pad = ((size_t)(163840 / (blocks + 1)) + 256) & ~(size_t)255;
-  return std::max(used, pad);
+  const size_t total = ((size_t)(163840 / (blocks + 1)) + 256) & ~(size_t)255;
+  return std::max(used, total > stat ? total - stat : 0);
 }
 
 template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), INTEG ? occupancy_lds(B, 0, MIXED ? 0 : 6) : 0, c->stream, c->n, c->npad, c->maxneigh,
+  const size_t lds = INTEG ? occupancy_lds(static_lds<k_lj<MIXED, EV, TPA, INTEG, Q, ONE>>(), B, 0, MIXED ? 0 : 6) : 0;
+  hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), lds, c->stream, c->n, c->npad, c->maxneigh,
                      c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->xq, c->qs, c->nbr, c->numneigh,
                      c->lj_dev, c->f, c->vatom, c->partials, ig);
   prof_end(c);
@@ -612,13 +625,13 @@ void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const size_t kn = (size_t)c->eam.nr + 1;
   prof_begin(c, MDX_K_EAM_RHO);
   hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>), dim3(G), dim3(B),
-                     occupancy_lds(B, TL ? kn * sizeof(float4) : 0, 0), c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
+                     occupancy_lds(static_lds<k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>>(), B, TL ? kn * sizeof(float4) : 0, 0), c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
                      c->nbr, c->numneigh, c->rho, c->fp, c->eemb);
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
   const size_t tl_bytes = TL ? 3 * kn * sizeof(float4) : 0;
   hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE, TL>), dim3(G), dim3(B),
-                     INTEG ? occupancy_lds(B, tl_bytes, 0) : tl_bytes, c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1],
+                     INTEG ? occupancy_lds(static_lds<k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE, TL>>(), B, tl_bytes, 0) : tl_bytes, c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1],
                      c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
                      c->partials, ig);
   prof_end(c);

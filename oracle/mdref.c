@@ -39,6 +39,12 @@
 #include <string.h>
 #include <stdint.h>
 #include "mdref.h"
+#ifdef MDREF_OMP
+#include <omp.h>
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
 
 #define MAXT 8
 #define MAXSWAP 64
@@ -126,6 +132,9 @@ struct mdref {
       vflag_fdotr;
   double eng_vdwl, virial[6];
   double *eatom, *vatom;
+  /* thread-private pass buffers of the threaded baseline build (MDREF_OMP) */
+  double *thr;
+  size_t thr_cap;
   /* run state */
   long ntimestep;
   int thermo_every;
@@ -251,6 +260,7 @@ void mdref_free(mdref *s) {
   free(s->frho); free(s->rhor); free(s->z2r);
   free(s->frho_spline); free(s->rhor_spline); free(s->z2r_spline);
   free(s->rho); free(s->fp); free(s->eatom); free(s->vatom);
+  free(s->thr);
   free(s);
 }
 
@@ -1321,6 +1331,113 @@ static int build_list(mdref *s) {
   return 0;
 }
 
+
+#ifdef MDREF_OMP
+/* Threaded build for the multi-core CPU baseline: thread t builds the rows
+   of atoms [t*n/T, (t+1)*n/T) into its own buffer with the candidate loop of
+   build_list (half newton, half no-newton or full), and the buffers are
+   concatenated in thread order, so the list equals build_list's. */
+static int build_list_omp(mdref *s) {
+  bin_atoms(s);
+  int nlocal = s->nlocal;
+  if (nlocal > s->maxlist) {
+    s->maxlist = s->nmax;
+    free(s->ilist); free(s->numneigh); free(s->firstneigh);
+    s->ilist = (int *)malloc(sizeof(int) * s->maxlist);
+    s->numneigh = (int *)malloc(sizeof(int) * s->maxlist);
+    s->firstneigh = (int *)malloc(sizeof(int) * s->maxlist);
+  }
+  const int T = omp_get_max_threads();
+  int **buf = (int **)calloc(T, sizeof(int *));
+  long *used = (long *)calloc(T, sizeof(long));
+  int nteam = 1;
+  const double *x = s->x;
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#pragma omp single
+    nteam = nt;
+    const int lo = (int)((long)nlocal * t / nt), hi = (int)((long)nlocal * (t + 1) / nt);
+    long cap = 1024, u = 0;
+    int *b = (int *)malloc(sizeof(int) * cap);
+#define PUSH(j)                                           \
+  do {                                                    \
+    if (u >= cap) {                                       \
+      cap *= 2;                                           \
+      b = (int *)realloc(b, sizeof(int) * cap);           \
+    }                                                     \
+    b[u++] = (j);                                         \
+  } while (0)
+    for (int i = lo; i < hi; i++) {
+      long start = u;
+      int itype = s->type[i];
+      double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+      int ibin = coord2bin(s, &x[3 * i]);
+      if (s->list_style == MDREF_LIST_HALF_NEWTON) {
+        for (int j = s->bins[i]; j >= 0; j = s->bins[j]) {
+          if (j >= nlocal) {
+            if (x[3 * j + 2] < ztmp) continue;
+            if (x[3 * j + 2] == ztmp) {
+              if (x[3 * j + 1] < ytmp) continue;
+              if (x[3 * j + 1] == ytmp && x[3 * j] < xtmp) continue;
+            }
+          }
+          double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+          double rsq = delx * delx + dely * dely + delz * delz;
+          if (rsq <= s->cutneighsq[itype][s->type[j]]) PUSH(j);
+        }
+        for (int k = 0; k < s->nstencil; k++)
+          for (int j = s->binhead[ibin + s->stencil[k]]; j >= 0; j = s->bins[j]) {
+            double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+            double rsq = delx * delx + dely * dely + delz * delz;
+            if (rsq <= s->cutneighsq[itype][s->type[j]]) PUSH(j);
+          }
+      } else {
+        for (int k = 0; k < s->nstencil; k++)
+          for (int j = s->binhead[ibin + s->stencil[k]]; j >= 0; j = s->bins[j]) {
+            if (s->list_style == MDREF_LIST_HALF_NONEWTON) {
+              if (j <= i) continue;
+            } else if (i == j) continue;
+            double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+            double rsq = delx * delx + dely * dely + delz * delz;
+            if (rsq <= s->cutneighsq[itype][s->type[j]]) PUSH(j);
+          }
+      }
+#undef PUSH
+      s->ilist[i] = i;
+      s->firstneigh[i] = (int)start;
+      s->numneigh[i] = (int)(u - start);
+    }
+    buf[t] = b;
+    used[t] = u;
+  }
+  long total = 0;
+  for (int t = 0; t < nteam; t++) total += used[t];
+  if (total > s->nneigh_alloc) {
+    s->nneigh_alloc = total;
+    free(s->neighbors);
+    s->neighbors = (int *)malloc(sizeof(int) * (total ? total : 1));
+  }
+  long off = 0;
+  int over = 0;
+  for (int t = 0; t < nteam; t++) {
+    const int lo = (int)((long)nlocal * t / nteam), hi = (int)((long)nlocal * (t + 1) / nteam);
+    memcpy(s->neighbors + off, buf[t], sizeof(int) * used[t]);
+    for (int i = lo; i < hi; i++) {
+      s->firstneigh[i] += (int)off;
+      if (s->numneigh[i] > s->one) over = 1;
+    }
+    off += used[t];
+    free(buf[t]);
+  }
+  free(buf);
+  free(used);
+  s->inum = nlocal;
+  if (over) return fail(s, "Neighbor list overflow, boost neigh_modify one");
+  return 0;
+}
+#endif
+
 /* neighbor.cpp:1380-1461 */
 static int neigh_build(mdref *s) {
   s->ago = 0;
@@ -1333,12 +1450,18 @@ static int neigh_build(mdref *s) {
     }
     memcpy(s->xhold, s->x, sizeof(double) * 3 * s->nlocal);
   }
+#ifdef MDREF_OMP
+  if (omp_get_max_threads() > 1) return build_list_omp(s);
+#endif
   return build_list(s);
 }
 
 /* neighbor.cpp:1322-1372 (fixed box) */
 static int check_distance(mdref *s) {
   int flag = 0;
+#ifdef MDREF_OMP
+#pragma omp parallel for schedule(static) reduction(max : flag)
+#endif
   for (int i = 0; i < s->nlocal; i++) {
     double delx = s->x[3 * i] - s->xhold[3 * i];
     double dely = s->x[3 * i + 1] - s->xhold[3 * i + 1];
@@ -1630,6 +1753,199 @@ static void eam_compute(mdref *s, int eflag, int vflag) {
 #undef FRHO
 }
 
+
+#ifdef MDREF_OMP
+/* Threaded force passes for the multi-core CPU baseline, on steps without
+   energy/virial tallies: the half-list loops of lj_compute / eam_compute
+   with each thread's share of atoms (static schedule) accumulating into
+   its own force (and eam density) array, summed after the pass in thread
+   order, the scheme of the reference's USER-OMP styles
+   (src/USER-OMP/pair_lj_cut_omp.cpp:38-76, pair_eam_omp.cpp:40-94 with
+   thr_data.cpp's per-thread arrays). */
+static double *thr_arrays(mdref *s, size_t per_thread) {
+  size_t need = per_thread * (size_t)omp_get_max_threads();
+  if (need > s->thr_cap) {
+    free(s->thr);
+    s->thr = (double *)malloc(sizeof(double) * need);
+    s->thr_cap = need;
+  }
+  return s->thr;
+}
+
+static void sum_thr(double *out, const double *thr, size_t per_thread, int nt) {
+#pragma omp for schedule(static)
+  for (size_t k = 0; k < per_thread; k++) {
+    double acc = 0.0;
+    for (int q = 0; q < nt; q++) acc += thr[(size_t)q * per_thread + k];
+    out[k] += acc;
+  }
+}
+
+static void lj_force_omp(mdref *s) {
+  const double *x = s->x;
+  const int nlocal = s->nlocal, nall = nlocal + s->nghost, newton_pair = s->newton_pair;
+  const int full = s->list_style == MDREF_LIST_FULL;
+  const size_t per = (size_t)3 * nall;
+  double *thr = thr_arrays(s, per);
+#pragma omp parallel
+  {
+    const int nt = omp_get_num_threads();
+    double *f = thr + (size_t)omp_get_thread_num() * per;
+    memset(f, 0, sizeof(double) * per);
+#pragma omp for schedule(static)
+    for (int ii = 0; ii < s->inum; ii++) {
+      int i = s->ilist[ii];
+      double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+      int itype = s->type[i];
+      const int *jlist = &s->neighbors[s->firstneigh[i]];
+      int jnum = s->numneigh[i];
+      for (int jj = 0; jj < jnum; jj++) {
+        int j = jlist[jj] & NEIGHMASK;
+        double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        int jtype = s->type[j];
+        if (rsq < s->cutsq[itype][jtype]) {
+          double r2inv = 1.0 / rsq;
+          double r6inv = r2inv * r2inv * r2inv;
+          double forcelj = r6inv * (s->lj1[itype][jtype] * r6inv - s->lj2[itype][jtype]);
+          double fpair = forcelj * r2inv;
+          f[3 * i] += delx * fpair;
+          f[3 * i + 1] += dely * fpair;
+          f[3 * i + 2] += delz * fpair;
+          if (!full && (newton_pair || j < nlocal)) {
+            f[3 * j] -= delx * fpair;
+            f[3 * j + 1] -= dely * fpair;
+            f[3 * j + 2] -= delz * fpair;
+          }
+        }
+      }
+    }
+    sum_thr(s->f, thr, per, nt);
+  }
+}
+
+static void eam_force_omp(mdref *s) {
+  const double *x = s->x;
+  double *fout = s->f;
+  const int nlocal = s->nlocal, nall = nlocal + s->nghost, newton_pair = s->newton_pair;
+  const int full = s->list_style == MDREF_LIST_FULL;
+  const int nr = s->nr, nrho = s->nrho;
+  const double rdr = s->rdr, rdrho = s->rdrho;
+#define RHOR(t, m) (&s->rhor_spline[((size_t)(t) * (nr + 1) + (m)) * 7])
+#define Z2R(t, m) (&s->z2r_spline[((size_t)(t) * (nr + 1) + (m)) * 7])
+#define FRHO(t, m) (&s->frho_spline[((size_t)(t) * (nrho + 1) + (m)) * 7])
+  const int nrho_arr = (newton_pair && !full) ? nall : nlocal;
+  for (int i = 0; i < nrho_arr; i++) s->rho[i] = 0.0;
+  const size_t per = (size_t)3 * nall;
+  double *thr = thr_arrays(s, per);
+  /* density pass: each thread's rho lives in the first nall doubles of its block */
+#pragma omp parallel
+  {
+    const int nt = omp_get_num_threads();
+    double *rho = thr + (size_t)omp_get_thread_num() * per;
+    memset(rho, 0, sizeof(double) * nall);
+#pragma omp for schedule(static)
+    for (int ii = 0; ii < s->inum; ii++) {
+      int i = s->ilist[ii];
+      double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+      int itype = s->type[i];
+      const int *jlist = &s->neighbors[s->firstneigh[i]];
+      int jnum = s->numneigh[i];
+      for (int jj = 0; jj < jnum; jj++) {
+        int j = jlist[jj] & NEIGHMASK;
+        double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < s->cutforcesq) {
+          int jtype = s->type[j];
+          double p = sqrt(rsq) * rdr + 1.0;
+          int m = (int)p;
+          m = MIN(m, nr - 1);
+          p -= m;
+          p = MIN(p, 1.0);
+          const double *coeff = RHOR(s->type2rhor[jtype][itype], m);
+          rho[i] += ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+          if (!full && (newton_pair || j < nlocal)) {
+            coeff = RHOR(s->type2rhor[itype][jtype], m);
+            rho[j] += ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+          }
+        }
+      }
+    }
+#pragma omp for schedule(static)
+    for (int k = 0; k < nrho_arr; k++) {
+      double acc = 0.0;
+      for (int q = 0; q < nt; q++) acc += thr[(size_t)q * per + k];
+      s->rho[k] += acc;
+    }
+  }
+  if (newton_pair && !full) comm_reverse_rho(s);
+  OMP_FOR
+  for (int ii = 0; ii < s->inum; ii++) {
+    int i = s->ilist[ii];
+    double p = s->rho[i] * rdrho + 1.0;
+    int m = (int)p;
+    m = MAX(1, MIN(m, nrho - 1));
+    p -= m;
+    p = MIN(p, 1.0);
+    const double *coeff = FRHO(s->type2frho[s->type[i]], m);
+    s->fp[i] = (coeff[0] * p + coeff[1]) * p + coeff[2];
+  }
+  comm_forward_fp(s);
+#pragma omp parallel
+  {
+    const int nt = omp_get_num_threads();
+    double *f = thr + (size_t)omp_get_thread_num() * per;
+    memset(f, 0, sizeof(double) * per);
+#pragma omp for schedule(static)
+    for (int ii = 0; ii < s->inum; ii++) {
+      int i = s->ilist[ii];
+      double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+      int itype = s->type[i];
+      const int *jlist = &s->neighbors[s->firstneigh[i]];
+      int jnum = s->numneigh[i];
+      for (int jj = 0; jj < jnum; jj++) {
+        int j = jlist[jj] & NEIGHMASK;
+        double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < s->cutforcesq) {
+          int jtype = s->type[j];
+          double r = sqrt(rsq);
+          double p = r * rdr + 1.0;
+          int m = (int)p;
+          m = MIN(m, nr - 1);
+          p -= m;
+          p = MIN(p, 1.0);
+          const double *coeff = RHOR(s->type2rhor[itype][jtype], m);
+          double rhoip = (coeff[0] * p + coeff[1]) * p + coeff[2];
+          coeff = RHOR(s->type2rhor[jtype][itype], m);
+          double rhojp = (coeff[0] * p + coeff[1]) * p + coeff[2];
+          coeff = Z2R(s->type2z2r[itype][jtype], m);
+          double z2p = (coeff[0] * p + coeff[1]) * p + coeff[2];
+          double z2 = ((coeff[3] * p + coeff[4]) * p + coeff[5]) * p + coeff[6];
+          double recip = 1.0 / r;
+          double phi = z2 * recip;
+          double phip = z2p * recip - phi * recip;
+          double psip = s->fp[i] * rhojp + s->fp[j] * rhoip + phip;
+          double fpair = -psip * recip;
+          f[3 * i] += delx * fpair;
+          f[3 * i + 1] += dely * fpair;
+          f[3 * i + 2] += delz * fpair;
+          if (!full && (newton_pair || j < nlocal)) {
+            f[3 * j] -= delx * fpair;
+            f[3 * j + 1] -= dely * fpair;
+            f[3 * j + 2] -= delz * fpair;
+          }
+        }
+      }
+    }
+    sum_thr(fout, thr, per, nt);
+  }
+#undef RHOR
+#undef Z2R
+#undef FRHO
+}
+#endif
+
 /* verlet.cpp:325-371 */
 static void force_clear(mdref *s) {
   int n = s->nlocal + (s->newton_pair ? s->nghost : 0);
@@ -1637,6 +1953,14 @@ static void force_clear(mdref *s) {
 }
 
 static void pair_compute(mdref *s, int eflag, int vflag) {
+#ifdef MDREF_OMP
+  if (!eflag && !vflag) {
+    s->evflag = s->vflag_fdotr = s->eflag_global = s->eflag_atom = 0;
+    if (s->pair_style == MDREF_PAIR_LJ) lj_force_omp(s);
+    else eam_force_omp(s);
+    return;
+  }
+#endif
   if (s->pair_style == MDREF_PAIR_LJ) lj_compute(s, eflag, vflag);
   else eam_compute(s, eflag, vflag);
 }
@@ -1644,6 +1968,7 @@ static void pair_compute(mdref *s, int eflag, int vflag) {
 /* fix_nve.cpp:65-105, 109-142 */
 static void nve_initial(mdref *s) {
   double dtv = s->dt, dtf = 0.5 * s->dt * s->ftm2v;
+  OMP_FOR
   for (int i = 0; i < s->nlocal; i++) {
     double dtfm = dtf / s->mass[s->type[i]];
     s->v[3 * i] += dtfm * s->f[3 * i];
@@ -1657,6 +1982,7 @@ static void nve_initial(mdref *s) {
 
 static void nve_final(mdref *s) {
   double dtf = 0.5 * s->dt * s->ftm2v;
+  OMP_FOR
   for (int i = 0; i < s->nlocal; i++) {
     double dtfm = dtf / s->mass[s->type[i]];
     s->v[3 * i] += dtfm * s->f[3 * i];
@@ -1774,6 +2100,17 @@ int mdref_compute(mdref *s, int eflag, int vflag) {
 
 /* ------------------------------------------------------------------ */
 /* accessors                                                          */
+
+/* threads of the threaded baseline build (1 in the serial build) */
+int mdref_threads(int n) {
+#ifdef MDREF_OMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 int mdref_natoms(mdref *s) { return s->natoms; }
 int mdref_nghost(mdref *s) { return s->nghost; }

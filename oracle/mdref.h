@@ -70,6 +70,13 @@ int mdref_run(mdref *s, int nsteps, double *thermo, int max_rows, int *nrows);
    eflag/vflag as Integrate::ev_set (1 global, 2 per-atom; vflag 1/2 global, 4 per-atom) */
 int mdref_compute(mdref *s, int eflag, int vflag);
 
+/* Threads of the threaded CPU-baseline build (libmdref_omp.so, -fopenmp
+   -DMDREF_OMP): sets the count when n > 0 and returns it; the serial build
+   returns 1.  The threaded build runs the force passes of steps without
+   energy/virial tallies and the list build on threads, with per-thread
+   force arrays summed after each pass (the reference's USER-OMP scheme). */
+int mdref_threads(int n);
+
 int mdref_natoms(mdref *s);
 int mdref_nghost(mdref *s);
 /* per-atom output ordered by tag (tag 1..N -> row 0..N-1) */

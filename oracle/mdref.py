@@ -33,12 +33,37 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+_LIB_OMP_PATH = os.path.join(_HERE, "_build", "libmdref_omp.so")
+_lib_omp = None
+
+
+def lib_threaded():
+    """The threaded CPU-baseline build (-O3 -fopenmp -DMDREF_OMP): the same
+    functions; force passes without tallies and the list build on threads."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(_LIB_OMP_PATH):
+            build()
+        _lib_omp = _setup(ctypes.CDLL(_LIB_OMP_PATH))
+    return _lib_omp
+
+
+def set_threads(n):
+    """Threads of the threaded build (returns the count in effect)."""
+    return lib_threaded().mdref_threads(int(n))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        _lib = _setup(ctypes.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _setup(L):
+    if True:
         L.mdref_new.restype = _p
         L.mdref_free.argtypes = [_p]
         L.mdref_error.restype = ctypes.c_char_p
@@ -88,8 +113,9 @@ def lib():
         L.mdref_ntimestep.restype = ctypes.c_long
         L.mdref_ranpark_uniform_seq.argtypes = [_i, _i, _ip]
         L.mdref_ranpark_uniform_seq.restype = _d
-        _lib = L
-    return _lib
+        L.mdref_threads.argtypes = [_i]
+        L.mdref_threads.restype = _i
+    return L
 
 
 def _ptr(a, kind=_dp):
@@ -103,8 +129,8 @@ class RefError(RuntimeError):
 class Ref:
     """One oracle system (one LAMMPS 'world' on one process)."""
 
-    def __init__(self, units=LJ):
-        self._L = lib()
+    def __init__(self, units=LJ, threaded=False):
+        self._L = lib_threaded() if threaded else lib()
         self.h = self._L.mdref_new()
         self._ck(self._L.mdref_units(self.h, units))
 

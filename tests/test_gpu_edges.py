@@ -166,6 +166,56 @@ def test_neighbor_overflow_is_reported():
     ctx.close()
 
 
+def _crowded(ncrowd, seed):
+    """A sparse periodic background plus `ncrowd` atoms inside one bin:
+    box 20 sigma, neighbour cutoff 2.8, 14 bins of 1.4286 per dimension, the
+    crowd uniform in [5.75, 6.9)^3 (bin 4 in each dimension)."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*(np.arange(10),) * 3, indexing="ij"), -1).reshape(-1, 3) * 2.0 + 0.5
+    g = g[np.any(np.abs(g - 6.3) > 3.5, axis=1)]  # keep the background clear of the crowd
+    return np.vstack([g, 5.75 + 1.15 * rng.random((ncrowd, 3))])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_crowded_bin_gives_the_oracle_pair_set(prec):
+    """400 atoms in one bin (each has >= 399 neighbours, one = 2000 allows
+    them): the per-bin ranking of the counting sort runs over the whole bin
+    and the list equals the oracle's half_bin_newton pair set."""
+    from lammps_amd import mdx
+    x = _crowded(400, 3)
+    L = 20.0
+    ctx = lj_context((0, 0, 0), (L, L, L), (1, 1, 1), x, mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    ctx.neigh_build()
+    st = ctx.stats()
+    assert st["max_neighbors"] >= 399
+    ref = M.Ref(M.LJ)
+    ref.box((0, 0, 0), (L, L, L))
+    ref.add_atoms(x, np.ones(len(x), dtype=np.int32))
+    ref.mass(1, 1.0)
+    ref.pair_lj(2.5)
+    ref.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ref.neighbor(0.3)
+    ref.setup()
+    got = np.sort(np.array([min(a, b) << 32 | max(a, b) for a, b in ctx.neigh_pairs()], dtype=np.int64))
+    want = np.sort(np.array([min(a, b) << 32 | max(a, b) for a, b in ref.list_pairs()], dtype=np.int64))
+    assert np.array_equal(got, want)
+    ctx.close()
+
+
+def test_collapsed_bin_is_reported_as_overflow():
+    """2500 atoms in one bin with one = 2000: every one of them has more than
+    `one` neighbours, the reference's "Neighbor list overflow"
+    (neigh_half_bin.cpp:414-415); the build reports it as a status code
+    after a bounded sort pass."""
+    from lammps_amd import mdx
+    x = _crowded(2500, 4)
+    ctx = lj_context((0, 0, 0), (20.0, 20.0, 20.0), (1, 1, 1), x, mdx.FP64, one=2000)
+    with pytest.raises(mdx.MdxError) as e:
+        ctx.neigh_build()
+    assert e.value.code == mdx.ERR_OVERFLOW and "boost neigh_modify one" in str(e.value)
+    ctx.close()
+
+
 def test_lost_atom_in_a_closed_box_is_reported():
     """boundary f: an atom outside the box is lost (the reference's
     'Lost atoms' check, thermo.cpp lost/error), an error and no crash"""

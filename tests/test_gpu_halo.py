@@ -159,3 +159,33 @@ def test_halo_argument_errors_launch_nothing():
         G.halo_unpack(len(x) - 1, 2, host.ctypes.data)
     assert e.value.code == mdx.ERR_ARG and "must exist" in str(e.value)
     G.close()
+
+
+def test_stale_and_replaced_send_lists():
+    """A list build permutes the atoms, so a send list made before it names
+    the wrong atoms: packing it is an error until the list is rebuilt (as
+    borders rebuilds the send lists at every reneighbouring).  One-pass
+    lists replace every earlier list, not only the first nswap."""
+    import torch
+    from lammps_amd import mdx
+    x, L = perturbed_lattice(6, seed=4)
+    G = lj_ctx((0, 0, 0), (L, L, L), (1, 1, 1), x, "fp64")
+    G.neigh_build()
+    h = L / 2
+    c3 = G.halo_sendlist(3, (0, 0, 0), (h, h, h), (0, 0, 0))
+    regions = [((0, 0, 0), (h, L, L), (0, 0, 0)), ((h, 0, 0), (L, L, L), (-L, 0, 0))]
+    counts = G.halo_sendlists(regions)
+    assert c3 > 0 and sum(counts) == len(x)
+    buf = torch.zeros(3 * len(x) + 3 * c3, dtype=torch.float64, device="cuda")
+    G.halo_pack(-1, buf.data_ptr())  # swap 3's old list is gone: the two new lists only
+    torch.cuda.synchronize()
+    packed = buf.cpu().numpy().reshape(-1, 3)
+    assert np.all(packed[len(x):] == 0.0)
+    G.neigh_build()
+    for swap in (0, -1):
+        with pytest.raises(mdx.MdxError) as e:
+            G.halo_pack(swap, buf.data_ptr())
+        assert e.value.code == mdx.ERR_ARG and "last list build" in str(e.value)
+    G.halo_sendlists(regions)
+    G.halo_pack(-1, buf.data_ptr())
+    G.close()

@@ -327,3 +327,43 @@ def test_nve_free_flight_is_exact_ballistic():
     unwrapped = xs + im * 15.0
     np.testing.assert_allclose(vs, v, rtol=0, atol=0)
     np.testing.assert_allclose(unwrapped, x + 50 * 0.005 * v, rtol=0, atol=1e-13)
+
+
+def _threaded_vs_serial(kind, threads):
+    out = []
+    for threaded in (False, True):
+        if threaded:
+            assert M.set_threads(threads) == threads
+        r = M.Ref(M.LJ if kind == "lj" else M.METAL, threaded=threaded)
+        if kind == "lj":
+            r.lattice_box(M.FCC, 0.8442, (0, 0, 0), (8, 8, 8))
+            r.mass(1, 1.0)
+            r.velocity_create(1.44, 87287, dist="gaussian", loop="geom")
+            r.pair_lj(2.5)
+            r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+            r.neighbor(0.3, every=2, delay=0, check=True)
+        else:
+            r.lattice_box(M.FCC, 3.615, (0, 0, 0), (6, 6, 6))
+            r.pair_eam(os.path.join(GOLDEN, "Cu_u3.eam"))
+            r.velocity_create(1600.0, 376847, dist="uniform", loop="geom")
+            r.neighbor(1.0, every=1, delay=2, check=True)
+        r.timestep(0.005)
+        r.setup()
+        rows = r.run(30)
+        x, v, f, _, _ = r.state()
+        out.append((rows, x, f, r.neigh_stats()))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["lj", "eam"])
+def test_threaded_baseline_build_follows_the_serial_oracle(kind):
+    """The threaded CPU-baseline build (libmdref_omp.so: -O3, 3 threads,
+    per-thread force arrays) runs the serial oracle's trajectory: identical
+    neighbour totals and build counts, thermo rows and forces within 1e-10
+    relative (summation order of the per-thread arrays only)."""
+    (rs, xs, fs, ns), (rt, xt, ft, nt) = _threaded_vs_serial(kind, 3)
+    assert ns == nt
+    assert np.max(np.abs(rs - rt) / np.maximum(1e-300, np.abs(rs)).clip(1e-12)) <= 1e-10
+    scale = np.sqrt(np.mean(np.sum(fs * fs, axis=1)))
+    assert np.max(np.linalg.norm(ft - fs, axis=1)) <= 1e-10 * scale
+    assert np.max(np.abs(xt - xs)) <= 1e-10
