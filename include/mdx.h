@@ -142,6 +142,36 @@ int mdx_neigh_pairs(mdx_ctx *ctx, int *pairs, long cap, long *npairs);
 int mdx_compute(mdx_ctx *ctx, int eflag, int vflag, double *f, double *eng_vdwl, double *virial,
                 double *eatom, double *vatom);
 
+/* Limits of mdx_compute_noalloc (a file-scope device array of
+   MDX_NOALLOC_SCRATCH_BYTES per GPU holds its work arrays). */
+#define MDX_NOALLOC_MAX_ATOMS 32768
+#define MDX_NOALLOC_NEIGH_PER_ATOM 128 /* list entries per atom at the maximum atom count */
+#define MDX_NOALLOC_MAX_BINS (1 << 19)
+#define MDX_NOALLOC_SCRATCH_BYTES (36L << 20)
+
+/* mdx_compute without device allocation.  No counterpart in the reference:
+   for callers that cannot allocate device memory once set up (serving
+   loops, harnesses that forbid hipMalloc after start-up).  One
+   Pair::compute on n atoms whose positions x ([n][3] doubles) and types
+   (int[n], 1-based) are DEVICE memory of the context's GPU, in an order the
+   caller chooses; f ([n][3] doubles, device) receives f += the pair forces
+   in that order, eng_vdwl and virial[6] (host, optional) the global sums
+   when eflag/vflag ask for them (per-atom outputs are not offered).  The
+   context supplies the box, pair style and tables, precision, neighbour
+   skin and `one`, and the launch shape; its own atoms, if any, are neither
+   read nor changed.  The list is built from scratch at every call (skin
+   included, as mdx_compute's first build), in a file-scope device array
+   of MDX_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends,
+   so n <= MDX_NOALLOC_MAX_ATOMS, at most MDX_NOALLOC_MAX_BINS bins, and at
+   most MDX_NOALLOC_MAX_ATOMS * MDX_NOALLOC_NEIGH_PER_ATOM / n (and `one`)
+   neighbours per atom; beyond these the call returns MDX_ERR_ARG or
+   MDX_ERR_OVERFLOW without computing.  The call allocates nothing, is
+   synchronous at return, and every call on one GPU shares that GPU's
+   array: calls that overlap on one GPU (from several threads or contexts)
+   race. */
+int mdx_compute_noalloc(mdx_ctx *ctx, int n, const double *x, const int *type, int eflag, int vflag, double *f,
+                        double *eng_vdwl, double *virial);
+
 /* EAM intermediate arrays after the last compute (rho, fp), caller order. */
 int mdx_eam_rho_fp(mdx_ctx *ctx, double *rho, double *fp);
 

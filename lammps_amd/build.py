@@ -24,7 +24,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdx.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["api.cpp", "host_setup.cpp", "neigh.hip", "pair.hip", "integrate.hip", "batch.hip", "halo.hip"]
+SOURCES = ["api.cpp", "host_setup.cpp", "neigh.hip", "pair.hip", "integrate.hip", "batch.hip", "halo.hip",
+           "noalloc.hip"]
 ARCH = "gfx950"
 
 

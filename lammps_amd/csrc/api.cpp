@@ -123,8 +123,10 @@ void update_qpos(mdx_ctx *c) {
   c->qpos = on;
 }
 
+}  // namespace
+
 // Pair::init + Neighbor::init cutoffs (src/pair.cpp:219-231, src/neighbor.cpp:239-279)
-int prepare(mdx_ctx *c) {
+int mdx::prepare_ctx(mdx_ctx *c) {
   if (!c->box_set) {
     set_error("box not set (mdx_set_box)");
     return MDX_ERR_ARG;
@@ -173,11 +175,19 @@ int prepare(mdx_ctx *c) {
   // enough blocks for any launch: block >= 64 threads, up to 32 threads per atom
   const long G = ((long)c->n * 32 + 63) / 64 + 1 + kXCD;
   if (G * 8 > c->cap_partials) {
+    if (c->fixed_storage) {
+      set_error("reduction partials exceed the fixed storage");
+      return MDX_ERR_ARG;
+    }
     if ((rc = dev_alloc(&c->partials, (size_t)G * 8))) return rc;
     c->cap_partials = G * 8;
   }
   return MDX_OK;
 }
+
+namespace {
+
+int prepare(mdx_ctx *c) { return prepare_ctx(c); }
 
 int ensure_built(mdx_ctx *c) {
   int rc = prepare(c);
@@ -228,6 +238,11 @@ int mdx_create(int device, mdx_ctx **out) {
   if (hipMemset(c->dflags, 0, 64) != hipSuccess) {
     set_error("device flags initialisation failed");
     return MDX_ERR_HIP;
+  }
+  int rc = noalloc_init(device);  // this device's scratch address, once per device
+  if (rc) {
+    (void)mdx_destroy(c);
+    return rc;
   }
   *out = c;
   return MDX_OK;
@@ -416,6 +431,7 @@ int mdx_pair_lj_init(mdx_ctx *c, int ntypes, const double *cutsq, const double *
   hipError_t e = hipMemcpy(c->lj_dev, &t, sizeof(LJTab), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "lj table upload");
   c->pair_style = MDX_PAIR_LJ;
+  c->pair_ntypes = ntypes;
   c->built = false;
   c->run_setup_done = false;
   return MDX_OK;
@@ -516,6 +532,7 @@ int mdx_pair_eam_init(mdx_ctx *c, int ntypes, int nrho, double drho, int nr, dou
   for (int i = 1; i <= ntypes; i++)
     for (int j = 1; j <= ntypes; j++) c->cutsq[i * nt1 + j] = cutforcesq;
   c->pair_style = MDX_PAIR_EAM;
+  c->pair_ntypes = ntypes;
   c->built = false;
   c->run_setup_done = false;
   return MDX_OK;
@@ -626,6 +643,13 @@ int mdx_compute(mdx_ctx *c, int eflag, int vflag, double *f, double *eng_vdwl, d
   if (vatom && ev && (rc = gather_soa(c, c->vatom, 6, vatom))) return rc;
   hipError_t e = hipStreamSynchronize(c->stream);
   return e == hipSuccess ? MDX_OK : hip_fail(e, "compute");
+}
+
+int mdx_compute_noalloc(mdx_ctx *c, int n, const double *x, const int *type, int eflag, int vflag, double *f,
+                        double *eng_vdwl, double *virial) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  return compute_noalloc(c, n, x, type, eflag, vflag, f, eng_vdwl, virial);
 }
 
 int mdx_eam_rho_fp(mdx_ctx *c, double *rho, double *fp) {

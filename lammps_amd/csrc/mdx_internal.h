@@ -370,6 +370,11 @@ struct mdx_ctx {
   long kernel_n[MDX_NKERNEL] = {0};
   long total_neighbors = 0;
   int max_neighbors = 0;
+  int pair_ntypes = 0;  // ntypes of the last pair init (atoms may not be set)
+  // storage owned by someone else (mdx_compute_noalloc's scratch): growth
+  // paths report an error instead of freeing and allocating
+  bool fixed_storage = false;
+  long nbr_cap_elems = 0;  // list entries available when fixed_storage
 };
 
 namespace mdx {
@@ -399,6 +404,13 @@ void halo_free(mdx_ctx *c);
 void prof_begin(mdx_ctx *c, int kind);
 void prof_end(mdx_ctx *c);
 int prof_resolve(mdx_ctx *c);
+
+// Pair::init + Neighbor::init cutoffs, bin grid, first list and partials (api.cpp)
+int prepare_ctx(mdx_ctx *c);
+// no-allocation compute (noalloc.hip)
+int noalloc_init(int device);
+int compute_noalloc(mdx_ctx *c, int n, const double *x, const int *type, int eflag, int vflag, double *f,
+                    double *eng_vdwl, double *virial);
 
 // neighbour build (neigh.hip)
 int neigh_setup_grid(mdx_ctx *c);

@@ -506,6 +506,10 @@ int neigh_setup_grid(mdx_ctx *c) {
   }
   c->nbins = (int)nb;
   if (c->nbins + 1 > c->cap_bins) {
+    if (c->fixed_storage) {
+      set_error("the box needs more neighbour bins than the fixed storage holds");
+      return MDX_ERR_ARG;
+    }
     (void)hipFree(c->bin_count);
     (void)hipFree(c->bin_start);
     c->cap_bins = c->nbins + 1;
@@ -527,6 +531,15 @@ int neigh_setup_grid(mdx_ctx *c) {
 int alloc_list(mdx_ctx *c, int cap) {
   // a multiple of 4 entries: rows start 16-B aligned (k_build_lane's stores)
   cap = (std::max(cap, 16) + 3) / 4 * 4;
+  if (c->fixed_storage) {  // the list lives in storage this context does not own
+    if ((long)cap * c->npad > c->nbr_cap_elems) {
+      set_error("Neighbor list overflow: more neighbours per atom than the fixed storage holds");
+      return MDX_ERR_OVERFLOW;
+    }
+    c->maxneigh = cap;
+    c->nbr_elems = (long)cap * c->npad;
+    return MDX_OK;
+  }
   (void)hipFree(c->nbr);
   c->nbr = nullptr;
   if (hipMalloc(&c->nbr, sizeof(int) * (size_t)cap * c->npad) != hipSuccess) {
@@ -561,6 +574,10 @@ int neigh_build(mdx_ctx *c) {
   size_t need = 0;
   (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
   if (need > c->scan_tmp_bytes) {
+    if (c->fixed_storage) {
+      set_error("bin scan storage exceeds the fixed storage");
+      return MDX_ERR_ARG;
+    }
     (void)hipFree(c->scan_tmp);
     c->scan_tmp_bytes = need;
     if (hipMalloc(&c->scan_tmp, need) != hipSuccess) {

@@ -33,6 +33,7 @@ def hip_runtime_paths():
 FP64, MIXED = 0, 1
 POS_FP64, POS_Q32 = 0, 1
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_OVERFLOW = 0, -1, -2, -3, -4
+NOALLOC_MAX_ATOMS = 32768  # MDX_NOALLOC_MAX_ATOMS (include/mdx.h)
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -103,6 +104,7 @@ def lib():
             "mdx_halo_sendlists": [_p, _i, _dp, _dp, _dp, _ip],
             "mdx_halo_pack": [_p, _i, ctypes.c_void_p],
             "mdx_halo_unpack": [_p, _i, _i, ctypes.c_void_p],
+            "mdx_compute_noalloc": [_p, _i, ctypes.c_void_p, ctypes.c_void_p, _i, _i, ctypes.c_void_p, _dp, _dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -310,6 +312,16 @@ class Context:
         va = np.zeros((n, 6)) if peratom else None
         self._ck(self._L.mdx_compute(self.h, eflag, vflag, _ptr(f), ctypes.byref(e), _ptr(w), _ptr(ea), _ptr(va)))
         return {"f": f, "eng": e.value, "virial": w, "eatom": ea, "vatom": va}
+
+    def compute_noalloc(self, n, x_dev, type_dev, f_dev, eflag=1, vflag=1):
+        """mdx_compute_noalloc on device arrays given as integer addresses
+        (x [n][3] and f [n][3] doubles, type int32[n], device memory of the
+        context's GPU): f += the pair forces; returns (eng_vdwl, virial)."""
+        e = ctypes.c_double(0.0)
+        w = np.zeros(6)
+        self._ck(self._L.mdx_compute_noalloc(self.h, n, ctypes.c_void_p(x_dev), ctypes.c_void_p(type_dev), eflag,
+                                             vflag, ctypes.c_void_p(f_dev), ctypes.byref(e), _ptr(w)))
+        return e.value, w
 
     def eam_rho_fp(self):
         r = np.zeros(self.n); fp = np.zeros(self.n)
