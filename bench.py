@@ -423,13 +423,16 @@ def fit_lj_ka(reps=10, check_oracle=True):
         fm.batch.autotune(True)
         loss_and_grad(fm)
         torch.cuda.synchronize()
+        k0 = [fm.batch.kernel_time(k)[0] for k in (1, 2)]
         t0 = time.perf_counter()
         for _ in range(reps):
             loss, ge, gs, F = loss_and_grad(fm)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
+        kt = [(fm.batch.kernel_time(k)[0] - k0[q]) / reps for q, k in enumerate((1, 2))]
         res[name] = dict(loss=loss, grad_eps=ge.tolist(), grad_sigma=gs.tolist(), ms_per_loss_and_grad=dt * 1e3,
-                         configs_per_s=S / dt, atom_forces_per_s=S * N / dt)
+                         configs_per_s=S / dt, atom_forces_per_s=S * N / dt,
+                         kernel_ms={"lj": kt[0], "lj_vjp": kt[1]})
     # central differences of the device FP64 loss (h = 1e-6 relative)
     fd = []
     for which, base in ((0, eps), (1, sig)):
@@ -495,24 +498,33 @@ def eam_replicas(reps=10, check_oracle=True, fd_knots=6):
     b64.eam_init(p["drho"], p["dr"], p["cut"], *target)
     Ft, _ = b64.eam_forces(energy=False)
     b64.eam_init(p["drho"], p["dr"], p["cut"], p["frho"], p["rhor"], p["z2r"])
+    import torch
+    Ftd = torch.tensor(Ft, device="cuda")
     res = {}
     for name, prec in (("mixed", B.MIXED), ("fp64", B.FP64)):
         b = make(prec)
         b.eam_init(p["drho"], p["dr"], p["cut"], p["frho"], p["rhor"], p["z2r"])
         b.autotune(True)
-        F, _ = b.eam_forces()
-        b.eam_vjp(2.0 * (F - Ft) / (3.0 * n))
+        Fd = torch.empty((n, 3), dtype=torch.float64, device="cuda")  # forces stay on the device
+        b.eam_forces(out=Fd)
+        Gd = (2.0 / (3.0 * n)) * (Fd - Ftd)
+        b.eam_vjp(Gd)
+        k0 = [b.kernel_time(k)[0] for k in (3, 4, 5, 6)]
         t0 = time.perf_counter()
         for _ in range(reps):
-            F, e = b.eam_forces()
+            _, e = b.eam_forces(out=Fd)
         t_fwd = (time.perf_counter() - t0) / reps
-        G = 2.0 * (F - Ft) / (3.0 * n)
+        Gd = (2.0 / (3.0 * n)) * (Fd - Ftd)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            g = b.eam_vjp(G)
+            g = b.eam_vjp(Gd)
         t_adj = (time.perf_counter() - t0) / reps
+        kt = [(b.kernel_time(k)[0] - k0[q]) / reps for q, k in enumerate((3, 4, 5, 6))]
+        F = Fd.cpu().numpy()
         res[name] = dict(ms_forward=t_fwd * 1e3, ms_adjoint=t_adj * 1e3, replica_forces_per_s=len(natoms) / t_fwd,
                          replica_force_and_grad_per_s=len(natoms) / (t_fwd + t_adj),
+                         kernel_ms={"eam_rho": kt[0], "eam_force": kt[1], "eam_adj_a": kt[2], "eam_adj_b": kt[3]},
                          launch={B.KIND_NAMES[k]: b.get_launch(k) for k in (3, 4, 5, 6)},
                          loss=float(np.sum((F - Ft) ** 2) / (3.0 * n)))
         res[name]["grad"] = g

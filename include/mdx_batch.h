@@ -46,6 +46,13 @@ int mdx_batch_create(int device, int nrep, const int *natoms, const double *lo, 
                      const int *periodic, int ntypes, const int *type, mdx_batch **out);
 int mdx_batch_destroy(mdx_batch *b);
 
+/* Stream of the batch's work: external = 0 its own stream (the default),
+   1 the caller's `stream` (a hipStream_t of the batch's device; NULL is that
+   device's null stream), so the kernels queue behind the caller's work on
+   that stream (e.g. PyTorch's current stream) with no device-wide
+   synchronisation.  Calls still return when their work is done. */
+int mdx_batch_set_stream(mdx_batch *b, int external, void *stream);
+
 /* positions [N][3], host or device; wrapped into periodic boxes */
 int mdx_batch_set_positions(mdx_batch *b, const double *x);
 /* MDX_FP64 or MDX_MIXED (FP64 positions and distances, FP32 potential

@@ -43,6 +43,7 @@ def lib():
             "mdx_batch_autotune": [_p, _i],
             "mdx_batch_kernel_time": [_p, _i, _p, _p],
             "mdx_batch_neigh_stats": [_p, _p, _p],
+            "mdx_batch_set_stream": [_p, _i, _p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -102,6 +103,15 @@ class Batch:
             x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.n, 3)
         self._keep_x = x
         self._ck(self._L.mdx_batch_set_positions(self.h, _addr(x)))
+
+    def set_stream(self, stream=None):
+        """Queue the batch's work on `stream` (a torch.cuda.Stream, or an
+        integer handle; None: the batch's own stream)."""
+        if stream is None:
+            self._ck(self._L.mdx_batch_set_stream(self.h, 0, None))
+        else:
+            handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+            self._ck(self._L.mdx_batch_set_stream(self.h, 1, ctypes.c_void_p(handle)))
 
     def precision(self, p):
         self._ck(self._L.mdx_batch_set_precision(self.h, p))

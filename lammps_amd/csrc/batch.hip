@@ -47,9 +47,31 @@ constexpr int kTabLDS = 6144;  // doubles of spline table staged in LDS at most 
 
 struct RepGeom {
   double lo[3], prd[3], bininv[3];
+  // 32-bit fixed-point positions of the mixed-precision force passes:
+  // q = round((x - lo) * qscale) (mod 2^32 periodic, where the wrapping
+  // difference is the minimum-image separation), qstep = prd * 2^-32
+  // (periodic) or prd * 2^-30, exact
+  double qscale[3], qstep[3];
   int nbin[3], periodic[3];
   int bin_off;
 };
+
+// a pair within kQBand of the cutoff (relative, on rsq) from the fixed-point
+// separation is re-tested on the FP64 positions, so the cutoff decision is
+// the FP64 one
+constexpr double kQBand = 2e-6;
+
+__device__ __forceinline__ int4 quantize_rep(const RepGeom &g, const double4 &p) {
+  return make_int4((int)(unsigned)(long long)rint((p.x - g.lo[0]) * g.qscale[0]),
+                   (int)(unsigned)(long long)rint((p.y - g.lo[1]) * g.qscale[1]),
+                   (int)(unsigned)(long long)rint((p.z - g.lo[2]) * g.qscale[2]), (int)p.w);
+}
+
+__device__ __forceinline__ double qsep(int qi, int qj, double step) {
+  return (double)(int)((unsigned)qi - (unsigned)qj) * step;
+}
+
+
 
 // lj/cut tables (ntypes+1)^2: cutsq, lj1..lj4 (init_one), and the
 // gradient factors A1 = 48 s^12, A2 = 24 s^6, B1 = 576 eps s^11, B2 = 144 eps s^5
@@ -85,6 +107,38 @@ __device__ __forceinline__ void shift_of(int code, double &sx, double &sy, doubl
   sx = (double)(code / 9 - 1);
   sy = (double)((code / 3) % 3 - 1);
   sz = (double)(code % 3 - 1);
+}
+
+// Separation of the pair (i, list entry) and whether it lies inside cutsq:
+// from the FP64 positions with the entry's image shift, or (MIXED) from the
+// fixed-point mirror with the FP64 re-test within kQBand of the cutoff.
+template <bool MIXED>
+__device__ __forceinline__ bool b_pair(const double4 &xi, const int4 &qi, const double4 *__restrict__ x,
+                                       const int4 *__restrict__ xq, const RepGeom &g, int ent, double cutsq,
+                                       double &dx, double &dy, double &dz, double &rsq) {
+  const int j = ent & MDX_IDX_MASK;
+  if (MIXED) {
+    const int4 qj = xq[j];
+    dx = qsep(qi.x, qj.x, g.qstep[0]);
+    dy = qsep(qi.y, qj.y, g.qstep[1]);
+    dz = qsep(qi.z, qj.z, g.qstep[2]);
+    rsq = dx * dx + dy * dy + dz * dz;
+    if (rsq < cutsq * (1.0 - kQBand)) return true;
+    if (!(rsq < cutsq * (1.0 + kQBand))) return false;
+  }
+  double sx, sy, sz;
+  shift_of((int)((unsigned)ent >> MDX_IDX_BITS), sx, sy, sz);
+  const double4 xj = x[j];
+  const double ex = xi.x - (xj.x + sx * g.prd[0]), ey = xi.y - (xj.y + sy * g.prd[1]),
+               ez = xi.z - (xj.z + sz * g.prd[2]);
+  const double esq = ex * ex + ey * ey + ez * ez;
+  if (!MIXED) {
+    dx = ex;
+    dy = ey;
+    dz = ez;
+    rsq = esq;
+  }
+  return esq < cutsq;
 }
 
 __device__ __forceinline__ int local_bin(const RepGeom &g, double x, int d) {
@@ -126,12 +180,15 @@ __global__ void k_b_load(int n, const double *__restrict__ xin, const int *__res
 }
 
 __global__ void k_b_permute(int n, const int *__restrict__ idx_s, const double4 *__restrict__ xw,
-                            const int *__restrict__ rep_c, double4 *__restrict__ x, int *__restrict__ rep,
-                            int *__restrict__ orig, int *__restrict__ inv) {
+                            const int *__restrict__ rep_c, const RepGeom *__restrict__ geom, double4 *__restrict__ x,
+                            int4 *__restrict__ xq, int *__restrict__ rep, int *__restrict__ orig,
+                            int *__restrict__ inv) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int c = idx_s[k];
-  x[k] = xw[c];
+  const double4 p = xw[c];
+  x[k] = p;
+  xq[k] = quantize_rep(geom[rep_c[c]], p);
   rep[k] = rep_c[c];
   orig[k] = c;
   inv[c] = k;
@@ -209,13 +266,30 @@ __global__ __launch_bounds__(1024) void k_b_build(int n, const double4 *__restri
 
 // ---------------------------------------------------------------- lj/cut
 
+// MIXED: neighbours are read from the fixed-point mirror (16 B instead of
+// 32, no image decode: the wrapping difference is the minimum-image
+// separation), separations rebuilt in FP64, a pair within kQBand of the
+// cutoff re-tested on the FP64 positions with its image shift; FP32 pair
+// arithmetic, FP64 accumulation.
 template <bool MIXED, int TPA>
 __global__ __launch_bounds__(1024) void k_b_lj(int n, int nt1, const double4 *__restrict__ x,
-                                               const int *__restrict__ rep, const RepGeom *__restrict__ geom,
-                                               const int *__restrict__ nbr, const int *__restrict__ numneigh, int cap,
+                                               const int4 *__restrict__ xq, const int *__restrict__ rep,
+                                               const RepGeom *__restrict__ geom, const int *__restrict__ nbr,
+                                               const int *__restrict__ numneigh, int cap,
                                                const double *__restrict__ tab, double4 *__restrict__ f) {
   __shared__ double s_tab[T_N * kNT];
+  __shared__ double s_qlo[MIXED ? kNT : 1], s_qhi[MIXED ? kNT : 1];
+  __shared__ float s_f[MIXED ? 4 * kNT : 1];
   for (int k = threadIdx.x; k < T_N * kNT; k += blockDim.x) s_tab[k] = tab[k];
+  if (MIXED)
+    for (int k = threadIdx.x; k < kNT; k += blockDim.x) {
+      s_qlo[k] = tab[T_CUTSQ * kNT + k] * (1.0 - kQBand);
+      s_qhi[k] = tab[T_CUTSQ * kNT + k] * (1.0 + kQBand);
+      s_f[k] = (float)tab[T_LJ1 * kNT + k];
+      s_f[kNT + k] = (float)tab[T_LJ2 * kNT + k];
+      s_f[2 * kNT + k] = (float)tab[T_LJ3 * kNT + k];
+      s_f[3 * kNT + k] = (float)tab[T_LJ4 * kNT + k];
+    }
   __syncthreads();
   const int apb = blockDim.x / TPA;
   int end;
@@ -231,34 +305,54 @@ __global__ __launch_bounds__(1024) void k_b_lj(int n, int nt1, const double4 *__
     const int *row = nbr + (size_t)i * cap;
     const int jn = numneigh[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-    for (int k = lane; k < jn; k += TPA) {
-      const int ent = row[k];
-      const int j = ent & MDX_IDX_MASK;
-      double sx, sy, sz;
-      shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
-      const double4 xj = x[j];
-      const double dx = xi.x - (xj.x + sx * px), dy = xi.y - (xj.y + sy * py), dz = xi.z - (xj.z + sz * pz);
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      const int t = ti * nt1 + (int)xj.w;
-      if (rsq < s_tab[T_CUTSQ * kNT + t]) {
-        double fpair, ev;
-        if (MIXED) {
-          const float r2inv = 1.0f / (float)rsq;
+    if constexpr (MIXED) {
+      const int4 qi = xq[i];
+      const double q0 = g.qstep[0], q1 = g.qstep[1], q2 = g.qstep[2];
+      for (int k = lane; k < jn; k += TPA) {
+        const int ent = row[k];
+        const int j = ent & MDX_IDX_MASK;
+        const int4 qj = xq[j];
+        const double dx = qsep(qi.x, qj.x, q0), dy = qsep(qi.y, qj.y, q1), dz = qsep(qi.z, qj.z, q2);
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        const int t = ti * nt1 + qj.w;
+        bool inside = rsq < s_qlo[t];
+        if (!inside && rsq < s_qhi[t]) {
+          double sx, sy, sz;
+          shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
+          const double4 xj = x[j];
+          const double ex = xi.x - (xj.x + sx * px), ey = xi.y - (xj.y + sy * py), ez = xi.z - (xj.z + sz * pz);
+          inside = ex * ex + ey * ey + ez * ez < s_tab[T_CUTSQ * kNT + t];
+        }
+        if (inside) {
+          const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
           const float r6inv = r2inv * r2inv * r2inv;
-          const float forcelj = r6inv * ((float)s_tab[T_LJ1 * kNT + t] * r6inv - (float)s_tab[T_LJ2 * kNT + t]);
-          fpair = (double)(forcelj * r2inv);
-          ev = (double)(r6inv * ((float)s_tab[T_LJ3 * kNT + t] * r6inv - (float)s_tab[T_LJ4 * kNT + t]));
-        } else {
+          const double fpair = (double)(r6inv * (s_f[t] * r6inv - s_f[kNT + t]) * r2inv);
+          fx += dx * fpair;
+          fy += dy * fpair;
+          fz += dz * fpair;
+          e += 0.5 * (double)(r6inv * (s_f[2 * kNT + t] * r6inv - s_f[3 * kNT + t]));
+        }
+      }
+    } else {
+      for (int k = lane; k < jn; k += TPA) {
+        const int ent = row[k];
+        const int j = ent & MDX_IDX_MASK;
+        double sx, sy, sz;
+        shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
+        const double4 xj = x[j];
+        const double dx = xi.x - (xj.x + sx * px), dy = xi.y - (xj.y + sy * py), dz = xi.z - (xj.z + sz * pz);
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        const int t = ti * nt1 + (int)xj.w;
+        if (rsq < s_tab[T_CUTSQ * kNT + t]) {
           const double r2inv = 1.0 / rsq;
           const double r6inv = r2inv * r2inv * r2inv;
           const double forcelj = r6inv * (s_tab[T_LJ1 * kNT + t] * r6inv - s_tab[T_LJ2 * kNT + t]);
-          fpair = forcelj * r2inv;
-          ev = r6inv * (s_tab[T_LJ3 * kNT + t] * r6inv - s_tab[T_LJ4 * kNT + t]);
+          const double fpair = forcelj * r2inv;
+          fx += dx * fpair;
+          fy += dy * fpair;
+          fz += dz * fpair;
+          e += 0.5 * (r6inv * (s_tab[T_LJ3 * kNT + t] * r6inv - s_tab[T_LJ4 * kNT + t]));
         }
-        fx += dx * fpair;
-        fy += dy * fpair;
-        fz += dz * fpair;
-        e += 0.5 * ev;
       }
     }
     fx = gsum<TPA>(fx);
@@ -368,7 +462,8 @@ __device__ __forceinline__ void r_spline(const EamTab &et, T r, int &m, T &p) {
 }
 
 template <bool MIXED, int TPA>
-__global__ __launch_bounds__(1024) void k_b_eam_rho(int n, const double4 *__restrict__ x, const int *__restrict__ rep,
+__global__ __launch_bounds__(1024) void k_b_eam_rho(int n, const double4 *__restrict__ x,
+                                                    const int4 *__restrict__ xq, const int *__restrict__ rep,
                                                     const RepGeom *__restrict__ geom, const int *__restrict__ nbr,
                                                     const int *__restrict__ numneigh, int cap, EamTab et,
                                                     const double *__restrict__ rhor_d, const float *__restrict__ rhor_f,
@@ -389,20 +484,14 @@ __global__ __launch_bounds__(1024) void k_b_eam_rho(int n, const double4 *__rest
     const int i = base + threadIdx.x / TPA;
     if (i >= end) continue;
     const double4 xi = x[i];
+    const int4 qi = MIXED ? xq[i] : make_int4(0, 0, 0, 0);
     const RepGeom &g = geom[rep[i]];
-    const double px = g.prd[0], py = g.prd[1], pz = g.prd[2];
     const int *row = nbr + (size_t)i * cap;
     const int jn = numneigh[i];
     double acc = 0.0;
     for (int k = lane; k < jn; k += TPA) {
-      const int ent = row[k];
-      const int j = ent & MDX_IDX_MASK;
-      double sx, sy, sz;
-      shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
-      const double4 xj = x[j];
-      const double dx = xi.x - (xj.x + sx * px), dy = xi.y - (xj.y + sy * py), dz = xi.z - (xj.z + sz * pz);
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      if (rsq < et.cutsq) {
+      double dx, dy, dz, rsq;
+      if (b_pair<MIXED>(xi, qi, x, xq, g, row[k], et.cutsq, dx, dy, dz, rsq)) {
         int m;
         T p;
         if constexpr (MIXED) (void)eam_index_mixed(rsq, et.rdr, et.nr, m, p);
@@ -432,7 +521,7 @@ __global__ __launch_bounds__(1024) void k_b_eam_rho(int n, const double4 *__rest
 
 template <bool MIXED, int TPA>
 __global__ __launch_bounds__(1024) void k_b_eam_force(int n, const double4 *__restrict__ x,
-                                                      const int *__restrict__ rep, const RepGeom *__restrict__ geom,
+                                                      const int4 *__restrict__ xq, const int *__restrict__ rep, const RepGeom *__restrict__ geom,
                                                       const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                                       int cap, EamTab et, const double *__restrict__ rhor_d,
                                                       const double *__restrict__ z2r_d, const float *__restrict__ rhor_f,
@@ -459,22 +548,17 @@ __global__ __launch_bounds__(1024) void k_b_eam_force(int n, const double4 *__re
     const int i = base + threadIdx.x / TPA;
     if (i >= end) continue;
     const double4 xi = x[i];
+    const int4 qi = MIXED ? xq[i] : make_int4(0, 0, 0, 0);
     const double fpi = fp[i];
     const RepGeom &g = geom[rep[i]];
-    const double px = g.prd[0], py = g.prd[1], pz = g.prd[2];
     const int *row = nbr + (size_t)i * cap;
     const int jn = numneigh[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     for (int k = lane; k < jn; k += TPA) {
       const int ent = row[k];
-      const int j = ent & MDX_IDX_MASK;
-      double sx, sy, sz;
-      shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
-      const double4 xj = x[j];
-      const double dx = xi.x - (xj.x + sx * px), dy = xi.y - (xj.y + sy * py), dz = xi.z - (xj.z + sz * pz);
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      if (rsq < et.cutsq) {
-        const double fpj = fp[j];
+      double dx, dy, dz, rsq;
+      if (b_pair<MIXED>(xi, qi, x, xq, g, ent, et.cutsq, dx, dy, dz, rsq)) {
+        const double fpj = fp[ent & MDX_IDX_MASK];
         int m;
         T p, r;
         if constexpr (MIXED) r = eam_index_mixed(rsq, et.rdr, et.nr, m, p);
@@ -487,7 +571,7 @@ __global__ __launch_bounds__(1024) void k_b_eam_force(int n, const double4 *__re
         c = z2r + m * 7;
         const T z2p = (c[0] * p + c[1]) * p + c[2];
         const T z2 = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
-        const T recip = (T)1 / r;
+        const T recip = MIXED ? (T)__builtin_amdgcn_rcpf((float)r) : (T)1 / r;
         const T phi = z2 * recip;
         const T phip = z2p * recip - phi * recip;
         const T psip = (T)fpi * rhop + (T)fpj * rhop + phip;
@@ -656,6 +740,30 @@ __global__ __launch_bounds__(1024) void k_b_eam_adj_b(int n, const double4 *__re
 
 // ---------------------------------------------------------------- utilities
 
+// Column sums of the per-block partial rows, in two fixed-order passes
+// (deterministic): k_b_colsum_chunks sums chunk ry of the rows (rows
+// [rows*ry/nchunk, rows*(ry+1)/nchunk)) for 64 columns per block, four row
+// lanes per column combined in LDS; k_b_colsum then sums the nchunk chunk
+// rows.  One thread per column over every row leaves the device mostly idle
+// (a few hundred columns, thousands of rows).
+constexpr int kColChunks = 64;
+__global__ __launch_bounds__(256) void k_b_colsum_chunks(const double *__restrict__ part, int rows, int cols,
+                                                         double *__restrict__ mid) {
+  __shared__ double s[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), sub = threadIdx.x >> 6;
+  const int ry = blockIdx.y, nchunk = gridDim.y;
+  const int r0 = (int)((long)rows * ry / nchunk), r1 = (int)((long)rows * (ry + 1) / nchunk);
+  double acc = 0.0;
+  if (c < cols)
+    for (int r = r0 + sub; r < r1; r += 4) acc += part[(size_t)r * cols + c];
+  s[sub][threadIdx.x & 63] = acc;
+  __syncthreads();
+  if (sub == 0 && c < cols) {
+    const int l = threadIdx.x & 63;
+    mid[(size_t)ry * cols + c] = ((s[0][l] + s[1][l]) + s[2][l]) + s[3][l];
+  }
+}
+
 // column sums over rows in a fixed order (deterministic)
 __global__ void k_b_colsum(const double *__restrict__ part, int rows, int cols, double *__restrict__ out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -709,7 +817,8 @@ using namespace mdx;
 
 struct mdx_batch {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where the batch's work goes: own_stream or the caller's
+  hipStream_t own_stream = nullptr;  // created with the batch
   int nrep = 0, ntypes = 0, n = 0, precision = MDX_FP64;
   double skin = 0.0;
   std::vector<int> natoms, off;
@@ -723,6 +832,7 @@ struct mdx_batch {
   double *xin = nullptr;  // caller order staging [N][3]
   int *type_c = nullptr, *rep_c = nullptr;
   double4 *xw = nullptr, *x = nullptr, *f = nullptr, *g = nullptr;
+  int4 *xq = nullptr;  // fixed-point copy of x (mixed-precision force passes)
   int *rep = nullptr, *orig = nullptr, *inv = nullptr;
   unsigned *key = nullptr, *key_s = nullptr;
   int *idx = nullptr, *idx_s = nullptr;
@@ -881,13 +991,17 @@ int ensure_partials(mdx_batch *b, size_t doubles) {
 }
 
 int colsum(mdx_batch *b, int rows, int cols, std::vector<double> &out) {
-  if ((size_t)cols > b->colsum_cap) {
-    int rc = dalloc(&b->colsum, cols);
+  const size_t need = (size_t)(kColChunks + 1) * cols;  // chunk sums, then the result
+  if (need > b->colsum_cap) {
+    int rc = dalloc(&b->colsum, need);
     if (rc) return rc;
-    b->colsum_cap = cols;
+    b->colsum_cap = need;
   }
-  hipLaunchKernelGGL(k_b_colsum, dim3((cols + 255) / 256), dim3(256), 0, b->stream, b->partials, rows, cols,
-                     b->colsum);
+  const int nchunk = std::max(1, std::min(kColChunks, rows));
+  double *mid = b->colsum + cols;
+  hipLaunchKernelGGL(k_b_colsum_chunks, dim3((cols + 63) / 64, nchunk), dim3(256), 0, b->stream, b->partials, rows,
+                     cols, mid);
+  hipLaunchKernelGGL(k_b_colsum, dim3((cols + 255) / 256), dim3(256), 0, b->stream, mid, nchunk, cols, b->colsum);
   out.assign(cols, 0.0);
   hipError_t e = hipMemcpyAsync(out.data(), b->colsum, sizeof(double) * cols, hipMemcpyDeviceToHost, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -981,8 +1095,8 @@ int build(mdx_batch *b) {
   if ((e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp, scb, b->bin_count, b->bin_start, b->total_bins + 1,
                                             b->stream)) != hipSuccess)
     return bfail(e, "bin scan");
-  hipLaunchKernelGGL(k_b_permute, dim3((n + 255) / 256), dim3(256), 0, b->stream, n, b->idx_s, b->xw, b->rep_c, b->x,
-                     b->rep, b->orig, b->inv);
+  hipLaunchKernelGGL(k_b_permute, dim3((n + 255) / 256), dim3(256), 0, b->stream, n, b->idx_s, b->xw, b->rep_c,
+                     b->geom, b->x, b->xq, b->rep, b->orig, b->inv);
   if (b->cap == 0) {
     b->cap = 96;
     if ((rc = dalloc(&b->nbr, (size_t)b->cap * n))) return rc;
@@ -1163,15 +1277,19 @@ int mdx_batch_create(int device, int nrep, const int *natoms, const double *lo, 
       b->geom_h[r].periodic[d] = periodic[3 * r + d] ? 1 : 0;
       b->geom_h[r].nbin[d] = 1;
       b->geom_h[r].bininv[d] = 1.0 / b->geom_h[r].prd[d];
+      const double span = periodic[3 * r + d] ? 4294967296.0 : 1073741824.0;
+      b->geom_h[r].qscale[d] = span / b->geom_h[r].prd[d];
+      b->geom_h[r].qstep[d] = b->geom_h[r].prd[d] / span;
     }
   hipError_t e;
   int rc;
-  if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&b->stream)) != hipSuccess ||
+  if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&b->own_stream)) != hipSuccess ||
       (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
     rc = bfail(e, "batch stream");
     mdx_batch_destroy(b);
     return rc;
   }
+  b->stream = b->own_stream;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     b->num_cu = prop.multiProcessorCount;
@@ -1183,7 +1301,8 @@ int mdx_batch_create(int device, int nrep, const int *natoms, const double *lo, 
     for (int i = b->off[r]; i < b->off[r + 1]; i++) rep_c[i] = r;
   if ((rc = dalloc(&b->geom, nrep)) || (rc = dalloc(&b->off_d, nrep + 1)) || (rc = dalloc(&b->xin, 3 * (size_t)n)) ||
       (rc = dalloc(&b->staging, std::max<size_t>(3 * (size_t)n, nrep))) || (rc = dalloc(&b->type_c, n)) || (rc = dalloc(&b->rep_c, n)) ||
-      (rc = dalloc(&b->xw, n)) || (rc = dalloc(&b->x, n)) || (rc = dalloc(&b->f, n)) || (rc = dalloc(&b->g, n)) ||
+      (rc = dalloc(&b->xw, n)) || (rc = dalloc(&b->x, n)) || (rc = dalloc(&b->xq, n)) || (rc = dalloc(&b->f, n)) ||
+      (rc = dalloc(&b->g, n)) ||
       (rc = dalloc(&b->rep, n)) || (rc = dalloc(&b->orig, n)) || (rc = dalloc(&b->inv, n)) ||
       (rc = dalloc(&b->key, n)) || (rc = dalloc(&b->key_s, n)) || (rc = dalloc(&b->idx, n)) ||
       (rc = dalloc(&b->idx_s, n)) || (rc = dalloc(&b->numneigh, n)) || (rc = dalloc(&b->dflags, 16)) ||
@@ -1202,7 +1321,7 @@ int mdx_batch_destroy(mdx_batch *b) {
   if (!b) return MDX_OK;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  void *ptrs[] = {b->geom,    b->off_d,    b->xin,      b->staging,  b->type_c,  b->rep_c,   b->xw,      b->x,
+  void *ptrs[] = {b->xq, b->geom,    b->off_d,    b->xin,      b->staging,  b->type_c,  b->rep_c,   b->xw,      b->x,
                   b->f,       b->g,        b->rep,      b->orig,     b->inv,     b->key,     b->key_s,   b->idx,
                   b->idx_s,   b->bin_count, b->bin_start, b->sort_tmp, b->scan_tmp, b->nbr,    b->numneigh, b->dflags,
                   b->rho,     b->fp,       b->eemb,     b->vadj,     b->partials, b->colsum,  b->ljtab,   b->frho_d,
@@ -1211,8 +1330,20 @@ int mdx_batch_destroy(mdx_batch *b) {
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->own_stream) {
+    (void)hipStreamSynchronize(b->own_stream);
+    (void)hipStreamDestroy(b->own_stream);
+  }
   delete b;
+  return MDX_OK;
+}
+
+int mdx_batch_set_stream(mdx_batch *b, int external, void *stream) {
+  int rc = check_b(b);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(b->stream);  // work already queued finishes where it was queued
+  if (e != hipSuccess) return bfail(e, "batch stream switch");
+  b->stream = external ? static_cast<hipStream_t>(stream) : b->own_stream;
   return MDX_OK;
 }
 
@@ -1311,10 +1442,12 @@ int mdx_batch_lj_forces(mdx_batch *b, double *f, double *energy) {
     const int G = persistent_grid(b, t);
     if (mixed)
       TPA_DISPATCH(t.tpa, hipLaunchKernelGGL((k_b_lj<true, TP>), dim3(G), dim3(t.block), 0, b->stream, b->n, nt1,
-                                             b->x, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->ljtab, b->f))
+                                             b->x, b->xq, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->ljtab,
+                                             b->f))
     else
       TPA_DISPATCH(t.tpa, hipLaunchKernelGGL((k_b_lj<false, TP>), dim3(G), dim3(t.block), 0, b->stream, b->n, nt1,
-                                             b->x, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->ljtab, b->f))
+                                             b->x, b->xq, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->ljtab,
+                                             b->f))
   };
   if ((rc = run_kind(b, MDX_BK_LJ, launch))) return rc;
   if (energy) {
@@ -1431,11 +1564,11 @@ int eam_forward(mdx_batch *b, bool with_force, int precision) {
     const int G = persistent_grid(b, t);
     if (mixed)
       TPA_DISPATCH(t.tpa, hipLaunchKernelGGL((k_b_eam_rho<true, TP>), dim3(G), dim3(t.block), sh1, b->stream, b->n,
-                                             b->x, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et, b->rhor_d,
+                                             b->x, b->xq, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et, b->rhor_d,
                                              b->rhor_f, b->frho_d, lds1 ? 1 : 0, b->rho, b->fp, b->eemb))
     else
       TPA_DISPATCH(t.tpa, hipLaunchKernelGGL((k_b_eam_rho<false, TP>), dim3(G), dim3(t.block), sh1, b->stream, b->n,
-                                             b->x, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et, b->rhor_d,
+                                             b->x, b->xq, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et, b->rhor_d,
                                              b->rhor_f, b->frho_d, lds1 ? 1 : 0, b->rho, b->fp, b->eemb))
   };
   if ((rc = run_kind(b, MDX_BK_EAM_RHO, rho_launch))) return rc;
@@ -1445,11 +1578,11 @@ int eam_forward(mdx_batch *b, bool with_force, int precision) {
     const int G = persistent_grid(b, t);
     if (mixed)
       TPA_DISPATCH(t.tpa, hipLaunchKernelGGL((k_b_eam_force<true, TP>), dim3(G), dim3(t.block), sh2, b->stream, b->n,
-                                             b->x, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et, b->rhor_d,
+                                             b->x, b->xq, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et, b->rhor_d,
                                              b->z2r_d, b->rhor_f, b->z2r_f, lds2 ? 1 : 0, b->fp, b->eemb, b->f))
     else
       TPA_DISPATCH(t.tpa, hipLaunchKernelGGL((k_b_eam_force<false, TP>), dim3(G), dim3(t.block), sh2, b->stream,
-                                             b->n, b->x, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et,
+                                             b->n, b->x, b->xq, b->rep, b->geom, b->nbr, b->numneigh, b->cap, b->et,
                                              b->rhor_d, b->z2r_d, b->rhor_f, b->z2r_f, lds2 ? 1 : 0, b->fp, b->eemb,
                                              b->f))
   };

@@ -73,7 +73,7 @@ class LJForceMatch:
 
     def __call__(self, eps_pairs, sigma_pairs):
         return ext().lj_forces(eps_pairs, sigma_pairs, self.cut, self.batch.h.value, self.ntypes, self.batch.n,
-                               self.device)
+                               self.device, self.torch.cuda.current_stream(self.device).cuda_stream)
 
 
 class EAMForceMatch:
@@ -95,7 +95,7 @@ class EAMForceMatch:
 
     def __call__(self, frho, rhor, z2r):
         return ext().eam_forces(frho, rhor, z2r, self.batch.h.value, self.batch.n, self.device, self.drho, self.dr,
-                                self.cut)
+                                self.cut, self.torch.cuda.current_stream(self.device).cuda_stream)
 
 
 __all__ = ["FP64", "MIXED", "LJForceMatch", "EAMForceMatch", "pair_index", "pair_table", "ext"]
