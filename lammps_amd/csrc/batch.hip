@@ -306,52 +306,75 @@ __global__ __launch_bounds__(1024) void k_b_lj(int n, int nt1, const double4 *__
     const int jn = numneigh[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     if constexpr (MIXED) {
+      // chunks of four consecutive entries per lane (one 16-B index load,
+      // rows start 16-B aligned), then four 16-B gathers in flight
       const int4 qi = xq[i];
       const double q0 = g.qstep[0], q1 = g.qstep[1], q2 = g.qstep[2];
-      for (int k = lane; k < jn; k += TPA) {
-        const int ent = row[k];
-        const int j = ent & MDX_IDX_MASK;
-        const int4 qj = xq[j];
-        const double dx = qsep(qi.x, qj.x, q0), dy = qsep(qi.y, qj.y, q1), dz = qsep(qi.z, qj.z, q2);
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        const int t = ti * nt1 + qj.w;
-        bool inside = rsq < s_qlo[t];
-        if (!inside && rsq < s_qhi[t]) {
-          double sx, sy, sz;
-          shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
-          const double4 xj = x[j];
-          const double ex = xi.x - (xj.x + sx * px), ey = xi.y - (xj.y + sy * py), ez = xi.z - (xj.z + sz * pz);
-          inside = ex * ex + ey * ey + ez * ez < s_tab[T_CUTSQ * kNT + t];
+      for (int k0 = 4 * lane; k0 < jn; k0 += 4 * TPA) {
+        const int4 e4 = *reinterpret_cast<const int4 *>(row + k0);
+        const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
+        int ent[4];
+        int4 qj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          ent[u] = k0 + u < jn ? ev[u] : i;  // the row's tail past jn is not written
+          qj[u] = xq[ent[u] & MDX_IDX_MASK];
         }
-        if (inside) {
-          const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
-          const float r6inv = r2inv * r2inv * r2inv;
-          const double fpair = (double)(r6inv * (s_f[t] * r6inv - s_f[kNT + t]) * r2inv);
-          fx += dx * fpair;
-          fy += dy * fpair;
-          fz += dz * fpair;
-          e += 0.5 * (double)(r6inv * (s_f[2 * kNT + t] * r6inv - s_f[3 * kNT + t]));
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (k0 + u >= jn) continue;
+          const double dx = qsep(qi.x, qj[u].x, q0), dy = qsep(qi.y, qj[u].y, q1), dz = qsep(qi.z, qj[u].z, q2);
+          const double rsq = dx * dx + dy * dy + dz * dz;
+          const int t = ti * nt1 + qj[u].w;
+          bool inside = rsq < s_qlo[t];
+          if (!inside && rsq < s_qhi[t]) {
+            double sx, sy, sz;
+            shift_of((unsigned)ent[u] >> MDX_IDX_BITS, sx, sy, sz);
+            const double4 xj = x[ent[u] & MDX_IDX_MASK];
+            const double ex = xi.x - (xj.x + sx * px), ey = xi.y - (xj.y + sy * py), ez = xi.z - (xj.z + sz * pz);
+            inside = ex * ex + ey * ey + ez * ez < s_tab[T_CUTSQ * kNT + t];
+          }
+          if (inside) {
+            const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
+            const float r6inv = r2inv * r2inv * r2inv;
+            const double fpair = (double)(r6inv * (s_f[t] * r6inv - s_f[kNT + t]) * r2inv);
+            fx += dx * fpair;
+            fy += dy * fpair;
+            fz += dz * fpair;
+            e += 0.5 * (double)(r6inv * (s_f[2 * kNT + t] * r6inv - s_f[3 * kNT + t]));
+          }
         }
       }
     } else {
-      for (int k = lane; k < jn; k += TPA) {
-        const int ent = row[k];
-        const int j = ent & MDX_IDX_MASK;
-        double sx, sy, sz;
-        shift_of((unsigned)ent >> MDX_IDX_BITS, sx, sy, sz);
-        const double4 xj = x[j];
-        const double dx = xi.x - (xj.x + sx * px), dy = xi.y - (xj.y + sy * py), dz = xi.z - (xj.z + sz * pz);
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        const int t = ti * nt1 + (int)xj.w;
-        if (rsq < s_tab[T_CUTSQ * kNT + t]) {
-          const double r2inv = 1.0 / rsq;
-          const double r6inv = r2inv * r2inv * r2inv;
-          const double forcelj = r6inv * (s_tab[T_LJ1 * kNT + t] * r6inv - s_tab[T_LJ2 * kNT + t]);
-          const double fpair = forcelj * r2inv;
-          fx += dx * fpair;
-          fy += dy * fpair;
-          fz += dz * fpair;
-          e += 0.5 * (r6inv * (s_tab[T_LJ3 * kNT + t] * r6inv - s_tab[T_LJ4 * kNT + t]));
+      for (int k0 = 4 * lane; k0 < jn; k0 += 4 * TPA) {
+        const int4 e4 = *reinterpret_cast<const int4 *>(row + k0);
+        const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
+        int ent[4];
+        double4 xj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          ent[u] = k0 + u < jn ? ev[u] : i;
+          xj[u] = x[ent[u] & MDX_IDX_MASK];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (k0 + u >= jn) continue;
+          double sx, sy, sz;
+          shift_of((unsigned)ent[u] >> MDX_IDX_BITS, sx, sy, sz);
+          const double dx = xi.x - (xj[u].x + sx * px), dy = xi.y - (xj[u].y + sy * py),
+                       dz = xi.z - (xj[u].z + sz * pz);
+          const double rsq = dx * dx + dy * dy + dz * dz;
+          const int t = ti * nt1 + (int)xj[u].w;
+          if (rsq < s_tab[T_CUTSQ * kNT + t]) {
+            const double r2inv = 1.0 / rsq;
+            const double r6inv = r2inv * r2inv * r2inv;
+            const double forcelj = r6inv * (s_tab[T_LJ1 * kNT + t] * r6inv - s_tab[T_LJ2 * kNT + t]);
+            const double fpair = forcelj * r2inv;
+            fx += dx * fpair;
+            fy += dy * fpair;
+            fz += dz * fpair;
+            e += 0.5 * (r6inv * (s_tab[T_LJ3 * kNT + t] * r6inv - s_tab[T_LJ4 * kNT + t]));
+          }
         }
       }
     }

@@ -367,3 +367,63 @@ def test_threaded_baseline_build_follows_the_serial_oracle(kind):
     scale = np.sqrt(np.mean(np.sum(fs * fs, axis=1)))
     assert np.max(np.linalg.norm(ft - fs, axis=1)) <= 1e-10 * scale
     assert np.max(np.abs(xt - xs)) <= 1e-10
+
+
+def _lehmer(seed, n):
+    """The minimal standard generator (Park and Miller 1988) by its
+    definition, x <- 16807 x mod (2^31 - 1), in exact integer arithmetic."""
+    out, x = [], seed
+    for _ in range(n):
+        x = (16807 * x) % 2147483647
+        out.append(x / 2147483647.0)
+    return out
+
+
+def _polar_pairs(u):
+    """Gaussian draws from uniforms u by the polar method, in the order
+    random_park.cpp:55-76 returns them: each accepted pair (v1, v2) yields
+    v2 * fac first, then v1 * fac."""
+    out, k = [], 0
+    while k + 1 < len(u):
+        v1, v2 = 2.0 * u[k] - 1.0, 2.0 * u[k + 1] - 1.0
+        k += 2
+        rsq = v1 * v1 + v2 * v2
+        if rsq >= 1.0 or rsq == 0.0:
+            continue
+        fac = np.sqrt(-2.0 * np.log(rsq) / rsq)
+        out += [v2 * fac, v1 * fac]
+    return out
+
+
+def test_gaussian_velocity_create_follows_the_published_generator():
+    """velocity create dist gaussian loop all (velocity.cpp:207-240) on two
+    atoms of equal mass: after mom yes, atom 1's velocity is parallel to
+    (g1 - g4, g2 - g5, g3 - g6), the first six draws of the polar method on
+    the Park-Miller sequence from the seed.  The expected draws come from the
+    generator's definition (exact integers) and the reference's return order;
+    swapping the pair's order gives a different direction, so the test
+    separates the two."""
+    from lammps_amd import mdx
+    g = _polar_pairs(_lehmer(1, 40))[:6]
+    want = np.array([g[0] - g[3], g[1] - g[4], g[2] - g[5]])
+    want /= np.linalg.norm(want)
+    swapped = []
+    for a in range(0, 6, 2):
+        swapped += [g[a + 1], g[a]]
+    alt = np.array([swapped[0] - swapped[3], swapped[1] - swapped[4], swapped[2] - swapped[5]])
+    alt /= np.linalg.norm(alt)
+    assert np.linalg.norm(want - alt) > 1e-3
+    x = np.array([[1.0, 1.0, 1.0], [3.0, 3.0, 3.0]])
+    t = np.ones(2, dtype=np.int32)
+    mass = np.array([0.0, 1.0])
+    v_host = mdx.velocity_create(x, t, mass, 1.0, 1, dist="gaussian", loop="all")
+    r = M.Ref(M.LJ)
+    r.box((0, 0, 0), (5, 5, 5))
+    r.add_atoms(x, t)
+    r.mass(1, 1.0)
+    r.velocity_create(1.0, 1, dist="gaussian", loop="all")
+    v_ref = r.state()[1]
+    for v in (v_host, v_ref):
+        d = v[0] / np.linalg.norm(v[0])
+        assert np.allclose(v[0], -v[1], rtol=0, atol=1e-14)
+        assert np.linalg.norm(d - want) <= 1e-12
