@@ -230,12 +230,13 @@ int mdx_create(int device, mdx_ctx **out) {
   if (hipMalloc(&c->dflags, 64) != hipSuccess || hipMalloc(&c->red_out, sizeof(double) * MDX_RED_SLOTS) != hipSuccess ||
       hipHostMalloc(&c->red_host, sizeof(double) * MDX_RED_SLOTS, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->trig_host, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
-      hipMalloc(&c->lj_dev, sizeof(LJTab)) != hipSuccess) {
+      hipMalloc(&c->trig_dev, 64) != hipSuccess || hipMalloc(&c->lj_dev, sizeof(LJTab)) != hipSuccess) {
     set_error("Insufficient memory on accelerator");
     return MDX_ERR_NOMEM;
   }
   memset(c->trig_host, 0, 64);
-  if (hipMemset(c->dflags, 0, 64) != hipSuccess) {
+  if (hipMemset(c->dflags, 0, 64) != hipSuccess || hipMemset(c->trig_dev, 0, 64) != hipSuccess ||
+      hipEventCreateWithFlags(&c->step_ev, hipEventDisableTiming) != hipSuccess) {
     set_error("device flags initialisation failed");
     return MDX_ERR_HIP;
   }
@@ -253,13 +254,14 @@ int mdx_destroy(mdx_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_atoms(c);
-  void *ptrs[] = {c->dflags, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->eam_pk, c->eam_pk_f, c->bin_count, c->bin_start,
+  void *ptrs[] = {c->dflags, c->trig_dev, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->eam_pk, c->eam_pk_f, c->bin_count, c->bin_start,
                   c->scan_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   halo_free(c);
   for (hipEvent_t &ev : c->stage_ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (c->step_ev) (void)hipEventDestroy(c->step_ev);
   if (c->red_host) (void)hipHostFree(c->red_host);
   if (c->trig_host) (void)hipHostFree(c->trig_host);
   for (auto &r : c->evpool) {
@@ -716,6 +718,26 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
   const bool fused = c->fuse;
   bool pending_initial = true;
   auto is_decide = [c](int ago) { return ago >= c->delay && ago % c->every == 0; };
+  // fix nve fused into the force kernel of this step (see above)
+  auto step_args = [&](bool is_last) {
+    IntegArgs ig{};
+    for (int t = 0; t < MDX_NT1; t++) ig.mass[t] = c->mass[t];
+    ig.dtf = dtf;
+    ig.dtv = dtv;
+    ig.triggersq = c->triggersq;
+    ig.next = is_last ? 0 : 1;
+    ig.check = (!is_last && c->check && is_decide(c->ago + 1)) ? 1 : 0;
+    ig.write_f = is_last ? 1 : 0;
+    ig.v = c->v;
+    ig.xn = c->xb;
+    ig.xqn = c->qpos ? c->xqb : nullptr;
+    ig.qs = c->qs;
+    ig.xhold = c->xhold;
+    ig.trig = c->trig_host;
+    ig.trig_dev = c->trig_dev;
+    ig.stamp = ig.check ? next_trigger_stamp(c) : 0;
+    return ig;
+  };
   for (int s = 0; s < nsteps; s++) {
     const long step = ++c->ntimestep;
     const bool is_last = s == nsteps - 1;
@@ -726,12 +748,29 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
     const int check_now = decide_step && c->check;
     if (pending_initial && (rc = nve_initial(c, dtv, dtf, check_now))) return rc;
     c->ago = ago_next;
-    bool rebuild = false;
+    bool rebuild = false, queued = false;
     if (decide_step) {
       if (!c->check) rebuild = true;
       else {
         int flag = 0;
-        if ((rc = neigh_check_displacement(c, &flag))) return rc;
+        if (fused && !pending_initial && !out) {
+          // Look-ahead decide: the trigger was computed by the previous
+          // step's fused kernel.  This step's kernels are queued before the
+          // host reads it, predicated on its device copy, so the device
+          // never idles while the host decides; when it fired they do
+          // nothing and the step is queued again after the rebuild.
+          const int prev = c->trig_stamp;
+          IntegArgs ig = step_args(is_last);
+          ig.skip = c->trig_dev;
+          ig.skip_stamp = prev;
+          if ((rc = pair_compute(c, 0, &ig))) return rc;
+          queued = true;
+          hipError_t e = hipEventSynchronize(c->step_ev);
+          if (e != hipSuccess) return hip_fail(e, "trigger wait");
+          flag = __atomic_load_n(c->trig_host, __ATOMIC_ACQUIRE) == prev ? 1 : 0;
+        } else if ((rc = neigh_check_displacement(c, &flag))) {
+          return rc;
+        }
         rebuild = flag != 0;
         if (rebuild && c->ago == std::max(c->every, c->delay)) c->ndanger++;
       }
@@ -741,22 +780,12 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
       c->ncalls++;
     }
     if (fused) {
-      IntegArgs ig{};
-      for (int t = 0; t < MDX_NT1; t++) ig.mass[t] = c->mass[t];
-      ig.dtf = dtf;
-      ig.dtv = dtv;
-      ig.triggersq = c->triggersq;
-      ig.next = is_last ? 0 : 1;
-      ig.check = (!is_last && c->check && is_decide(c->ago + 1)) ? 1 : 0;
-      ig.write_f = is_last ? 1 : 0;
-      ig.v = c->v;
-      ig.xn = c->xb;
-      ig.xqn = c->qpos ? c->xqb : nullptr;
-      ig.qs = c->qs;
-      ig.xhold = c->xhold;
-      ig.trig = c->trig_host;
-      ig.stamp = ig.check ? next_trigger_stamp(c) : 0;
-      if ((rc = pair_compute(c, out ? 1 : 0, &ig))) return rc;
+      if (!queued || rebuild) {
+        IntegArgs ig = step_args(is_last);
+        if ((rc = pair_compute(c, out ? 1 : 0, &ig))) return rc;
+      }
+      hipError_t e = hipEventRecord(c->step_ev, c->stream);
+      if (e != hipSuccess) return hip_fail(e, "step event");
       if (out) {
         c->eng_vdwl = c->red_host[0];
         for (int k = 0; k < 6; k++) c->virial[k] = c->red_host[1 + k];
@@ -766,7 +795,7 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
         for (int k = 0; k < 6; k++) row[2 + k] = c->virial[k];
         if (cb) cb(user, c->ntimestep, row);
       }
-      if (ig.next) {
+      if (!is_last) {
         std::swap(c->x, c->xb);
         if (c->qpos) std::swap(c->xq, c->xqb);
       }
@@ -870,6 +899,7 @@ int mdx_autotune(mdx_ctx *c, int reps) {
     ig.qs = c->qs;
     ig.xhold = c->xhold;
     ig.trig = c->trig_host;
+    ig.trig_dev = c->trig_dev;
     igp = &ig;
   }
   for (int tpa : {4, 8, 16})

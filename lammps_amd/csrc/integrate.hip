@@ -26,7 +26,8 @@ __global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double d
                                                     double4 *__restrict__ x, double4 *__restrict__ v,
                                                     const double4 *__restrict__ f,
                                                     const double4 *__restrict__ xhold, int check,
-                                                    double triggersq, int *__restrict__ trig, int stamp,
+                                                    double triggersq, int *__restrict__ trig,
+                                                    int *__restrict__ trig_dev, int stamp,
                                                     int4 *__restrict__ xq, QScale qs) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(256) void k_nve_initial(int n, MassTab mt, double d
     const double4 h = xhold[i];
     const double dx = p.x - h.x, dy = p.y - h.y, dz = p.z - h.z;
     const double rsq = dx * dx + dy * dy + dz * dz;
-    mark_trigger(rsq > triggersq, trig, stamp);
+    mark_trigger(rsq > triggersq, trig, trig_dev, stamp);
   }
 }
 
@@ -184,7 +185,7 @@ int nve_initial(mdx_ctx *c, double dtv, double dtf, int check) {
   const int stamp = check ? next_trigger_stamp(c) : 0;
   prof_begin(c, MDX_K_NVE_INITIAL);
   hipLaunchKernelGGL(k_nve_initial, dim3(G), dim3(B), 0, c->stream, c->n, mass_tab(c), dtv, dtf, c->x, c->v, c->f,
-                     c->xhold, check, c->triggersq, c->trig_host, stamp, c->qpos ? c->xq : nullptr, c->qs);
+                     c->xhold, check, c->triggersq, c->trig_host, c->trig_dev, stamp, c->qpos ? c->xq : nullptr, c->qs);
   prof_end(c);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDX_OK : hip_fail(e, "nve initial");

@@ -239,11 +239,27 @@ __device__ __forceinline__ double qdelta(int qi, int qj, double step) {
 
 // Rebuild trigger of the displacement check (Neighbor::check_distance,
 // src/neighbor.cpp:1322-1372): the first triggering lane of each wavefront
-// stores the check's stamp into the pinned host word, so a wavefront makes
-// at most one host write however many of its atoms moved.
-__device__ __forceinline__ void mark_trigger(bool moved, int *trig, int stamp) {
+// stores the check's stamp into the pinned host word (the host's decide
+// reads it) and into a device word (kernels queued before that decide read
+// it), so a wavefront makes at most one write of each however many of its
+// atoms moved.
+__device__ __forceinline__ void mark_trigger(bool moved, int *trig, int *trig_dev, int stamp) {
   const unsigned long long m = __ballot(moved);
-  if (moved && (int)(__ffsll((long long)m) - 1) == (int)(threadIdx.x & 63)) *trig = stamp;
+  if (moved && (int)(__ffsll((long long)m) - 1) == (int)(threadIdx.x & 63)) {
+    *trig = stamp;
+    *trig_dev = stamp;
+  }
+}
+
+// A step's kernels queued before the host has read the trigger they depend
+// on (mdx_run's look-ahead decide) do nothing when that trigger fired, i.e.
+// when the device word holds the check's stamp; the host then rebuilds the
+// list and queues the step again.  The answer is the same for every thread
+// of the launch: while the word holds `stamp` no thread runs a body that
+// could write it, and otherwise the only value a body can write is a later
+// check's stamp, which differs from `stamp` as well.
+__device__ __forceinline__ bool skip_step(const int *word, int stamp) {
+  return word && __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == stamp;
 }
 
 // Velocity-Verlet work fused into the force kernel (fix nve final integrate
@@ -259,8 +275,11 @@ struct IntegArgs {
   int4 *xqn;  // fixed-point copy of xn (mixed gathers), or null
   QScale qs;
   const double4 *xhold;
-  int *trig;   // pinned host word: set to stamp when an atom moved past the trigger
+  int *trig;      // pinned host word: set to stamp when an atom moved past the trigger
+  int *trig_dev;  // device copy of the same word
   int stamp;
+  const int *skip;  // look-ahead launch: do nothing when *skip == skip_stamp (or null)
+  int skip_stamp;
 };
 
 struct Launch {
@@ -352,7 +371,9 @@ struct mdx_ctx {
   // straight into pinned host memory: a check stores its own stamp, so the
   // word needs no clearing and no device-to-host copy before it is read.
   int *trig_host = nullptr;
+  int *trig_dev = nullptr;       // device copy of the trigger word
   int trig_stamp = 0;
+  hipEvent_t step_ev = nullptr;  // recorded after each fused step's kernels (look-ahead decide)
   // results of the last compute
   double eng_vdwl = 0.0, virial[6] = {0, 0, 0, 0, 0, 0};
   // launch shape and timing (event pairs recorded on `stream`, resolved

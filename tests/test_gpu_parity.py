@@ -174,6 +174,40 @@ def test_fused_integration_is_bit_identical_to_separate_kernels(lj_state, eam_st
         np.testing.assert_allclose(r1, r2, rtol=1e-13, atol=1e-13 * max(abs(q) for q in r2))
 
 
+def test_look_ahead_decide_relaunches_the_steps_it_skipped(eam_state):
+    """check yes, every 1, delay 2, no thermo output inside the run: from
+    the second step of a run on, a step's kernels are queued before the host
+    has read the displacement trigger they depend on.  A step whose trigger
+    fired does nothing on the device and is queued again after the rebuild,
+    so each of rho and fused force+integrate is launched once per step plus
+    once per such rebuild (all but the builds of the run's first and last
+    steps), and the run equals the unfused path bit for bit."""
+    from lammps_amd import mdx
+    r = eam_state
+    lo, hi = box_of(r, 20, 3.615)
+    out = {}
+    for fused in (True, False):
+        ctx = device_from_oracle(r, lo, hi, "eam", mdx.FP64, 1.0, 1, 2, True)
+        ctx.fused(fused)
+        ctx.setup()
+        ctx.profiling(True)
+        k0 = ctx.stats()["kernel_launches"]
+        ctx.run(60, 0.005, 1.0 / 1.0364269e-4, 0)
+        st = ctx.stats()
+        k = [a - b for a, b in zip(st["kernel_launches"], k0)]
+        out[fused] = dict(atoms=ctx.get_atoms(), builds=st["ncalls"], k=k)
+        ctx.close()
+    for a, b in zip(out[True]["atoms"], out[False]["atoms"]):
+        assert np.array_equal(a, b)
+    builds = out[True]["builds"]
+    assert builds == out[False]["builds"] >= 3
+    k = out[True]["k"]
+    relaunched = k[mdx.K_FUSED] - 60
+    assert builds - 2 <= relaunched <= builds and relaunched >= 1
+    assert k[mdx.K_EAM_RHO] == k[mdx.K_FUSED]
+    assert out[False]["k"][mdx.K_EAM_FORCE] == 60
+
+
 def test_lj_bench_run_matches_reference_log():
     """bench/in.lj on the device (FP64): thermo rows at steps 0 and 100 equal
     the log's printed digits, 1202833 neighbours, 5 builds."""
