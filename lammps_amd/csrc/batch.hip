@@ -877,6 +877,7 @@ struct mdx_batch {
   int pair = MDX_PAIR_NONE;
   double *ljtab = nullptr;  // T_N * kNT
   double lj_cutmax = 0.0;
+  std::vector<double> lj_tab;  // the table last uploaded (a repeated init uploads nothing)
   EamTab et{};
   double *frho_d = nullptr, *rhor_d = nullptr, *z2r_d = nullptr;
   float *rhor_f = nullptr, *z2r_f = nullptr;
@@ -1444,8 +1445,10 @@ int mdx_batch_lj_init(mdx_batch *b, const double *eps, const double *sigma, cons
       tab[T_B2 * kNT + t] = 144.0 * ep * std::pow(sg, 5.0);
       cutmax = std::max(cutmax, c);
     }
+  if (b->pair == MDX_PAIR_LJ && b->ljtab && tab == b->lj_tab) return MDX_OK;
   if (!b->ljtab && (rc = dalloc(&b->ljtab, T_N * kNT))) return rc;
   if ((rc = upload(b->ljtab, tab.data(), sizeof(double) * T_N * kNT))) return rc;
+  b->lj_tab = tab;
   b->pair = MDX_PAIR_LJ;
   b->lj_cutmax = cutmax;
   return MDX_OK;
