@@ -227,16 +227,19 @@ int mdx_create(int device, mdx_ctx **out) {
     delete c;
     return hip_fail(e, "stream");
   }
-  if (hipMalloc(&c->dflags, 64) != hipSuccess || hipMalloc(&c->red_out, sizeof(double) * MDX_RED_SLOTS) != hipSuccess ||
+  if (hipMalloc(&c->dflags_base, 128) != hipSuccess || hipMalloc(&c->red_out, sizeof(double) * MDX_RED_SLOTS) != hipSuccess ||
       hipHostMalloc(&c->red_host, sizeof(double) * MDX_RED_SLOTS, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&c->trig_host, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostMalloc(&c->bflags_host, 64, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&c->trig_dev, 64) != hipSuccess || hipMalloc(&c->lj_dev, sizeof(LJTab)) != hipSuccess) {
     set_error("Insufficient memory on accelerator");
     return MDX_ERR_NOMEM;
   }
   memset(c->trig_host, 0, 64);
-  if (hipMemset(c->dflags, 0, 64) != hipSuccess || hipMemset(c->trig_dev, 0, 64) != hipSuccess ||
-      hipEventCreateWithFlags(&c->step_ev, hipEventDisableTiming) != hipSuccess) {
+  c->dflags = c->dflags_base;
+  if (hipMemset(c->dflags_base, 0, 128) != hipSuccess || hipMemset(c->trig_dev, 0, 64) != hipSuccess ||
+      hipEventCreateWithFlags(&c->step_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->build_ev, hipEventDisableTiming) != hipSuccess) {
     set_error("device flags initialisation failed");
     return MDX_ERR_HIP;
   }
@@ -254,7 +257,7 @@ int mdx_destroy(mdx_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_atoms(c);
-  void *ptrs[] = {c->dflags, c->trig_dev, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->eam_pk, c->eam_pk_f, c->bin_count, c->bin_start,
+  void *ptrs[] = {c->dflags_base, c->trig_dev, c->red_out, c->lj_dev, c->eam_tab, c->eam_tab_f, c->eam_pk, c->eam_pk_f, c->bin_count, c->bin_start,
                   c->scan_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -262,6 +265,8 @@ int mdx_destroy(mdx_ctx *c) {
   for (hipEvent_t &ev : c->stage_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->step_ev) (void)hipEventDestroy(c->step_ev);
+  if (c->build_ev) (void)hipEventDestroy(c->build_ev);
+  if (c->bflags_host) (void)hipHostFree(c->bflags_host);
   if (c->red_host) (void)hipHostFree(c->red_host);
   if (c->trig_host) (void)hipHostFree(c->trig_host);
   for (auto &r : c->evpool) {
@@ -761,8 +766,8 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
           // nothing and the step is queued again after the rebuild.
           const int prev = c->trig_stamp;
           IntegArgs ig = step_args(is_last);
-          ig.skip = c->trig_dev;
-          ig.skip_stamp = prev;
+          ig.gate.trig = c->trig_dev;
+          ig.gate.stamp = prev;
           if ((rc = pair_compute(c, 0, &ig))) return rc;
           queued = true;
           hipError_t e = hipEventSynchronize(c->step_ev);
@@ -776,11 +781,31 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
       }
     }
     if (rebuild) {
-      if ((rc = neigh_build(c))) return rc;
+      if (fused && !pending_initial && !out) {
+        // Deferred build check: the step is queued behind the build, gated
+        // on the build's flags, before the host reads them; on a list
+        // overflow it does nothing, and the host grows the list, builds
+        // again and queues the step again.
+        if ((rc = neigh_build_launch(c))) return rc;
+        IntegArgs ig = step_args(is_last);
+        ig.gate.bflags = c->dflags;
+        ig.gate.cap = c->maxneigh;
+        if ((rc = pair_compute(c, 0, &ig))) return rc;
+        queued = true;
+        bool regrow = false;
+        if ((rc = neigh_build_finish(c, &regrow))) return rc;
+        if (regrow) {
+          if ((rc = neigh_build(c))) return rc;
+          queued = false;
+        }
+      } else {
+        if ((rc = neigh_build(c))) return rc;
+        queued = false;
+      }
       c->ncalls++;
     }
     if (fused) {
-      if (!queued || rebuild) {
+      if (!queued) {
         IntegArgs ig = step_args(is_last);
         if ((rc = pair_compute(c, out ? 1 : 0, &ig))) return rc;
       }

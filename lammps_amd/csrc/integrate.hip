@@ -87,20 +87,23 @@ __global__ __launch_bounds__(256) void k_ke(int n, MassTab mt, const double4 *__
   }
 }
 
-// fixed-order final reduction of per-block partials [nblocks][nslots],
-// nslots <= MDX_RED_SLOTS: one pass over the records (a thread reads whole
-// records b = t, t + 256, ...), then a fixed tree per slot
-__global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__ partials, int nblocks, int nslots,
-                                                     double *__restrict__ out) {
+// Fixed-order reduction of per-block records [nrec][nslots], nslots <=
+// MDX_RED_SLOTS: block r sums records [r*per, min((r+1)*per, nrec)) (a
+// thread reads whole records start + t, start + t + 256, ..., then a fixed
+// tree per slot) into record r of `out`.  One block with per = nrec is the
+// whole reduction; many blocks are its first level.
+__global__ __launch_bounds__(256) void k_reduce_records(const double *__restrict__ rec, int nrec, int nslots, int per,
+                                                       double *__restrict__ out) {
   __shared__ double s[MDX_RED_SLOTS][256];
   double acc[MDX_RED_SLOTS];
 #pragma unroll
   for (int k = 0; k < MDX_RED_SLOTS; k++) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += 256) {
-    const double *rec = partials + (size_t)b * nslots;
+  const long start = (long)blockIdx.x * per, end = min((long)nrec, start + per);
+  for (long b = start + threadIdx.x; b < end; b += 256) {
+    const double *r = rec + (size_t)b * nslots;
 #pragma unroll
     for (int k = 0; k < MDX_RED_SLOTS; k++)
-      if (k < nslots) acc[k] += rec[k];
+      if (k < nslots) acc[k] += r[k];
   }
 #pragma unroll
   for (int k = 0; k < MDX_RED_SLOTS; k++) s[k][threadIdx.x] = acc[k];
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(256) void k_final_reduce(const double *__restrict__
       for (int k = 0; k < MDX_RED_SLOTS; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
     __syncthreads();
   }
-  if ((int)threadIdx.x < nslots) out[threadIdx.x] = s[threadIdx.x][0];
+  if ((int)threadIdx.x < nslots) out[(size_t)blockIdx.x * nslots + threadIdx.x] = s[threadIdx.x][0];
 }
 
 // caller-order host positions -> device (sorted) order, type kept in w
@@ -208,8 +211,22 @@ int neigh_check_displacement(mdx_ctx *c, int *flag) {
   return MDX_OK;
 }
 
+// Sum of the per-block records into red_host.  Many records (64,000 for the
+// fused lj/cut kernel at 2.048 M atoms) are first reduced by up to 1024
+// blocks of at least 256 contiguous records each, into the space after the
+// records, then by one block: the partition depends only on nblocks, so the
+// sums are reproducible.
 int finish_reduction(mdx_ctx *c, int nblocks, int nslots) {
-  hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, nblocks, nslots, c->red_out);
+  const int per = std::max(256, (nblocks + 1023) / 1024);
+  const int nlev = (nblocks + per - 1) / per;
+  if (nblocks > 2 * per && (long)(nblocks + nlev) * nslots <= c->cap_partials) {
+    double *lev = c->partials + (size_t)nblocks * nslots;
+    hipLaunchKernelGGL(k_reduce_records, dim3(nlev), dim3(256), 0, c->stream, c->partials, nblocks, nslots, per, lev);
+    hipLaunchKernelGGL(k_reduce_records, dim3(1), dim3(256), 0, c->stream, lev, nlev, nslots, nlev, c->red_out);
+  } else {
+    hipLaunchKernelGGL(k_reduce_records, dim3(1), dim3(256), 0, c->stream, c->partials, nblocks, nslots, nblocks,
+                       c->red_out);
+  }
   hipError_t e = hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * nslots, hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) return hip_fail(e, "reduction copy");
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "reduction sync");

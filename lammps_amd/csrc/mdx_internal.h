@@ -251,15 +251,34 @@ __device__ __forceinline__ void mark_trigger(bool moved, int *trig, int *trig_de
   }
 }
 
-// A step's kernels queued before the host has read the trigger they depend
-// on (mdx_run's look-ahead decide) do nothing when that trigger fired, i.e.
-// when the device word holds the check's stamp; the host then rebuilds the
-// list and queues the step again.  The answer is the same for every thread
-// of the launch: while the word holds `stamp` no thread runs a body that
-// could write it, and otherwise the only value a body can write is a later
-// check's stamp, which differs from `stamp` as well.
-__device__ __forceinline__ bool skip_step(const int *word, int stamp) {
-  return word && __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == stamp;
+// Gate of a step's kernels queued before the host has checked what they
+// depend on (mdx_run's look-ahead): they do nothing when
+//   * the displacement trigger they were queued behind fired (the device
+//     word holds the check's stamp), or
+//   * the list build queued before them failed: its largest row exceeds
+//     the list capacity, an atom left a non-periodic box or a bin overflowed
+//     (the build's flags 0, 4 and 5).
+// The host then rebuilds (or reports the error) and queues the step again.
+// The answer is the same for every thread of the launch: no kernel of the
+// step writes the build flags, and while the trigger word holds `stamp` no
+// thread runs a body that could write it; otherwise the only value a body
+// can write is a later check's stamp, which differs from `stamp` as well.
+struct StepGate {
+  const int *trig;    // device trigger word, or null
+  int stamp;
+  const int *bflags;  // build flags, or null
+  int cap;            // list capacity the build ran with
+};
+
+__device__ __forceinline__ bool gate_closed(const StepGate &g) {
+  if (g.trig && __hip_atomic_load(g.trig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g.stamp) return true;
+  if (g.bflags) {
+    const int mx = __hip_atomic_load(g.bflags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int lost = __hip_atomic_load(g.bflags + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int bin = __hip_atomic_load(g.bflags + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mx > g.cap || lost || bin) return true;
+  }
+  return false;
 }
 
 // Velocity-Verlet work fused into the force kernel (fix nve final integrate
@@ -278,8 +297,7 @@ struct IntegArgs {
   int *trig;      // pinned host word: set to stamp when an atom moved past the trigger
   int *trig_dev;  // device copy of the same word
   int stamp;
-  const int *skip;  // look-ahead launch: do nothing when *skip == skip_stamp (or null)
-  int skip_stamp;
+  StepGate gate;  // look-ahead launch: do nothing when the gate is closed (all null: always open)
 };
 
 struct Launch {
@@ -372,6 +390,11 @@ struct mdx_ctx {
   // word needs no clearing and no device-to-host copy before it is read.
   int *trig_host = nullptr;
   int *trig_dev = nullptr;       // device copy of the trigger word
+  int *bflags_host = nullptr;    // pinned copy of the last build's flags
+  int *dflags_base = nullptr;    // two halves of 16 build flags (null: dflags alone, cleared per build)
+  bool flags_dirty = false;      // a build started and did not pass its flags check
+  bool bins_dirty = true;        // bin counters not known to be clear
+  hipEvent_t build_ev = nullptr; // recorded after that copy
   int trig_stamp = 0;
   hipEvent_t step_ev = nullptr;  // recorded after each fused step's kernels (look-ahead decide)
   // results of the last compute
@@ -436,6 +459,12 @@ int compute_noalloc(mdx_ctx *c, int n, const double *x, const int *type, int efl
 // neighbour build (neigh.hip)
 int neigh_setup_grid(mdx_ctx *c);
 int neigh_build(mdx_ctx *c);
+// neigh_build in two halves: the launches (sort, list, an asynchronous copy
+// of the build flags to pinned memory), and the check of those flags, which
+// waits only for that copy.  finish sets *regrow when the list capacity was
+// too small (the caller rebuilds with neigh_build, which regrows).
+int neigh_build_launch(mdx_ctx *c);
+int neigh_build_finish(mdx_ctx *c, bool *regrow);
 int alloc_list(mdx_ctx *c, int cap);
 int neigh_check_displacement(mdx_ctx *c, int *flag);
 

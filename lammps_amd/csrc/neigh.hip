@@ -212,9 +212,15 @@ __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict
 // entry of a bin takes the rank of its atom index among the bin's entries.
 // The result is the order a stable sort of (bin, current index) gives:
 // bins ascending, current index ascending within a bin.
+// Also clears, for the next build, the bin counters (the scan has consumed
+// them) and the other half of the double-buffered build flags (null: none),
+// so that a build issues no memsets.
 __global__ void k_bin_scatter(int n, const unsigned *__restrict__ key, const int *__restrict__ slot,
-                              const int *__restrict__ bin_start, int *__restrict__ tmp) {
+                              const int *__restrict__ bin_start, int *__restrict__ tmp, int *__restrict__ count,
+                              int ncount, int *__restrict__ next_flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = i; k < ncount; k += gridDim.x * blockDim.x) count[k] = 0;
+  if (next_flags && i < 16) next_flags[i] = 0;
   if (i >= n) return;
   tmp[bin_start[key[i]] + slot[i]] = i;
 }
@@ -523,7 +529,7 @@ int neigh_setup_grid(mdx_ctx *c) {
     c->scan_tmp_bytes = 0;
   }
   a.ntypes1 = c->ntypes + 1;
-
+  c->bins_dirty = true;  // a new grid starts from cleared counters
   return MDX_OK;
 }
 
@@ -551,7 +557,12 @@ int alloc_list(mdx_ctx *c, int cap) {
   return MDX_OK;
 }
 
-int neigh_build(mdx_ctx *c) {
+int launch_list(mdx_ctx *c, bool again);
+
+// The launches of a build: pbc wrap + bins, the stable counting sort, the
+// permutation and the list (capacity maxneigh), then the build flags copied
+// into pinned memory behind them.
+int neigh_build_launch(mdx_ctx *c) {
   NeighArgs &a = c->na;
   a.n = c->n;
   a.npad = c->npad;
@@ -564,11 +575,26 @@ int neigh_build(mdx_ctx *c) {
   const int B = 256;
   const int G = (c->n + B - 1) / B;
   hipError_t e;
-  if ((e = hipMemsetAsync(c->bin_count, 0, sizeof(int) * (c->nbins + 1), c->stream)) != hipSuccess)
+  // Build flags: with two halves, build k uses half k mod 2, which build k-1
+  // cleared (every reader of build k-1's flags precedes build k on the
+  // stream), and clears the other half for build k+1.  A build that did not
+  // finish leaves its half dirty: then both are cleared here.  Bin counters
+  // likewise: cleared by the previous build's scatter pass, or here.
+  int *next_flags = nullptr;
+  if (c->dflags_base) {
+    c->dflags = c->dflags_base + 16 * (c->build_serial & 1);
+    next_flags = c->dflags_base + 16 * ((c->build_serial + 1) & 1);
+    if (c->flags_dirty && (e = hipMemsetAsync(c->dflags_base, 0, 128, c->stream)) != hipSuccess)
+      return hip_fail(e, "memset flags");
+  } else if ((e = hipMemsetAsync(c->dflags, 0, 64, c->stream)) != hipSuccess) {
+    return hip_fail(e, "memset flags");
+  }
+  c->flags_dirty = true;  // until this build's flags check passes
+  if (c->bins_dirty && (e = hipMemsetAsync(c->bin_count, 0, sizeof(int) * (c->nbins + 1), c->stream)) != hipSuccess)
     return hip_fail(e, "memset bins");
-  if ((e = hipMemsetAsync(c->dflags, 0, 64, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
   hipLaunchKernelGGL(k_wrap_bin, dim3(G), dim3(B), 0, c->stream, a, c->x, c->image, c->bkey, c->bidx,
                      c->bin_count, c->dflags);
+  c->bins_dirty = true;  // until the scatter pass clears them
   // stable sort of (bin, current index): bin_start from the counts, then
   // scatter by arrival slot and rank within each bin (deterministic order)
   size_t need = 0;
@@ -587,7 +613,9 @@ int neigh_build(mdx_ctx *c) {
   }
   e = hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, need, c->bin_count, c->bin_start, c->nbins + 1, c->stream);
   if (e != hipSuccess) return hip_fail(e, "bin scan");
-  hipLaunchKernelGGL(k_bin_scatter, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bidx, c->bin_start, c->bidx_s);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bidx, c->bin_start, c->bidx_s,
+                     c->bin_count, c->nbins + 1, next_flags);
+  c->bins_dirty = false;
   hipLaunchKernelGGL(k_bin_rank, dim3(G), dim3(B), 0, c->stream, c->n, c->bkey, c->bin_start, c->bidx_s, c->bidx,
                      c->bkey_s, c->one + 1, c->dflags);
   hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx, c->x, c->v, c->tag, c->orig,
@@ -603,46 +631,82 @@ int neigh_build(mdx_ctx *c) {
     int rc = alloc_list(c, c->maxneigh);
     if (rc) return rc;
   }
-  // list build; regrow the per-atom capacity and rebuild on overflow
-  for (int attempt = 0; attempt < 4; attempt++) {
-    a.maxneigh = c->maxneigh;
+  return launch_list(c, false);
+}
+
+// the list pass of a build (the atoms are sorted), with its flags copied
+// into pinned memory behind it; `again`: a repeat after a regrow, whose
+// list flags are cleared first
+int launch_list(mdx_ctx *c, bool again) {
+  NeighArgs &a = c->na;
+  hipError_t e;
+  a.maxneigh = c->maxneigh;
+  if (again) {
     if ((e = hipMemsetAsync(c->dflags, 0, 3 * sizeof(int), c->stream)) != hipSuccess)
       return hip_fail(e, "memset flags");
     if ((e = hipMemsetAsync(c->dflags + 8, 0, 8, c->stream)) != hipSuccess) return hip_fail(e, "memset flags");
-    launch_build(c);
-    int flags[16];
-    if ((e = hipMemcpyAsync(flags, c->dflags, 64, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
-      return hip_fail(e, "read build flags");
-    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(e, "neighbour build");
-    if (flags[4]) {
-      set_error("Lost atoms: an atom left a non-periodic box");
-      return MDX_ERR_ARG;
-    }
-    const int mx = flags[0];
-    if (mx > c->one || flags[5]) {  // flags[5]: a bin holds more than one + 1 atoms (k_bin_rank)
-      set_error("Neighbor list overflow, boost neigh_modify one");
-      return MDX_ERR_OVERFLOW;
-    }
-    unsigned long long tot;
-    memcpy(&tot, &flags[8], sizeof(tot));
-    c->total_neighbors = (long)tot;
-    c->max_neighbors = mx;
-    if (mx <= c->maxneigh) break;
+  }
+  launch_build(c);
+  prof_end(c);
+  if ((e = hipMemcpyAsync(c->bflags_host, c->dflags, 64, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+    return hip_fail(e, "read build flags");
+  if ((e = hipEventRecord(c->build_ev, c->stream)) != hipSuccess) return hip_fail(e, "build event");
+  return MDX_OK;
+}
+
+// Waits for the flags of the last list pass and checks them: errors as the
+// reference reports them, else *regrow when the largest row exceeded the
+// capacity (the capacity is grown here, the caller lists again).
+int neigh_build_finish(mdx_ctx *c, bool *regrow) {
+  *regrow = false;
+  if (c->n == 0) return MDX_OK;
+  hipError_t e = hipEventSynchronize(c->build_ev);
+  if (e != hipSuccess) return hip_fail(e, "neighbour build");
+  int flags[16];
+  memcpy(flags, c->bflags_host, sizeof(flags));
+  if (flags[4]) {
+    set_error("Lost atoms: an atom left a non-periodic box");
+    return MDX_ERR_ARG;
+  }
+  const int mx = flags[0];
+  if (mx > c->one || flags[5]) {  // flags[5]: a bin holds more than one + 1 atoms (k_bin_rank)
+    set_error("Neighbor list overflow, boost neigh_modify one");
+    return MDX_ERR_OVERFLOW;
+  }
+  unsigned long long tot;
+  memcpy(&tot, &flags[8], sizeof(tot));
+  c->total_neighbors = (long)tot;
+  c->max_neighbors = mx;
+  if (mx > c->maxneigh) {
     // regrow: the accelerator library's max_nbors resize (lal_neighbor.cpp:582-595)
     int cap = ((mx + mx / 4 + 15) / 16) * 16;
     cap = std::min(cap, c->one);
     int rc = alloc_list(c, cap);
     if (rc) return rc;
+    *regrow = true;
+    return MDX_OK;
   }
-  if (c->max_neighbors > c->maxneigh) {
-    set_error("neighbour list capacity could not be grown to fit");
-    return MDX_ERR_OVERFLOW;
-  }
-  prof_end(c);
   c->built = true;
+  c->flags_dirty = false;
   c->build_serial++;
   c->ago = 0;
   return MDX_OK;
+}
+
+// list build; regrow the per-atom capacity and list again on overflow
+int neigh_build(mdx_ctx *c) {
+  int rc = neigh_build_launch(c);
+  for (int attempt = 0; !rc && attempt < 4; attempt++) {
+    bool regrow = false;
+    if ((rc = neigh_build_finish(c, &regrow)) || !regrow) break;
+    prof_begin(c, MDX_K_BUILD);
+    rc = launch_list(c, true);
+  }
+  if (!rc && !c->built) {
+    set_error("neighbour list capacity could not be grown to fit");
+    return MDX_ERR_OVERFLOW;
+  }
+  return rc;
 }
 
 }  // namespace mdx

@@ -188,6 +188,8 @@ int compute_noalloc(mdx_ctx *c, int n, const double *x, const int *type, int efl
   s.eam = c->eam;
   s.launch = c->launch;
   s.red_host = c->red_host;  // pinned host slots of the caller's context
+  s.bflags_host = c->bflags_host;
+  s.build_ev = c->build_ev;
   s.profile = false;
   s.fixed_storage = true;
   s.n = n;

@@ -210,7 +210,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
                                            const LJTab *__restrict__ tab, double4 *__restrict__ f,
                                            double *__restrict__ vatom, double *__restrict__ partials, IntegArgs ig) {
   if constexpr (INTEG) {
-    if (skip_step(ig.skip, ig.skip_stamp)) return;
+    if (gate_closed(ig.gate)) return;
   }
   __shared__ double s_cutsq[kNT2], s_lj1[kNT2], s_lj2[kNT2], s_lj3[kNT2], s_lj4[kNT2], s_off[kNT2];
   __shared__ double s_clo[Q ? kNT2 : 1], s_chi[Q ? kNT2 : 1];
@@ -375,9 +375,9 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
                                                 const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                                 const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                                 double *__restrict__ rho_out, double *__restrict__ fp_out,
-                                                double *__restrict__ eemb, const int *skip, int skip_stamp) {
+                                                double *__restrict__ eemb, StepGate gate) {
   static_assert(!TL || (MIXED && ONE), "LDS tables hold the one FP32 table of a single-type system");
-  if (skip_step(skip, skip_stamp)) return;
+  if (gate_closed(gate)) return;
   // type map in LDS: a kernel-argument array indexed per lane is a memory read
   __shared__ int s_t2r[kNT2];
   extern __shared__ float4 s_eam_tab[];  // TL: c3..c6 of knots 0..nr
@@ -469,7 +469,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
                                                   double *__restrict__ partials, IntegArgs ig) {
   static_assert(!TL || (MIXED && ONE), "LDS tables hold the one FP32 table of a single-type system");
   if constexpr (INTEG) {
-    if (skip_step(ig.skip, ig.skip_stamp)) return;
+    if (gate_closed(ig.gate)) return;
   }
   // type maps in LDS: kernel-argument arrays indexed per lane are memory reads
   __shared__ int s_t2r[kNT2], s_t2z[kNT2];
@@ -633,7 +633,7 @@ void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   prof_begin(c, MDX_K_EAM_RHO);
   hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>), dim3(G), dim3(B),
                      occupancy_lds(static_lds<k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>>(), B, TL ? kn * sizeof(float4) : 0, 0), c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
-                     c->nbr, c->numneigh, c->rho, c->fp, c->eemb, INTEG ? ig.skip : nullptr, ig.skip_stamp);
+                     c->nbr, c->numneigh, c->rho, c->fp, c->eemb, INTEG ? ig.gate : StepGate{});
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
   const size_t tl_bytes = TL ? 3 * kn * sizeof(float4) : 0;

@@ -232,6 +232,78 @@ def test_lost_atom_in_a_closed_box_is_reported():
     ctx.close()
 
 
+def _collapsing_droplet():
+    """864 atoms of an fcc droplet expanded by 12% with inward velocities
+    0.3 (x - centre), in a 30 sigma periodic box: the largest neighbour count
+    goes from 54 to about 120 within 150 steps (the oracle on the same
+    start), so a build inside the run needs more list capacity than the
+    setup build sized."""
+    a = (4.0 / 0.8442) ** (1.0 / 3.0) * 1.12
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*(np.arange(6),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = ((g[:, None] + basis[None]) * a).reshape(-1, 3)
+    x = x - x.mean(0) + 15.0
+    return x, -0.3 * (x - 15.0)
+
+
+def test_list_regrows_inside_a_fused_run():
+    """mdx_run's rebuilds queue the step behind the build before the host
+    reads the build's flags; when the list outgrew its capacity that step
+    does nothing and is queued again after the list is grown and rebuilt.
+    The fused run (this path) and the unfused run (builds checked before
+    the step) end bit for bit equal, with the same builds, and the capacity
+    grew during the run."""
+    from lammps_amd import mdx
+    x, v = _collapsing_droplet()
+    out = {}
+    for fused in (True, False):
+        ctx = mdx.Context(0)
+        ctx.set_box((0, 0, 0), (30, 30, 30), (1, 1, 1))
+        ctx.set_atoms(x, np.ones(len(x), dtype=np.int32), MASS, v=v, tags=np.arange(1, len(x) + 1))
+        ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+        ctx.precision(mdx.FP64)
+        ctx.neigh_modify(0.3, every=1, delay=0, check=True)
+        ctx.fused(fused)
+        ctx.setup()
+        cap0 = ctx.stats()["capacity"]
+        rows = ctx.run(200, 0.005, 1.0, 100)
+        st = ctx.stats()
+        out[fused] = dict(atoms=ctx.get_atoms(), rows=rows, st=st, cap0=cap0)
+        ctx.close()
+    for a, b in zip(out[True]["atoms"], out[False]["atoms"]):
+        assert np.array_equal(a, b)
+    st = out[True]["st"]
+    assert st["ncalls"] == out[False]["st"]["ncalls"] > 10
+    assert st["max_neighbors"] <= st["capacity"]
+    assert st["capacity"] > out[True]["cap0"]
+    assert len(out[True]["rows"]) == len(out[False]["rows"]) == 2
+    for (s1, r1), (s2, r2) in zip(out[True]["rows"], out[False]["rows"]):
+        assert s1 == s2
+        np.testing.assert_allclose(r1, r2, rtol=1e-13, atol=1e-13 * max(abs(q) for q in r2))
+
+
+def test_lost_atom_inside_a_fused_run_is_reported():
+    """boundary f, an atom flying out mid-run: the step queued behind the
+    build that finds it outside does nothing, and mdx_run returns the
+    reference's 'Lost atoms' error"""
+    from lammps_amd import mdx
+    x = random_gas(20, 6.0, 0.9, seed=3) + 2.0
+    v = np.zeros_like(x)
+    v[np.argmax(x[:, 1]), 1] = 20.0  # the topmost atom, 0.1 sigma per step towards y = 10
+    ctx = mdx.Context(0)
+    ctx.set_box((0, 0, 0), (10, 10, 10), (0, 0, 0))
+    ctx.set_atoms(x, np.ones(len(x), dtype=np.int32), MASS, v=v, tags=np.arange(1, len(x) + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    ctx.precision(mdx.FP64)
+    ctx.neigh_modify(0.3, every=1, delay=0, check=True)
+    ctx.fused(True)
+    ctx.setup()
+    with pytest.raises(mdx.MdxError) as e:
+        ctx.run(200, 0.005, 1.0, 0)
+    assert e.value.code == mdx.ERR_ARG and "Lost atoms" in str(e.value)
+    ctx.close()
+
+
 def test_periodic_box_below_twice_the_cutoff_is_rejected():
     from lammps_amd import mdx
     x = random_gas(5, 5.0, 1.0, seed=4)
