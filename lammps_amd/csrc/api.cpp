@@ -809,8 +809,13 @@ int mdx_run(mdx_ctx *c, int nsteps, double dt, double ftm2v, int thermo_every, m
         IntegArgs ig = step_args(is_last);
         if ((rc = pair_compute(c, out ? 1 : 0, &ig))) return rc;
       }
-      hipError_t e = hipEventRecord(c->step_ev, c->stream);
-      if (e != hipSuccess) return hip_fail(e, "step event");
+      // the next step's look-ahead decide waits for this step's kernels
+      // (which compute its trigger); an event costs the queue a few us, so
+      // it is recorded only then
+      if (!is_last && c->check && is_decide(c->ago + 1)) {
+        hipError_t e = hipEventRecord(c->step_ev, c->stream);
+        if (e != hipSuccess) return hip_fail(e, "step event");
+      }
       if (out) {
         c->eng_vdwl = c->red_host[0];
         for (int k = 0; k < 6; k++) c->virial[k] = c->red_host[1 + k];
