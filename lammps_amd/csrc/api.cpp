@@ -631,7 +631,8 @@ int mdx_compute(mdx_ctx *c, int eflag, int vflag, double *f, double *eng_vdwl, d
   int rc = check_ctx(c);
   if (rc) return rc;
   if ((rc = ensure_built(c))) return rc;
-  const int ev = (eflag || vflag) ? 1 : 0;
+  // the per-atom virial is tallied per pair only when it is asked for
+  const int ev = (eflag || vflag) ? (vatom ? 2 : 1) : 0;
   if ((rc = pair_compute(c, ev))) return rc;
   if (ev) {
     c->eng_vdwl = c->red_host[0];

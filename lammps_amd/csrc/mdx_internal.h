@@ -298,6 +298,11 @@ struct IntegArgs {
   int *trig_dev;  // device copy of the same word
   int stamp;
   StepGate gate;  // look-ahead launch: do nothing when the gate is closed (all null: always open)
+  // energy/virial launches (any pair kernel): 1 = per-atom virial wanted,
+  // tallied per pair (ev_tally); 0 = global virial only, from sum_i x_i f_i
+  // plus the image-shift term of the pairs that cross a periodic boundary
+  // (virial_fdotr_compute), about 6 FMAs per pair cheaper
+  int pervir;
 };
 
 struct Launch {

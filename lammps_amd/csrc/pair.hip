@@ -116,6 +116,38 @@ __device__ __forceinline__ void pair_delta(const double4 &xi, const double4 &xj,
   delz = xi.z - gz;
 }
 
+// Virial of one full-list entry (the pair's force on i is fpair * del).
+// pervir: the per-pair tally of ev_tally (src/pair.cpp:850-855), half of
+// del x f per entry, which also gives the per-atom virial.  Otherwise the
+// global virial is taken as virial_fdotr_compute does (src/pair.cpp:
+// 1402-1450): sum_i x_i f_i, formed once per atom in finish_atom, minus
+// half of s x f for every entry whose partner image is shifted by s (the
+// ghost atoms' share of the reference's sum over owned and ghost atoms):
+// sum_i sum_j del_ij f_ij / 2 with del_ij = x_i - x_j - s_ij and
+// f_ji = -f_ij.
+__device__ __forceinline__ void virial_entry(double (&w)[6], int pervir, int code, double px, double py, double pz,
+                                             double delx, double dely, double delz, double fpair) {
+  if (pervir) {
+    const double hf = 0.5 * fpair;
+    w[0] += delx * delx * hf;
+    w[1] += dely * dely * hf;
+    w[2] += delz * delz * hf;
+    w[3] += delx * dely * hf;
+    w[4] += delx * delz * hf;
+    w[5] += dely * delz * hf;
+  } else if (code != MDX_NOSHIFT) {
+    double sx, sy, sz;
+    image_shift(code, px, py, pz, sx, sy, sz);
+    const double h = -0.5 * fpair;
+    w[0] += sx * delx * h;
+    w[1] += sy * dely * h;
+    w[2] += sz * delz * h;
+    w[3] += sx * dely * h;
+    w[4] += sx * delz * h;
+    w[5] += sy * delz * h;
+  }
+}
+
 // fix nve for one atom right after its force is complete: final integrate
 // of this step, then (ig.next) the initial integrate of the next step into
 // ig.xn, with the rebuild trigger.  Same operations and order as
@@ -169,9 +201,17 @@ __device__ __forceinline__ void finish_atom(int i, bool owner, int npad, const d
   double mv2 = 0.0;
   if (owner) {
     if (!INTEG || EV || ig.write_f) f[i] = make_double4(fx, fy, fz, e);
-    if (EV)
+    if (EV && ig.pervir) {
 #pragma unroll
       for (int q = 0; q < 6; q++) vatom[(size_t)q * npad + i] = w[q];
+    } else if (EV) {  // the x_i f_i share of the global virial
+      w[0] += xi.x * fx;
+      w[1] += xi.y * fy;
+      w[2] += xi.z * fz;
+      w[3] += xi.x * fy;
+      w[4] += xi.x * fz;
+      w[5] += xi.y * fz;
+    }
     if (INTEG && LATE) {  // v and xhold read here, not held through the neighbour loop
       vpre = ig.v[i];
       if (ig.check) hpre = ig.xhold[i];
@@ -327,13 +367,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
           fz += delz * fpair;
           if (EV) {
             e += 0.5 * evdwl;
-            const double hf = 0.5 * fpair;
-            w[0] += delx * delx * hf;
-            w[1] += dely * dely * hf;
-            w[2] += delz * delz * hf;
-            w[3] += delx * dely * hf;
-            w[4] += delx * delz * hf;
-            w[5] += dely * delz * hf;
+            virial_entry(w, ig.pervir, (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz, fpair);
           }
         }
       }
@@ -555,13 +589,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
         fz += delz * fpair;
         if (EV) {
           e += 0.5 * phi;
-          const double hf = 0.5 * fpair;
-          w[0] += delx * delx * hf;
-          w[1] += dely * dely * hf;
-          w[2] += delz * delz * hf;
-          w[3] += delx * dely * hf;
-          w[4] += delx * delz * hf;
-          w[5] += dely * delz * hf;
+          virial_entry(w, ig.pervir, (unsigned)jc[u] >> MDX_IDX_BITS, px, py, pz, delx, dely, delz, fpair);
         }
       }
       }
@@ -701,8 +729,10 @@ int dispatch_prec(mdx_ctx *c, int ev, const IntegArgs &ig) {
 
 // Pair::compute on the device.  With `ig` the force
 // kernel also runs fix nve's final integrate of this step and, if ig->next,
-// the initial integrate of the next one (positions go to ig->xn).  With ev
-// the reduction's slots are {eng_vdwl, virial[6], sum m v^2 (fused only)}.
+// the initial integrate of the next one (positions go to ig->xn).  ev: 0
+// forces only, 1 also energy and global virial, 2 also the per-atom virial;
+// with ev the reduction's slots are {eng_vdwl, virial[6], sum m v^2 (fused
+// only)}.
 int pair_compute(mdx_ctx *c, int ev, const IntegArgs *ig) {
   if (c->n == 0) {  // no atoms: zero energy, virial and kinetic sums
     for (int k = 0; k < MDX_RED_SLOTS; k++) c->red_host[k] = 0.0;
@@ -714,12 +744,10 @@ int pair_compute(mdx_ctx *c, int ev, const IntegArgs *ig) {
   }
   int rc;
   const int nslots = 8;
-  if (ig) {
-    rc = dispatch_prec<true>(c, ev, *ig);
-  } else {
-    IntegArgs none{};
-    rc = dispatch_prec<false>(c, ev, none);
-  }
+  IntegArgs args = ig ? *ig : IntegArgs{};
+  args.pervir = ev >= 2 ? 1 : 0;
+  if (ig) rc = dispatch_prec<true>(c, ev, args);
+  else rc = dispatch_prec<false>(c, ev, args);
   if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
