@@ -91,6 +91,30 @@ def test_lj_forces_energy_virial(lj_state, prec):
     np.testing.assert_allclose(out["vatom"], va_ref, rtol=0, atol=100 * etol * np.abs(va_ref).max())
 
 
+@pytest.mark.parametrize("style", ["lj", "eam"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_global_virial_without_per_atom_virial_matches_the_oracle(lj_state, eam_state, style, prec):
+    """Without a per-atom virial output the global virial is formed as the
+    reference's virial_fdotr_compute does (sum_i x_i f_i plus the shifted
+    images' share), with one: as the per-entry tally.  Both equal the
+    oracle's virial (FP64 1e-11, mixed 1e-6 relative) and each other
+    (FP64 1e-12)."""
+    from lammps_amd import mdx
+    r = lj_state if style == "lj" else eam_state
+    lo, hi = box_of(r, 20, A_LJ if style == "lj" else 3.615)
+    ctx = device_from_oracle(r, lo, hi, style, precision_of(prec), 0.3 if style == "lj" else 1.0, 20, 0, False)
+    precision_of(prec, ctx)
+    tally = ctx.compute(1, 1, peratom=True)["virial"]
+    fdotr = ctx.compute(1, 1)["virial"]
+    _, wref = r.energy_virial()
+    tol = 1e-11 if prec == "fp64" else 1e-6
+    for w in (tally, fdotr):
+        np.testing.assert_allclose(w, wref, rtol=tol, atol=tol * np.abs(wref).max())
+    if prec == "fp64":
+        np.testing.assert_allclose(fdotr, tally, rtol=1e-12, atol=1e-12 * np.abs(tally).max())
+    ctx.close()
+
+
 @pytest.mark.parametrize("tpa", [1, 2, 4, 8, 16, 32])
 def test_lj_threads_per_atom_variants_agree(lj_state, tpa):
     """Every pair-kernel shape gives the oracle's forces and pair set."""
