@@ -101,12 +101,15 @@ def run_case(kind, inp, precision, steps, warmup):
     tp1 = time.perf_counter()
     sp = ctx.stats()
     ctx.profiling(False)
+    # the step kernel alone: `steps` launches back to back between two HIP
+    # events on the library's stream (no per-launch events, no thermo variant)
+    step_kernel_ms = ctx.time_step_kernel(max(steps, 10), dt, ftm2v)
     res = dict(n=n, seconds=secs, ms_per_step=1e3 * secs / steps, value=n * steps / secs / 1e6,
                builds=st["ncalls"] - st0["ncalls"], total_neighbors=st["total_neighbors"],
                max_neighbors=st["max_neighbors"], kernel_ms=sp["kernel_ms"], kernel_launches=sp["kernel_launches"],
                profiled_ms_per_step=1e3 * (tp1 - tp0) / steps, profiled_builds=sp["ncalls"] - st["ncalls"],
                kernel_sum_ms_per_step=sum(sp["kernel_ms"]) / steps,
-               end_row=rows[-1][1] if rows else None, ctx=ctx, launch=launch)
+               end_row=rows[-1][1] if rows else None, ctx=ctx, launch=launch, step_kernel_ms=step_kernel_ms)
     names = ("build", "lj", "eam_rho", "eam_force", "nve_initial", "nve_final", "fused")
     log("  kernels of the profiled repeat (ms/step, launches): " + ", ".join(
         "%s %.4f (%d)" % (nm, sp["kernel_ms"][k] / steps, sp["kernel_launches"][k])
@@ -593,7 +596,7 @@ def eam_replicas(reps=10, check_oracle=True, fd_knots=6):
 
 
 def kernel_row(name, ms, launches, bytes_per_launch):
-    """Roofline row of one kernel kind from the profiled repeat."""
+    """Roofline row of one kernel: `ms` over `launches` launches."""
     avg = ms / max(1, launches)
     ach = bytes_per_launch / (avg * 1e-3) / 1e9 if avg > 0 else None
     return dict(kernel=name, avg_launch_ms=avg, launches=launches, bytes_per_launch=bytes_per_launch,
@@ -673,11 +676,17 @@ def main():
     tpa = res["mixed"]["launch"]["tpa"]
     kname = "k_lj<true, false, %d, %s, true, %s>" % (tpa, "true" if fused else "false",
                                                       "true" if tpa in (4, 8, 16) else "false")
-    row = kernel_row(kname, km[kk], kl[kk], n * per_atom)
+    # duration of one launch: the step kernel timed back to back (HIP events
+    # around the whole sequence); the profiled repeat's per-launch events add
+    # their own queue time and include the thermo-step variant
+    row = kernel_row(kname, res["mixed"]["step_kernel_ms"], 1, n * per_atom)
     steps_per_build = K / max(1, res["mixed"]["profiled_builds"])
     step_bytes = n * (per_atom + (4.0 * n_h + 32.0) / steps_per_build)
     roofline = dict(bound="hbm", achieved=row["achieved"], peak=HBM_PEAK_GBS, unit="GB/s", frac=row["frac"],
                     traffic=profiled_traffic(kname), kernel=kname, avg_launch_ms=row["avg_launch_ms"],
+                    avg_launch_basis="step kernel launched %d times back to back between two HIP events "
+                                     "(mdx_time_step_kernel)" % max(K, 10),
+                    profiled_repeat_avg_launch_ms=km[kk] / max(1, kl[kk]),
                     bytes_per_launch=row["bytes_per_launch"], bytes_per_atom=per_atom,
                     bytes_per_atom_basis="4(n_h+1) + 16 + 24 + 200 + 16 (reference algorithm, half list)",
                     n_half_neighbors_per_atom=n_h, measured_copy_gbs=copy_gbs,

@@ -220,6 +220,13 @@ int mdx_get_launch(mdx_ctx *ctx, int *block, int *tpa);
    current configuration for threads per atom {4, 8, 16} x block {128, 256},
    `reps` launches each; keeps the fastest and recomputes the forces with it. */
 int mdx_autotune(mdx_ctx *ctx, int reps);
+/* Timing probe (no counterpart in the reference): the force kernel of a
+   non-thermo step of mdx_run (with fix nve fused: force + final integrate +
+   next initial integrate, the same reads and writes), launched `reps` times
+   back to back between two HIP events on the context's stream, into spare
+   buffers (velocities and positions are left as they were); *ms is the
+   mean duration of one launch.  The list must be built (mdx_setup). */
+int mdx_time_step_kernel(mdx_ctx *ctx, int reps, double dt, double ftm2v, double *ms);
 /* Position format of the mixed-precision gathers.
    MDX_POS_Q32: each neighbour is read as 32-bit fixed point per dimension
    (resolution prd * 2^-32 periodic, prd * 2^-30 otherwise) and its

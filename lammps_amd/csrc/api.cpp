@@ -891,6 +891,65 @@ int mdx_get_launch(mdx_ctx *c, int *block, int *tpa) {
 // Times the pair kernels (no energy/virial) on the current list for each
 // (threads per atom, block) candidate, keeps the fastest, then recomputes
 // the forces with it so f is that of the chosen shape.
+int mdx_time_step_kernel(mdx_ctx *c, int reps, double dt, double ftm2v, double *ms) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!ms || reps < 1 || !(dt > 0.0)) {
+    set_error("mdx_time_step_kernel: reps >= 1, dt > 0 and an output pointer required");
+    return MDX_ERR_ARG;
+  }
+  if (!c->built || c->n == 0) {
+    set_error("mdx_time_step_kernel: no neighbour list (run mdx_setup first)");
+    return MDX_ERR_ARG;
+  }
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return hip_fail(e, "timing events");
+  if ((e = hipEventCreate(&e1)) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_fail(e, "timing events");
+  }
+  const bool prof = c->profile;
+  c->profile = false;
+  IntegArgs ig{};
+  const IntegArgs *igp = nullptr;
+  if (c->fuse) {
+    // the step kernel's reads and writes, on scratch copies: velocities in
+    // the permute's spare buffer (free between builds), next positions into
+    // the spare position buffers (free until the next step writes them)
+    e = hipMemcpyAsync(c->vb, c->v, sizeof(double4) * c->n, hipMemcpyDeviceToDevice, c->stream);
+    for (int t = 0; t < MDX_NT1; t++) ig.mass[t] = c->mass[t] > 0.0 ? c->mass[t] : 1.0;
+    ig.dtf = 0.5 * dt * ftm2v;
+    ig.dtv = dt;
+    ig.triggersq = c->triggersq;
+    ig.next = 1;
+    ig.check = 0;
+    ig.write_f = 0;
+    ig.v = c->vb;
+    ig.xn = c->xb;
+    ig.xqn = c->qpos ? c->xqb : nullptr;
+    ig.qs = c->qs;
+    ig.xhold = c->xhold;
+    ig.trig = c->trig_host;
+    ig.trig_dev = c->trig_dev;
+    igp = &ig;
+  }
+  float t = 0.0f;
+  if (e == hipSuccess) rc = pair_compute(c, 0, igp);  // warm-up
+  if (!rc && e == hipSuccess) e = hipEventRecord(e0, c->stream);
+  for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = pair_compute(c, 0, igp);
+  if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+  if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->profile = prof;
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "step kernel timing");
+  *ms = (double)t / reps;
+  return MDX_OK;
+}
+
 int mdx_autotune(mdx_ctx *c, int reps) {
   int rc = check_ctx(c);
   if (rc) return rc;

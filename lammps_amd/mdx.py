@@ -100,6 +100,7 @@ def lib():
             "mdx_set_position_format": [_p, _i],
             "mdx_get_launch": [_p, _ip, _ip],
             "mdx_autotune": [_p, _i],
+            "mdx_time_step_kernel": [_p, _i, _d, _d, _p],
             "mdx_halo_sendlist": [_p, _i, _dp, _dp, _dp, _ip],
             "mdx_halo_sendlists": [_p, _i, _dp, _dp, _dp, _ip],
             "mdx_halo_pack": [_p, _i, ctypes.c_void_p],
@@ -283,6 +284,13 @@ class Context:
 
     def fused(self, on=True):
         self._ck(self._L.mdx_set_fused(self.h, 1 if on else 0))
+
+    def time_step_kernel(self, reps, dt, ftm2v=1.0):
+        """Mean duration (ms) of the step's force kernel, launched `reps` times
+        back to back between two HIP events (mdx_time_step_kernel)."""
+        ms = ctypes.c_double(0.0)
+        self._ck(self._L.mdx_time_step_kernel(self.h, reps, dt, ftm2v, ctypes.byref(ms)))
+        return ms.value
 
     def copy_bandwidth(self, nbytes=1 << 30, iters=20):
         g = ctypes.c_double(0.0)

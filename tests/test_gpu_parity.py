@@ -232,6 +232,29 @@ def test_look_ahead_decide_relaunches_the_steps_it_skipped(eam_state):
     assert out[False]["k"][mdx.K_EAM_FORCE] == 60
 
 
+@pytest.mark.parametrize("style", ["lj", "eam"])
+def test_step_kernel_timing_leaves_the_state_unchanged(lj_state, eam_state, style):
+    """mdx_time_step_kernel launches the step kernel into spare buffers: a
+    run after it equals, bit for bit, the same run without it."""
+    from lammps_amd import mdx
+    r = lj_state if style == "lj" else eam_state
+    lo, hi = box_of(r, 20, A_LJ if style == "lj" else 3.615)
+    ftm2v = 1.0 if style == "lj" else 1.0 / 1.0364269e-4
+    out = []
+    for probe in (False, True):
+        ctx = device_from_oracle(r, lo, hi, style, mdx.MIXED, 0.3 if style == "lj" else 1.0, 5, 0, True)
+        ctx.setup()
+        ctx.run(7, 0.005, ftm2v, 0)
+        if probe:
+            ms = ctx.time_step_kernel(5, 0.005, ftm2v)
+            assert 0.0 < ms < 100.0
+        ctx.run(9, 0.005, ftm2v, 0)
+        out.append(ctx.get_atoms())
+        ctx.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_lj_bench_run_matches_reference_log():
     """bench/in.lj on the device (FP64): thermo rows at steps 0 and 100 equal
     the log's printed digits, 1202833 neighbours, 5 builds."""
