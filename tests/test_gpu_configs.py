@@ -7,6 +7,9 @@ forces and gradients) on identical positions.
   C2  eam, fcc 40^3 = 256,000 atoms, a = 3.615 A, the synthetic funcfl table
       (seed 4951, 500 points, cutoff 4.95), skin 1.0, Gaussian 1600 K, after
       100 device NVE steps (dt 5 fs)
+  C1  lj/cut, fcc 20^3 = 32,000 atoms, Gaussian T* = 1.44 (seed 87287, loop
+      geom), every 20 delay 0 check yes, 100 NVE steps: the whole run on
+      the device against the oracle's run, thermo rows at steps 0, 50, 100
   C4  KA 80:20 lj/cut mixture, 64 snapshots x 8,000 atoms (T* = 0.8 device
       MD), force-matching loss against perturbed parameters
   C5  512 non-periodic fcc clusters of 200-5000 atoms (seed 7), 64-knot
@@ -20,6 +23,10 @@ Bars (stated here):
     max_rel = max_i |f_i - f_i^ref| / RMS_j |f_j^ref|;
   * energy and virial: relative <= 1e-11 (FP64) / 1e-6 (mixed); EAM rho and
     fp: relative <= 1e-11 (FP64) / 1e-6 (mixed);
+  * C1 run: the same builds (and dangerous builds) as the oracle's run;
+    thermo rows equal to the oracle's printed digits (FP64); for mixed, the
+    reference's regression rule, mean-square error of every thermo column
+    over the rows < 1e-6 (regress/benchmark.py:105-124);
   * gradients: against the FP64 oracle gradient <= 1e-9 (FP64) / 1e-6
     (mixed) relative, and against central differences (h = 1e-6 relative)
     of the device FP64 loss <= 1e-6 relative.
@@ -335,3 +342,46 @@ def test_c5_knot_gradient_matches_central_differences(c5):
             assert err <= 1e-6, (which, int(k), g[which][k], fd)
     b.close()
     print("C5 knot gradient vs central differences: worst %.3e" % worst)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_c1_lj_32k_gaussian_run_matches_the_oracle(prec):
+    from lammps_amd import mdx
+    r = M.Ref(M.LJ)
+    r.lattice_box(M.FCC, 0.8442, (0, 0, 0), (20, 20, 20))
+    r.mass(1, 1.0)
+    r.velocity_create(1.44, 87287, dist="gaussian", loop="geom")
+    r.pair_lj(2.5)
+    r.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    r.neighbor(0.3, every=20, delay=0, check=True)
+    r.timestep(0.005)
+    x0, v0, _, t0, _ = r.state()
+    r.thermo(50)
+    r.setup()
+    ref_rows = r.run(100)
+    L = 20 * (4.0 / 0.8442) ** (1.0 / 3.0)
+    ctx = mdx.Context(0)
+    ctx.set_box((0, 0, 0), (L, L, L), (1, 1, 1))
+    ctx.set_atoms(x0, t0, np.array([0.0, 1.0]), v=v0, tags=np.arange(1, len(t0) + 1))
+    ctx.pair_lj_from_coeffs(1, {(1, 1): (1.0, 1.0, 2.5)})
+    ctx.neigh_modify(0.3, every=20, delay=0, check=True)
+    ctx.precision(mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    rows = ctx.setup() + ctx.run(100, 0.005, 1.0, 50)
+    st = ctx.stats()
+    ctx.close()
+    n = len(t0)
+    got = {int(step): mdx.thermo_columns(raw, n, (L, L, L), "lj") for step, raw in rows}
+    ref = {int(q[0]): q[1:] for q in ref_rows}
+    common = sorted(set(got) & set(ref))
+    assert 50 in common and 100 in common
+    rs = r.neigh_stats()
+    assert st["ncalls"] == rs["builds"] > 0 and st["ndanger"] == rs["dangerous"]
+    if prec == "fp64":
+        for step in common:
+            for a, b in zip(got[step], ref[step]):
+                assert ("%12.8g" % a).strip() == ("%12.8g" % b).strip(), (step, a, b)
+    else:
+        g = np.array([got[k] for k in common])
+        q = np.array([ref[k] for k in common])
+        mse = np.mean((g - q) ** 2, axis=0)
+        assert np.all(mse < 1e-6), mse
