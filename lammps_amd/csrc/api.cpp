@@ -612,7 +612,7 @@ int mdx_neigh_pairs(mdx_ctx *c, int *pairs, long cap, long *npairs) {
   long cnt = 0;
   for (int i = 0; i < n; i++)
     for (int k = 0; k < num[i]; k++) {
-      const size_t q = (size_t)i * c->maxneigh + k;
+      const size_t q = list_index(i, k, c->npad);
       const int j = nb[q] & MDX_IDX_MASK;
       if (tag[i] < tag[j]) {
         if (pairs && cnt < cap) {

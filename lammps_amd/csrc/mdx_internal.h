@@ -8,9 +8,12 @@
 //   f      double4[npad]  fx, fy, fz, w = per-atom energy of the last compute
 //   tag    int[npad]      atom ID;  orig int[npad]: caller index of each atom
 //   image  int[npad]      periodic images, 10 bits per dimension (IMGMAX 512)
-//   nbr    int[npad][maxneigh]  full list, one row per atom (row major, so
-//                                the lanes of an atom read consecutive
-//                                entries); bits 0-26 hold the neighbour
+//   nbr    int4[maxneigh/4][npad]  full list in chunks of four entries,
+//                                chunk-major: chunk c of atom i at c * npad +
+//                                i, so one chunk of consecutive atoms is
+//                                contiguous (a wavefront's index loads are
+//                                whole lines, none kept for a later trip);
+//                                bits 0-26 of an entry hold the neighbour
 //                                index, bits 27-31 a periodic image code
 //                                (sx+1) + 3(sy+1) + 9(sz+1)
 //   vatom  double[6][npad] per-atom virial (SoA)
@@ -283,6 +286,11 @@ __device__ __forceinline__ bool gate_closed(const StepGate &g) {
 
 // Velocity-Verlet work fused into the force kernel (fix nve final integrate
 // of this step, then initial integrate of the next, src/fix_nve.cpp:65-142)
+// entry k of atom i in the chunk-major list (int offset)
+__host__ __device__ __forceinline__ size_t list_index(int i, int k, int npad) {
+  return (((size_t)(k >> 2) * npad + i) << 2) + (k & 3);
+}
+
 struct IntegArgs {
   double mass[MDX_NT1];
   double dtf, dtv, triggersq;

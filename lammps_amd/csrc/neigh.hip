@@ -320,7 +320,8 @@ __global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) v
   const int nbx = a.nbin[0], nby = a.nbin[1], nbz = a.nbin[2];
   int ix, iy, iz;
   bin_coords(a, (int)b, ix, iy, iz);
-  int *row = nbr + (size_t)i * a.maxneigh;  // maxneigh is a multiple of 4: rows are 16-B aligned
+  // this atom's column of the chunk-major list: chunk c at col[c * npad]
+  int4 *col = reinterpret_cast<int4 *>(nbr) + i;
   int cnt = 0;
   int q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // the last four accepted entries, q3 the newest
   const int t11 = a.ntypes1 + 1;       // (1, 1) in the type tables
@@ -431,19 +432,21 @@ __global__ __launch_bounds__(256, Q ? MDX_BUILD_Q_WAVES : MDX_BUILD_MIN_WAVES) v
               q2 = q3;
               q3 = j | (code << MDX_IDX_BITS);
               cnt++;
-              if ((cnt & 3) == 0 && cnt <= a.maxneigh) *reinterpret_cast<int4 *>(row + cnt - 4) = make_int4(q0, q1, q2, q3);
+              if ((cnt & 3) == 0 && cnt <= a.maxneigh) col[(size_t)((cnt - 4) >> 2) * a.npad] = make_int4(q0, q1, q2, q3);
             }
           }
         }
       }
     }
   }
-  // entries after the last full group of four: the newest `tail` of the register
+  // entries after the last full group of four: the newest `tail` of the
+  // register, into the first components of chunk t0/4
   const int tail = cnt & 3, t0 = cnt - tail;
   const int r0 = tail == 1 ? q3 : tail == 2 ? q2 : q1, r1 = tail == 2 ? q3 : q2;
-  if (tail > 0 && t0 < a.maxneigh) row[t0] = r0;
-  if (tail > 1 && t0 + 1 < a.maxneigh) row[t0 + 1] = r1;
-  if (tail > 2 && t0 + 2 < a.maxneigh) row[t0 + 2] = q3;
+  int *last = reinterpret_cast<int *>(col + (size_t)(t0 >> 2) * a.npad);
+  if (tail > 0 && t0 < a.maxneigh) last[0] = r0;
+  if (tail > 1 && t0 + 1 < a.maxneigh) last[1] = r1;
+  if (tail > 2 && t0 + 2 < a.maxneigh) last[2] = q3;
   numneigh[i] = cnt;
   atomicMax(&dflags[0], cnt);
   atomicAdd(nsum, (unsigned long long)cnt);
@@ -535,7 +538,7 @@ int neigh_setup_grid(mdx_ctx *c) {
 
 // (re)allocates the list with `cap` entries per atom
 int alloc_list(mdx_ctx *c, int cap) {
-  // a multiple of 4 entries: rows start 16-B aligned (k_build_lane's stores)
+  // a multiple of 4 entries: whole chunks of four (k_build_lane's stores)
   cap = (std::max(cap, 16) + 3) / 4 * 4;
   if (c->fixed_storage) {  // the list lives in storage this context does not own
     if ((long)cap * c->npad > c->nbr_cap_elems) {

@@ -297,7 +297,7 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
     const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
     const int row = static_cast<int>(xi.w) * nt1;
     const int jn = numneigh[i];
-    const int *nb = nbr + (size_t)i * cap;
+    const int4 *nb4 = reinterpret_cast<const int4 *>(nbr) + i;  // chunk c at nb4[c * npad]
     // UNROLL entries per lane per trip: all index loads, then all gathers,
     // are issued before the first use, so a lane has UNROLL gathers in flight
     constexpr bool CHUNK = kChunkLJ;
@@ -309,13 +309,14 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
       double4 xj[Q ? 1 : U];
       int4 qj[Q ? U : 1];
       if constexpr (CHUNK) {
-        const int4 e4 = *reinterpret_cast<const int4 *>(nb + k0);
+        const int4 e4 = nb4[(size_t)(k0 >> 2) * npad];
         const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
         for (int u = 0; u < U; u++) jc[u] = k0 + u < jn ? ev[u] : self;
       } else {
 #pragma unroll
-        for (int u = 0; u < U; u++) jc[u] = (k0 + u * KS < jn) ? list_entry(nb + k0 + u * KS) : self;
+        for (int u = 0; u < U; u++)
+          jc[u] = (k0 + u * KS < jn) ? list_entry(nbr + list_index(i, k0 + u * KS, npad)) : self;
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -431,7 +432,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
     const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
     ti = static_cast<int>(xi.w);
     const int jn = numneigh[i];
-    const int *nb = nbr + (size_t)i * cap;
+    const int4 *nb4 = reinterpret_cast<const int4 *>(nbr) + i;  // chunk c at nb4[c * npad]
     constexpr bool CHUNK = kChunkEAM;
     constexpr int U = CHUNK ? 4 : (Q ? kUnrollQ : kUnroll);
     constexpr int KS = CHUNK ? 1 : TPA;  // entry stride within a trip
@@ -441,13 +442,14 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
       double4 xj[Q ? 1 : U];
       int4 qj[Q ? U : 1];
       if constexpr (CHUNK) {
-        const int4 e4 = *reinterpret_cast<const int4 *>(nb + k0);
+        const int4 e4 = nb4[(size_t)(k0 >> 2) * npad];
         const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
         for (int u = 0; u < U; u++) jc[u] = k0 + u < jn ? ev[u] : self;
       } else {
 #pragma unroll
-        for (int u = 0; u < U; u++) jc[u] = (k0 + u * KS < jn) ? list_entry(nb + k0 + u * KS) : self;
+        for (int u = 0; u < U; u++)
+          jc[u] = (k0 + u * KS < jn) ? list_entry(nbr + list_index(i, k0 + u * KS, npad)) : self;
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -540,7 +542,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
     const int ti = static_cast<int>(xi.w);
     const double fpi = fp[i];
     const int jn = numneigh[i];
-    const int *nb = nbr + (size_t)i * cap;
+    const int4 *nb4 = reinterpret_cast<const int4 *>(nbr) + i;  // chunk c at nb4[c * npad]
     constexpr bool CHUNK = kChunkEAM;
     constexpr int U = CHUNK ? 4 : (Q ? kUnrollQ : kUnroll);
     constexpr int KS = CHUNK ? 1 : TPA;  // entry stride within a trip
@@ -551,13 +553,14 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
       int4 qjv[Q ? U : 1];
       double fpjv[U];
       if constexpr (CHUNK) {
-        const int4 e4 = *reinterpret_cast<const int4 *>(nb + k0);
+        const int4 e4 = nb4[(size_t)(k0 >> 2) * npad];
         const int ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
         for (int u = 0; u < U; u++) jc[u] = k0 + u < jn ? ev[u] : self;
       } else {
 #pragma unroll
-        for (int u = 0; u < U; u++) jc[u] = (k0 + u * KS < jn) ? list_entry(nb + k0 + u * KS) : self;
+        for (int u = 0; u < U; u++)
+          jc[u] = (k0 + u * KS < jn) ? list_entry(nbr + list_index(i, k0 + u * KS, npad)) : self;
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
