@@ -125,6 +125,56 @@ def test_ragged_random_systems_match_the_oracle(n, prec):
     ctx.close()
 
 
+def oracle_lj_pairs_forces(lo, hi, periodic, x):
+    ref = M.Ref(M.LJ)
+    ref.box(lo, hi, periodic)
+    ref.add_atoms(x, np.ones(len(x), dtype=np.int32))
+    ref.mass(1, 1.0)
+    ref.pair_lj(2.5)
+    ref.pair_coeff(1, 1, 1.0, 1.0, 2.5)
+    ref.neighbor(0.3)
+    ref.setup()
+    return {(min(a, b), max(a, b)) for a, b in ref.list_pairs()}, ref.state()[2]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", ["four_bins", "thin_slab"])
+def test_small_bin_grids_match_the_oracle(prec, shape):
+    """Bin grids at the edges of the list build's layouts.  four_bins: a
+    periodic box of 4 bins a side (just above twice the neighbour cutoff),
+    where a stencil row of 5 bins wraps onto bins it already holds, the
+    same atoms under two images.  thin_slab: a closed dimension thinner
+    than half the cutoff (one bin, so a stencil of 7 planes), which takes
+    the per-row build."""
+    from lammps_amd import mdx
+    if shape == "four_bins":
+        L = 5.8
+        x = random_gas(110, L, 0.9, seed=11)
+        lo, hi, per = (0.0, 0.0, 0.0), (L, L, L), (1, 1, 1)
+    else:
+        rng = np.random.default_rng(12)
+        pts = np.zeros((0, 3))
+        while len(pts) < 90:
+            p = rng.uniform((0.0, 0.0, 0.05), (10.0, 10.0, 0.95))
+            d = pts - p
+            d[:, :2] = (d[:, :2] + 5.0) % 10.0 - 5.0
+            if len(pts) == 0 or np.min(np.einsum("ij,ij->i", d, d)) >= 0.81:
+                pts = np.vstack([pts, p])
+        x = pts
+        lo, hi, per = (0.0, 0.0, 0.0), (10.0, 10.0, 1.0), (1, 1, 0)
+    want, fref = oracle_lj_pairs_forces(lo, hi, per, x)
+    ctx = lj_context(lo, hi, per, x, mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    out = ctx.compute(1, 1)
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    st = ctx.stats()
+    ctx.close()
+    assert shape != "thin_slab" or st["nbins"][2] == 1
+    assert shape != "four_bins" or st["nbins"][0] == 4
+    assert len(want) > 0 and got == want
+    mx, _ = force_errors(out["f"], fref)
+    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
+
+
 def test_list_capacity_regrows_for_a_dense_cluster():
     """A compact fcc droplet in a large periodic box: the first-guess row
     capacity comes from the mean density (far below the droplet's ~78
