@@ -227,13 +227,17 @@ int mdx_autotune(mdx_ctx *ctx, int reps);
    buffers (velocities and positions are left as they were); *ms is the
    mean duration of one launch.  The list must be built (mdx_setup). */
 int mdx_time_step_kernel(mdx_ctx *ctx, int reps, double dt, double ftm2v, double *ms);
-/* Position format of the mixed-precision gathers.
-   MDX_POS_Q32: each neighbour is read as 32-bit fixed point per dimension
-   (resolution prd * 2^-32 periodic, prd * 2^-30 otherwise) and its
-   separation rebuilt in FP64; pairs within 2e-6 relative of the cutoff are
-   re-tested on the FP64 positions, so the cutoff decision is the FP64 one.
-   MDX_POS_FP64: neighbours are read as FP64 positions.  FP64 precision
-   always reads FP64 positions. */
+/* Position format of the mixed-precision gathers and of the list build's
+   candidate test.
+   MDX_POS_Q32 (the default): each neighbour is read as 32-bit fixed point
+   per dimension (resolution prd * 2^-32 periodic, prd * 2^-30 otherwise)
+   and its separation rebuilt in FP64; pairs within 2e-6 relative of the
+   cutoff are re-tested on the FP64 positions, so the cutoff decision is the
+   FP64 one.  The list build tests its candidates the same way in either
+   precision (same pairs as an FP64 test).  FP64 precision's pair kernels
+   always read FP64 positions.
+   MDX_POS_FP64: neighbours and build candidates are read as FP64
+   positions. */
 int mdx_set_position_format(mdx_ctx *ctx, int format);
 
 /* Ghost-shell exchange of a spatially decomposed run (one context per GPU

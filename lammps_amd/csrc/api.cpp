@@ -152,15 +152,21 @@ int mdx::prepare_ctx(mdx_ctx *c) {
       c->cutforce = std::max(c->cutforce, cutoff);
     }
   update_qpos(c);
-  if (c->qpos) {
+  {
     // the mirror's rounding (half a step per coordinate) must stay well
     // inside the cutoff band of the fixed-point tests: relative rsq error
-    // 2*sqrt(3)*step/r below 5e-7 at the smallest cutoff
+    // 2*sqrt(3)*step/r below 5e-7 at the smallest cutoff.  Where it does
+    // (and the position format is Q32), the list build tests its
+    // candidates on the mirror in either precision (the band's FP64
+    // re-test keeps every decision the FP64 one), and mixed precision
+    // gathers from it.
     double rmin = 0.0, smax = 0.0;
     for (int q = 0; q < nt1 * nt1; q++)
       if (c->cutsq[q] > 0.0) rmin = rmin > 0.0 ? std::min(rmin, std::sqrt(c->cutsq[q])) : std::sqrt(c->cutsq[q]);
     for (int d = 0; d < 3; d++) smax = std::max(smax, c->qs.step[d]);
-    if (2.0 * std::sqrt(3.0) * smax > 5e-7 * rmin) c->qpos = false;
+    const bool fine = rmin > 0.0 && 2.0 * std::sqrt(3.0) * smax <= 5e-7 * rmin;
+    if (!fine) c->qpos = false;
+    c->qbuild = fine && c->pos_format == MDX_POS_Q32;
   }
   int rc = neigh_setup_grid(c);
   if (rc) return rc;

@@ -337,6 +337,7 @@ struct mdx_ctx {
   mdx::QScale qs{};
   int pos_format = MDX_POS_Q32;  // mixed precision gathers: MDX_POS_Q32 or MDX_POS_FP64
   bool qpos = false;             // xq is maintained (mixed precision, Q32)
+  bool qbuild = false;           // xq is written at each build for the fixed-point candidate test
   bool xq_stale = true;          // xq must be rebuilt from x before the next force call
   int *tag = nullptr, *tagb = nullptr, *orig = nullptr, *origb = nullptr, *image = nullptr, *imageb = nullptr;
   int *inv = nullptr;  // caller index -> device index

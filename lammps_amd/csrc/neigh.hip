@@ -703,11 +703,12 @@ __global__ __launch_bounds__(kFlatBlock, MDX_FLAT_WAVES) void k_build_flat(Neigh
   atomicAdd(nsum, (unsigned long long)cnt);
 }
 
-// The fixed-point build needs the mirror (kept for mixed-precision gathers,
-// written by k_permute just before the build) and a stencil that never holds
-// one atom under two periodic images.
+// The fixed-point build needs the mirror (written by k_permute just before
+// the build, in either precision, where its resolution fits the band; mixed
+// precision also keeps it current between builds for its gathers) and a
+// stencil that never holds one atom under two periodic images.
 bool q_build_ok(const mdx_ctx *c) {
-  if (!c->qpos || !c->xq) return false;
+  if (!(c->qpos || c->qbuild) || !c->xq) return false;
   for (int d = 0; d < 3; d++)
     if (c->na.periodic[d] && 2 * c->na.S[d] + 1 > c->na.nbin[d]) return false;
   return true;
@@ -888,7 +889,7 @@ int neigh_build_launch(mdx_ctx *c) {
                      c->bkey_s, c->one + 1, c->dflags);
   hipLaunchKernelGGL(k_permute, dim3(G), dim3(B), 0, c->stream, c->n, c->bidx, c->x, c->v, c->tag, c->orig,
                      c->image, c->xb, c->vb, c->tagb, c->origb, c->imageb, c->inv, c->xhold,
-                     c->qpos ? c->xq : nullptr, c->qs);
+                     (c->qpos || c->qbuild) ? c->xq : nullptr, c->qs);
   if (c->qpos) c->xq_stale = false;
   std::swap(c->x, c->xb);
   std::swap(c->v, c->vb);

@@ -373,9 +373,11 @@ def test_build_with_walls_matches_the_oracle(lj_state, periodic):
 
 @pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 0, 0), (1, 0, 1)])
 def test_fixed_point_build_gives_the_fp64_pair_set(lj_state, periodic):
-    """Mixed precision builds the list from the 32-bit fixed-point position
+    """With the Q32 position format (the default, in either precision) the
+    list build tests its candidates on the 32-bit fixed-point position
     mirror (FP32 test, FP64 re-test within 2e-6 of the cutoff): pair set,
-    total and largest neighbour count equal those of the FP64-position build."""
+    total and largest neighbour count equal those of the build on FP64
+    positions (MDX_POS_FP64)."""
     from lammps_amd import mdx
     x, _, _, t, _ = lj_state.state()
     L = 20 * A_LJ
