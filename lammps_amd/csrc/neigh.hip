@@ -96,6 +96,12 @@ __device__ __forceinline__ bool keep_exact(const NeighArgs &a, const double4 &xi
 // appears in the stencil under two images (2S+1 <= nbin in every periodic
 // dimension, checked on the host).
 constexpr double kBuildBand = 2e-6;
+
+// bins of the build's grid per neighbour cutoff (the reference's grid has 2)
+#ifndef MDX_BIN_DIV
+#define MDX_BIN_DIV 2
+#endif
+constexpr int kBinDiv = MDX_BIN_DIV;
 struct QStep {
   float s[3];
 };
@@ -466,9 +472,13 @@ __device__ __forceinline__ void code_shift(const NeighArgs &a, int code, double 
 #ifndef MDX_FLAT_WAVES
 #define MDX_FLAT_WAVES 5
 #endif
+#ifndef MDX_FLAT_MAXS
+#define MDX_FLAT_MAXS MDX_BIN_DIV
+#endif
 constexpr int kFlatBlock = 256;
-constexpr int kFlatRows = 5;             // rows of a stencil plane (S <= 2)
-constexpr int kFlatSeg = 2 * kFlatRows;  // queued segments per lane: one plane
+constexpr int kFlatMaxS = MDX_FLAT_MAXS;    // largest stencil extent of the flat build
+constexpr int kFlatRows = 2 * kFlatMaxS + 1;  // rows of a stencil plane
+constexpr int kFlatSeg = 2 * kFlatRows;      // queued segments per lane: one plane
 
 // The list build as one candidate stream per lane and stencil plane.
 // k_build_lane runs each stencil row for as long as the lane with the most
@@ -729,7 +739,7 @@ void launch_build(mdx_ctx *c) {
   // 2S+1 bins).  Larger stencils come from bins thinner than half the
   // cutoff, in a box dimension below it (one bin): the per-row build.
   const NeighArgs &na = c->na;
-  const bool flat = MDX_BUILD_FLAT && na.S[0] <= 2 && na.S[1] <= 2 && na.S[2] <= 2 &&
+  const bool flat = MDX_BUILD_FLAT && na.S[0] <= kFlatMaxS && na.S[1] <= kFlatMaxS && na.S[2] <= kFlatMaxS &&
                     (!na.periodic[0] || 2 * na.S[0] <= na.nbin[0]);
   if (flat) {
     if (q && one) go(k_build_flat<true, true>, c->xq);
@@ -756,11 +766,14 @@ int neigh_setup_grid(mdx_ctx *c) {
     }
   }
   // Neighbor::setup_bins: binsize = 0.5*cutneighmax, nbin = floor(bbox/binsize)
-  double binsize_optimal = 0.5 * cut;
-  if (binsize_optimal == 0.0) binsize_optimal = c->prd[0];
-  const double binsizeinv = 1.0 / binsize_optimal;
+  // (the reference's grid, kept for the tie rule); the build's own grid is
+  // cutneighmax / kBinDiv
+  const double ref_bs = cut > 0.0 ? 0.5 * cut : c->prd[0];
+  const double own_bs = cut > 0.0 ? cut / kBinDiv : c->prd[0];
   for (int d = 0; d < 3; d++) {
-    int nb = static_cast<int>(c->prd[d] * binsizeinv);
+    int nr = static_cast<int>(c->prd[d] / ref_bs);
+    if (nr == 0) nr = 1;
+    int nb = static_cast<int>(c->prd[d] / own_bs);
     if (nb == 0) nb = 1;
     const double bs = c->prd[d] / nb;
     a.lo[d] = c->lo[d];
@@ -771,8 +784,8 @@ int neigh_setup_grid(mdx_ctx *c) {
     a.periodic[d] = c->periodic[d];
     a.ref.bboxlo[d] = c->lo[d];
     a.ref.bboxhi[d] = c->hi[d];
-    a.ref.bininv[d] = a.bininv[d];
-    a.ref.nbin[d] = nb;
+    a.ref.bininv[d] = 1.0 / (c->prd[d] / nr);  // bininv = 1/binsize, rounded as the reference forms it
+    a.ref.nbin[d] = nr;
   }
   long nb = (long)a.nbin[0] * a.nbin[1] * a.nbin[2];
   if (nb > (1L << 30)) {
