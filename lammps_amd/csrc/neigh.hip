@@ -492,14 +492,15 @@ constexpr int kFlatSeg = 2 * kFlatRows;      // queued segments per lane: one pl
 // (a contiguous atom range with its image code) are queued in LDS.  Then
 // the queue is drained as one stream, four candidates at a time, a lane
 // moving to its next segment as soon as one ends, so the wavefront runs as
-// long as its busiest lane's plane total (over the five planes, ~1.35x the
-// candidates of its mean lane).  Candidate order, keep test and tie rule
+// long as its busiest lane's plane total (over a 5-plane stencil, ~1.35x
+// the candidates of its mean lane).  Candidate order, keep test and tie rule
 // are k_build_lane's: the rows are entry for entry the same.  Queue slot k
 // of lane t is [k * 256 + t] (consecutive lanes, consecutive banks); a
 // segment is held as two list entries (first, end), j | code <<
 // MDX_IDX_BITS, so the candidate's entry is the running value itself and
 // the self pair is one compare.  Measured at 2.048 M lj/cut atoms: 1.15 ms
-// per build against 1.33 (mixed), 1.41 against 1.66 (FP64).
+// per build against 1.33 (fixed-point candidates), 1.41 against 1.66 (FP64
+// candidates).
 template <bool Q, bool ONE>
 __global__ __launch_bounds__(kFlatBlock, MDX_FLAT_WAVES) void k_build_flat(NeighArgs a, const double4 *__restrict__ x,
                                                    const int4 *__restrict__ xq, QStep qs,
