@@ -170,18 +170,20 @@ __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict
   int img = image[i];
   int im[3] = {img & MDX_IMGMASK, (img >> 10) & MDX_IMGMASK, img >> 20};
   int ib[3];
-  bool lost = false;
+  bool lost = false, moved = false;
   for (int d = 0; d < 3; d++) {
     double lo = a.lo[d], hi = a.lo[d] + a.prd[d];
     if (a.periodic[d]) {
       if (c[d] < lo) {
         c[d] += a.prd[d];
         im[d] = (im[d] - 1) & MDX_IMGMASK;
+        moved = true;
       }
       if (c[d] >= hi) {
         c[d] -= a.prd[d];
         c[d] = fmax(c[d], lo);
         im[d] = (im[d] + 1) & MDX_IMGMASK;
+        moved = true;
       }
     } else if (c[d] < lo || c[d] >= hi) {
       lost = true;
@@ -191,11 +193,14 @@ __global__ void k_wrap_bin(NeighArgs a, double4 *__restrict__ x, int *__restrict
     ib[d] = b;
   }
   if (lost) atomicOr(&dflags[4], 1);
-  p.x = c[0];
-  p.y = c[1];
-  p.z = c[2];
-  x[i] = p;
-  image[i] = (im[2] << 20) | (im[1] << 10) | im[0];
+  // most atoms stay inside the box between builds: store only the wrapped ones
+  if (moved) {
+    p.x = c[0];
+    p.y = c[1];
+    p.z = c[2];
+    x[i] = p;
+    image[i] = (im[2] << 20) | (im[1] << 10) | im[0];
+  }
   b = (unsigned)bin_index(a, ib[0], ib[1], ib[2]);
   key[i] = b;
   }
