@@ -175,6 +175,39 @@ def test_small_bin_grids_match_the_oracle(prec, shape):
     assert mx <= (1e-11 if prec == "fp64" else 1e-5)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_long_box_beyond_the_pruning_grid_matches_the_oracle(prec):
+    """A periodic box 12,000 sigma long in x: more than 8192 bins in x, where
+    the build skips its per-atom stencil pruning (its FP32 bin coordinates
+    would lose the margins), and a fixed-point mirror too coarse for the
+    cutoff band, so both precisions test FP64 candidates.  Clumps of atoms
+    along x, one across the periodic face."""
+    from lammps_amd import mdx
+    Lx, Ly = 12000.0, 6.0
+    rng = np.random.default_rng(21)
+    pts = np.zeros((0, 3))
+    for cx in (1.0, 3000.0, 7000.5, Lx - 1.5):
+        while len(pts) < 40 * (1 + int(np.sum([cx > c for c in (1.0, 3000.0, 7000.5)]))):
+            p = np.array([cx + rng.uniform(-2.5, 2.5), rng.uniform(0.0, Ly), rng.uniform(0.0, Ly)])
+            p[0] %= Lx
+            d = pts - p
+            d[:, 0] = (d[:, 0] + Lx / 2) % Lx - Lx / 2
+            d[:, 1:] = (d[:, 1:] + Ly / 2) % Ly - Ly / 2
+            if len(pts) == 0 or np.min(np.einsum("ij,ij->i", d, d)) >= 0.81:
+                pts = np.vstack([pts, p])
+    lo, hi, per = (0.0, 0.0, 0.0), (Lx, Ly, Ly), (1, 1, 1)
+    want, fref = oracle_lj_pairs_forces(lo, hi, per, pts)
+    ctx = lj_context(lo, hi, per, pts, mdx.FP64 if prec == "fp64" else mdx.MIXED)
+    out = ctx.compute(1, 1)
+    got = {(int(a), int(b)) for a, b in ctx.neigh_pairs()}
+    st = ctx.stats()
+    ctx.close()
+    assert st["nbins"][0] > 8192
+    assert len(want) > 0 and got == want
+    mx, _ = force_errors(out["f"], fref)
+    assert mx <= (1e-11 if prec == "fp64" else 1e-5)
+
+
 def test_list_capacity_regrows_for_a_dense_cluster():
     """A compact fcc droplet in a large periodic box: the first-guess row
     capacity comes from the mean density (far below the droplet's ~78
