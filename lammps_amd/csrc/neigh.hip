@@ -123,7 +123,6 @@ constexpr int kPrune = MDX_BUILD_PRUNE;
 #ifndef MDX_BUILD_MIN_WAVES
 #define MDX_BUILD_MIN_WAVES 1
 #endif
-#define MDX_BUILD_BOUNDS __launch_bounds__(256, MDX_BUILD_MIN_WAVES)
 
 // The (ntypes+1)^2 cutoff table in LDS: a kernel-argument array indexed by
 // per-lane types is re-read from memory for every candidate.  Called by every
