@@ -482,7 +482,7 @@ __device__ __forceinline__ void code_shift(const NeighArgs &a, int code, double 
 constexpr int kFlatBlock = 256;
 constexpr int kFlatMaxS = MDX_FLAT_MAXS;    // largest stencil extent of the flat build
 constexpr int kFlatRows = 2 * kFlatMaxS + 1;  // rows of a stencil plane
-constexpr int kFlatSeg = 2 * kFlatRows;      // queued segments per lane: one plane
+constexpr int kFlatSeg = 2 * kFlatRows + 2;  // queued segments per lane: one plane, an end mark, a spare slot
 
 // The list build as one candidate stream per lane and stencil plane.
 // k_build_lane runs each stencil row for as long as the lane with the most
@@ -633,33 +633,28 @@ __global__ __launch_bounds__(kFlatBlock, MDX_FLAT_WAVES) void k_build_flat(Neigh
         }
     {
       // the plane's candidates as one stream per lane
-      unsigned e = 0, e1 = 0, ne = 0, ne1 = 0;  // current and next segment; (0, 0): none
-      if (nq > 0) {
-        e = lq0[0];
-        e1 = lq1[0];
-      }
-      if (nq > 1) {
-        ne = lq0[kFlatBlock];
-        ne1 = lq1[kFlatBlock];
-      }
-      int k = 2;
+      // end mark (0, 0) after the queued segments (a real segment ends at an
+      // entry > 0): a lane that reaches it stops advancing, so it reads at
+      // most one slot past it
+      lq0[nq * kFlatBlock] = 0u;
+      lq1[nq * kFlatBlock] = 0u;
+      unsigned e = lq0[0], e1 = lq1[0];                              // current segment
+      unsigned ne = lq0[kFlatBlock], ne1 = lq1[kFlatBlock];          // next (unused past the mark)
+      const unsigned *n0 = lq0 + 2 * kFlatBlock, *n1 = lq1 + 2 * kFlatBlock;
       for (;;) {
-        if (e == e1) break;  // queue drained (segments are never empty)
+        if (e == e1) break;  // the end mark (segments are never empty)
         unsigned eu[kLaneU];
 #pragma unroll
         for (int u = 0; u < kLaneU; u++) {
           eu[u] = e != e1 ? e : self;  // past the end: the lane's own atom, rejected below
           if (e != e1) e++;
-          if (e == e1) {  // segment done: take the prefetched one, prefetch the next
+          if (e == e1 && e1 != 0u) {  // segment done: take the prefetched one, prefetch the next
             e = ne;
             e1 = ne1;
-            ne = 0;
-            ne1 = 0;
-            if (k < nq) {
-              ne = lq0[k * kFlatBlock];
-              ne1 = lq1[k * kFlatBlock];
-            }
-            k++;
+            ne = *n0;
+            ne1 = *n1;
+            n0 += kFlatBlock;
+            n1 += kFlatBlock;
           }
         }
         double4 xj[Q ? 1 : kLaneU];
