@@ -265,12 +265,17 @@ constexpr bool kLateV = MDX_LATE_V;
 
 // ONE: a single atom type, so every pair uses the coefficients of type pair
 // (1,1): the table reads are loop-invariant and leave the neighbour loop.
+struct LJOne {  // the (1,1) coefficients of a single-type system, as kernel arguments (scalar registers)
+  double clo, chi, cut, lj1, lj2, lj3, lj4, off;
+  float f1, f2, f3, f4, foff;
+};
 template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double px, double py, double pz,
                                            const double4 *__restrict__ x, const int4 *__restrict__ xq, QScale qs,
                                            const int *__restrict__ nbr, const int *__restrict__ numneigh,
                                            const LJTab *__restrict__ tab, double4 *__restrict__ f,
-                                           double *__restrict__ vatom, double *__restrict__ partials, IntegArgs ig) {
+                                           double *__restrict__ vatom, double *__restrict__ partials, IntegArgs ig,
+                                           LJOne one) {
   if constexpr (INTEG) {
     if (gate_closed(ig.gate)) return;
   }
@@ -298,14 +303,14 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
   __syncthreads();
   const int i = (xcd_block() * blockDim.x + threadIdx.x) / TPA;
   const int lane = threadIdx.x % TPA;
-  // single type, force-only kernel: the (1,1) coefficients in registers for
-  // the whole loop (2% faster); the fused kernel keeps them in LDS, where the
-  // registers would cost it a wavefront per SIMD
-  constexpr bool HOIST = ONE && !INTEG;
+  // single type: the (1,1) coefficients come as kernel arguments, so they
+  // stay in scalar registers for the whole loop (no LDS reads, no vector
+  // registers)
+  constexpr bool HOIST = ONE;
   const int t11 = nt1 + 1;
-  const double o_clo = (HOIST && Q) ? s_clo[Q ? t11 : 0] : 0.0, o_chi = (HOIST && Q) ? s_chi[Q ? t11 : 0] : 0.0;
-  const double o_cut = HOIST ? s_cutsq[t11] : 0.0, o_lj1 = HOIST ? s_lj1[t11] : 0.0, o_lj2 = HOIST ? s_lj2[t11] : 0.0;
-  const float o_f1 = HOIST ? f_lj1[t11] : 0.0f, o_f2 = HOIST ? f_lj2[t11] : 0.0f;
+  (void)t11;
+  const double o_clo = one.clo, o_chi = one.chi, o_cut = one.cut, o_lj1 = one.lj1, o_lj2 = one.lj2;
+  const float o_f1 = one.f1, o_f2 = one.f2;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -399,13 +404,13 @@ __global__ MDX_PAIR_BOUNDS void k_lj(int n, int npad, int cap, int nt1, double p
             const float r6inv = r2inv * r2inv * r2inv;
             const float forcelj = r6inv * ((HOIST ? o_f1 : f_lj1[t]) * r6inv - (HOIST ? o_f2 : f_lj2[t]));
             fpair = (double)(forcelj * r2inv);
-            if (EV) evdwl = (double)(r6inv * (f_lj3[t] * r6inv - f_lj4[t]) - f_off[t]);
+            if (EV) evdwl = HOIST ? (double)(r6inv * (one.f3 * r6inv - one.f4) - one.foff) : (double)(r6inv * (f_lj3[t] * r6inv - f_lj4[t]) - f_off[t]);
           } else {
             const double r2inv = 1.0 / rsq;
             const double r6inv = r2inv * r2inv * r2inv;
             const double forcelj = r6inv * ((HOIST ? o_lj1 : s_lj1[t]) * r6inv - (HOIST ? o_lj2 : s_lj2[t]));
             fpair = forcelj * r2inv;
-            if (EV) evdwl = r6inv * (s_lj3[t] * r6inv - s_lj4[t]) - s_off[t];
+            if (EV) evdwl = HOIST ? r6inv * (one.lj3 * r6inv - one.lj4) - one.off : r6inv * (s_lj3[t] * r6inv - s_lj4[t]) - s_off[t];
           }
           fx += delx * fpair;
           fy += dely * fpair;
@@ -649,11 +654,11 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
 
 // Resident wavefronts per SIMD of the fused force kernels, held by giving
 // each workgroup enough LDS that no more workgroups fit on a CU (160 KiB of
-// LDS, four SIMDs).  The mixed fused lj/cut kernel gets no cap: its 66
-// registers already hold it to 7 wavefronts, where it runs fastest (2.048 M
-// atoms: 0.334 ms per launch; 0.347 ms held to 7 by LDS, 0.39 ms to 6 or 5,
-// 0.46 ms to 4; 0.40 ms at 8 with the 64-register build).  The FP64 one runs
-// 2-3% faster held to 6, its default.  MDX_PAIR_WAVES sets a cap (0: none)
+// LDS, four SIMDs).  The mixed fused lj/cut kernel runs fastest at 7: with
+// its single-type coefficients in scalar registers it needs 64 vector
+// registers, which would admit 8 (2.048 M atoms: 0.316 ms per launch at 8,
+// 0.306 held to 7; with 66 registers it was 0.303-0.306 at its own 7).  The
+// FP64 one runs 2-3% faster held to 6.  MDX_PAIR_WAVES sets a cap (0: none)
 // on both and on the EAM passes.
 int pair_wave_cap(int dflt) {
   static const int w = [] {
@@ -689,10 +694,28 @@ template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  const size_t lds = INTEG ? occupancy_lds(static_lds<k_lj<MIXED, EV, TPA, INTEG, Q, ONE>>(), B, 0, MIXED ? 0 : 6) : 0;
+  const size_t lds = INTEG ? occupancy_lds(static_lds<k_lj<MIXED, EV, TPA, INTEG, Q, ONE>>(), B, 0, MIXED ? 7 : 6) : 0;
+  LJOne one{};
+  if (ONE) {
+    const LJTab &t = c->lj_host;
+    const int q = (c->ntypes + 1) + 1;
+    one.clo = t.cutsq[q] * (1.0 - kBand);
+    one.chi = t.cutsq[q] * (1.0 + kBand);
+    one.cut = t.cutsq[q];
+    one.lj1 = t.lj1[q];
+    one.lj2 = t.lj2[q];
+    one.lj3 = t.lj3[q];
+    one.lj4 = t.lj4[q];
+    one.off = t.offset[q];
+    one.f1 = (float)t.lj1[q];
+    one.f2 = (float)t.lj2[q];
+    one.f3 = (float)t.lj3[q];
+    one.f4 = (float)t.lj4[q];
+    one.foff = (float)t.offset[q];
+  }
   hipLaunchKernelGGL((k_lj<MIXED, EV, TPA, INTEG, Q, ONE>), dim3(G), dim3(B), lds, c->stream, c->n, c->npad, c->maxneigh,
                      c->ntypes + 1, c->prd[0], c->prd[1], c->prd[2], c->x, c->xq, c->qs, c->nbr, c->numneigh,
-                     c->lj_dev, c->f, c->vatom, c->partials, ig);
+                     c->lj_dev, c->f, c->vatom, c->partials, ig, one);
   prof_end(c);
   c->last_grid = G;
 }
