@@ -658,8 +658,8 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
 // its single-type coefficients in scalar registers it needs 64 vector
 // registers, which would admit 8 (2.048 M atoms: 0.316 ms per launch at 8,
 // 0.306 held to 7; with 66 registers it was 0.303-0.306 at its own 7).  The
-// FP64 one runs 2-3% faster held to 6.  MDX_PAIR_WAVES sets a cap (0: none)
-// on both and on the EAM passes.
+// FP64 one runs fastest held to 5 (0.394 ms; 0.399 at 6, 0.410 at 7, 0.415
+// at 8).  MDX_PAIR_WAVES sets a cap (0: none) on both and on the EAM passes.
 int pair_wave_cap(int dflt) {
   static const int w = [] {
     const char *s = std::getenv("MDX_PAIR_WAVES");
@@ -694,7 +694,7 @@ template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE>
 void launch_lj_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_LJ);
-  const size_t lds = INTEG ? occupancy_lds(static_lds<k_lj<MIXED, EV, TPA, INTEG, Q, ONE>>(), B, 0, MIXED ? 7 : 6) : 0;
+  const size_t lds = INTEG ? occupancy_lds(static_lds<k_lj<MIXED, EV, TPA, INTEG, Q, ONE>>(), B, 0, MIXED ? 7 : 5) : 0;
   LJOne one{};
   if (ONE) {
     const LJTab &t = c->lj_host;
