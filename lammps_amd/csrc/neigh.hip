@@ -505,13 +505,22 @@ constexpr int kFlatSeg = 2 * kFlatRows + 2;  // queued segments per lane: one pl
 // the self pair is one compare.  Measured at 2.048 M lj/cut atoms: 1.15 ms
 // per build against 1.33 (fixed-point candidates), 1.41 against 1.66 (FP64
 // candidates).
+// launch-uniform values of the flat build, formed on the host so that they
+// stay in scalar registers: FP32 bin sizes and inverse, the pruning switch,
+// and for a single type the cutoff, its FP32 band and the pruning reach
+struct FlatConst {
+  float bsy, bsz, binx;
+  float c_lo, c_hi, reach;
+  double c_cut;
+  int prune;
+};
 template <bool Q, bool ONE>
 __global__ __launch_bounds__(kFlatBlock, MDX_FLAT_WAVES) void k_build_flat(NeighArgs a, const double4 *__restrict__ x,
                                                    const int4 *__restrict__ xq, QStep qs,
                                                    const unsigned *__restrict__ bkey,
                                                    const int *__restrict__ bin_start, int *__restrict__ nbr,
                                                    int *__restrict__ numneigh, int *__restrict__ dflags,
-                                                   unsigned long long *__restrict__ nsum) {
+                                                   unsigned long long *__restrict__ nsum, FlatConst fc) {
   __shared__ double s_cut[MDX_NT1 * MDX_NT1];
   __shared__ float s_lo[Q ? MDX_NT1 * MDX_NT1 : 1], s_hi[Q ? MDX_NT1 * MDX_NT1 : 1];
   __shared__ double s_cmax[MDX_NT1];
@@ -534,14 +543,14 @@ __global__ __launch_bounds__(kFlatBlock, MDX_FLAT_WAVES) void k_build_flat(Neigh
   int cnt = 0;
   int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
   const int t11 = a.ntypes1 + 1;
-  const float c_lo = (ONE && Q) ? s_lo[Q ? t11 : 0] : 0.0f, c_hi = (ONE && Q) ? s_hi[Q ? t11 : 0] : 0.0f;
-  const double c_cut = ONE ? s_cut[t11] : 0.0;
+  const float c_lo = fc.c_lo, c_hi = fc.c_hi;
+  const double c_cut = fc.c_cut;
   const float tx = static_cast<float>((xi.x - a.lo[0]) * a.bininv[0]),
               ty = static_cast<float>((xi.y - a.lo[1]) * a.bininv[1]),
               tz = static_cast<float>((xi.z - a.lo[2]) * a.bininv[2]);
-  const float bsy = static_cast<float>(1.0 / a.bininv[1]), bsz = static_cast<float>(1.0 / a.bininv[2]);
-  const float reach = static_cast<float>(s_cmax[ti] * (1.0 + 1e-4));
-  const int prune = (a.nbin[0] <= 8192 && a.nbin[1] <= 8192 && a.nbin[2] <= 8192) ? kPrune : 0;
+  const float bsy = fc.bsy, bsz = fc.bsz;
+  const float reach = ONE ? fc.reach : static_cast<float>(s_cmax[ti] * (1.0 + 1e-4));
+  const int prune = fc.prune;
   const unsigned self = static_cast<unsigned>(i) | (MDX_NOSHIFT << MDX_IDX_BITS);
   int nq = 0;
   for (int dz = -a.S[2]; dz <= a.S[2]; dz++) {
@@ -583,7 +592,7 @@ __global__ __launch_bounds__(kFlatBlock, MDX_FLAT_WAVES) void k_build_flat(Neigh
       const unsigned rowbase = static_cast<unsigned>(bin_row(a, jy, jz) * nbx);
       int x0 = ix - a.S[0], x1 = ix + a.S[0];
       if (prune == 1) {
-        const float wb = sqrtf(reach - dyz2) * static_cast<float>(a.bininv[0]) + 1e-3f;
+        const float wb = sqrtf(reach - dyz2) * fc.binx + 1e-3f;
         x0 = max(x0, static_cast<int>(ceilf(tx - 1.0f - wb)));
         x1 = min(x1, static_cast<int>(floorf(tx + wb)));
       }
@@ -742,10 +751,28 @@ void launch_build(mdx_ctx *c) {
   const bool flat = MDX_BUILD_FLAT && na.S[0] <= kFlatMaxS && na.S[1] <= kFlatMaxS && na.S[2] <= kFlatMaxS &&
                     (!na.periodic[0] || 2 * na.S[0] <= na.nbin[0]);
   if (flat) {
-    if (q && one) go(k_build_flat<true, true>, c->xq);
-    else if (q) go(k_build_flat<true, false>, c->xq);
-    else if (one) go(k_build_flat<false, true>, nullptr);
-    else go(k_build_flat<false, false>, nullptr);
+    FlatConst fc{};
+    fc.bsy = static_cast<float>(1.0 / na.bininv[1]);
+    fc.bsz = static_cast<float>(1.0 / na.bininv[2]);
+    fc.binx = static_cast<float>(na.bininv[0]);
+    fc.prune = (na.nbin[0] <= 8192 && na.nbin[1] <= 8192 && na.nbin[2] <= 8192) ? kPrune : 0;
+    if (one) {  // type 1: the (1,1) cutoff, its band, and the row maximum of the table (stage_cmax)
+      const int t11 = na.ntypes1 + 1;
+      double m = 0.0;
+      for (int u = 0; u < na.ntypes1; u++) m = std::fmax(m, na.cutneighsq[1 * na.ntypes1 + u]);
+      fc.c_cut = na.cutneighsq[t11];
+      fc.c_lo = static_cast<float>(na.cutneighsq[t11] * (1.0 - kBuildBand));
+      fc.c_hi = static_cast<float>(na.cutneighsq[t11] * (1.0 + kBuildBand));
+      fc.reach = static_cast<float>(m * (1.0 + 1e-4));
+    }
+    auto goflat = [&](auto kernel, const int4 *xq) {
+      hipLaunchKernelGGL(kernel, dim3(G), dim3(B), 0, c->stream, c->na, c->x, xq, qs, c->bkey_s, c->bin_start,
+                         c->nbr, c->numneigh, c->dflags, nsum, fc);
+    };
+    if (q && one) goflat(k_build_flat<true, true>, c->xq);
+    else if (q) goflat(k_build_flat<true, false>, c->xq);
+    else if (one) goflat(k_build_flat<false, true>, nullptr);
+    else goflat(k_build_flat<false, false>, nullptr);
     return;
   }
   if (q && one) go(k_build_lane<true, true>, c->xq);
