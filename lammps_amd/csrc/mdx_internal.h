@@ -67,6 +67,7 @@ struct LJTab {  // (ntypes+1)^2 row-major, as Pair::init_one fills them
 struct EAMArgs {
   int nrho, nr, nfrho, nrhor, nz2r, ntypes1;
   double rdr, rdrho, rhomax, cutforcesq;
+  double clo, chi;  // cutforcesq (1 -/+ the fixed-point re-test band), set at each launch
   int type2frho[MDX_NT1];
   int type2rhor[MDX_NT1 * MDX_NT1];
   int type2z2r[MDX_NT1 * MDX_NT1];

@@ -475,7 +475,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_rho(int n, int npad, int cap, double px, d
   const int lane = threadIdx.x % TPA;
   double rho = 0.0;
   int ti = 0;
-  const double clo = a.cutforcesq * (1.0 - kBand), chi = a.cutforcesq * (1.0 + kBand);
+  const double clo = a.clo, chi = a.chi;  // kernel arguments: scalar registers
   if (i < n) {
     const double4 xi = x[i];
     const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
@@ -584,7 +584,7 @@ __global__ MDX_PAIR_BOUNDS void k_eam_force(int n, int npad, int cap, double px,
     vpre = ig.v[i];
     if (ig.check) hpre = ig.xhold[i];
   }
-  const double clo = a.cutforcesq * (1.0 - kBand), chi = a.cutforcesq * (1.0 + kBand);
+  const double clo = a.clo, chi = a.chi;  // kernel arguments: scalar registers
   if (i < n) {
     xi = x[i];
     const int4 qi = Q ? xq[i] : make_int4(0, 0, 0, 0);
@@ -728,16 +728,19 @@ template <bool MIXED, bool EV, int TPA, bool INTEG, bool Q, bool ONE, bool TL>
 void launch_eam_t(mdx_ctx *c, int B, const IntegArgs &ig) {
   const int G = xcd_grid(((long)c->n * TPA + B - 1) / B);
   const size_t kn = (size_t)c->eam.nr + 1;
+  EAMArgs ea = c->eam;
+  ea.clo = ea.cutforcesq * (1.0 - kBand);
+  ea.chi = ea.cutforcesq * (1.0 + kBand);
   prof_begin(c, MDX_K_EAM_RHO);
   hipLaunchKernelGGL((k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>), dim3(G), dim3(B),
-                     occupancy_lds(static_lds<k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>>(), B, TL ? kn * sizeof(float4) : 0, 0), c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], c->eam, c->x, c->xq, c->qs,
+                     occupancy_lds(static_lds<k_eam_rho<MIXED, EV, TPA, Q, ONE, TL>>(), B, TL ? kn * sizeof(float4) : 0, 0), c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1], c->prd[2], ea, c->x, c->xq, c->qs,
                      c->nbr, c->numneigh, c->rho, c->fp, c->eemb, INTEG ? ig.gate : StepGate{});
   prof_end(c);
   prof_begin(c, INTEG ? MDX_K_FUSED : MDX_K_EAM_FORCE);
   const size_t tl_bytes = TL ? 3 * kn * sizeof(float4) : 0;
   hipLaunchKernelGGL((k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE, TL>), dim3(G), dim3(B),
                      INTEG ? occupancy_lds(static_lds<k_eam_force<MIXED, EV, TPA, INTEG, Q, ONE, TL>>(), B, tl_bytes, 0) : tl_bytes, c->stream, c->n, c->npad, c->maxneigh, c->prd[0], c->prd[1],
-                     c->prd[2], c->eam, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
+                     c->prd[2], ea, c->x, c->xq, c->qs, c->nbr, c->numneigh, c->fp, c->eemb, c->f, c->vatom,
                      c->partials, ig);
   prof_end(c);
   c->last_grid = G;
