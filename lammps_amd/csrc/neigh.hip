@@ -10,13 +10,16 @@
 // sorted by bin (counting sort: arrival slots, then ranks by atom index)
 // so that every x-row of bins is one contiguous atom range.  One lane per
 // atom then scans the 5x5 rows of its stencil (each row at most 3
-// contiguous segments: wrapped-low, main, wrapped-high) and keeps j iff
-// rsq <= cutneighsq, with rsq evaluated in FP64 without contraction on the
-// periodic image x_j + s*L, the value the reference computes for a ghost
-// atom (atom_vec_atomic.cpp:290-297).  Rows list candidates in scan order.
-// The result is a FULL list, one row of `cap` entries per atom (row major);
-// each entry holds the neighbour index (bits 0-26) and its periodic image
-// code (bits 27-31) so pair kernels rebuild the same image between builds.
+// contiguous segments: wrapped-low, main, wrapped-high; k_build_flat takes
+// them a plane at a time as one candidate stream) and keeps j iff
+// rsq <= cutneighsq, with the rsq the reference computes for a ghost atom:
+// FP64 without contraction on the periodic image x_j + s*L
+// (atom_vec_atomic.cpp:290-297), decided first on the fixed-point mirror in
+// FP32 and re-tested that way within a narrow band of the cutoff.
+// Candidates are listed in scan order.  The result is a FULL list in chunks
+// of four entries, chunk-major (chunk c of atom i at c * npad + i); each
+// entry holds the neighbour index (bits 0-26) and its periodic image code
+// (bits 27-31) so pair kernels rebuild the same image between builds.
 #include <hipcub/hipcub.hpp>
 
 #include "mdx_internal.h"
