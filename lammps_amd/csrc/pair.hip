@@ -9,19 +9,23 @@
 // ev_tally_full), and the global sums come from a fixed-order block tree.
 //
 // Threads per atom (TPA, a template parameter dividing 64): the TPA lanes of
-// an atom walk its row k = lane, lane+TPA, ...  Rows list neighbours in
-// bin-sorted order, so consecutive lanes gather consecutive atoms and one
-// 64-lane load touches a few cache lines instead of up to 64 (the vector
-// memory unit resolves about one line per clock, so this is what bounds a
-// thread-per-atom gather).  The TPA partial sums are combined with
-// cross-lane shuffles; lane 0 of the group writes the atom.
+// an atom share its list.  The lj/cut kernels give each lane chunks of four
+// consecutive entries (one 16-B index load per chunk; the FP64 kernel with
+// four lanes per atom transposes the group's chunks so that one gather
+// covers four consecutive entries); the eam kernels take entries lane,
+// lane+TPA, ...  Lists hold neighbours in bin-sorted order, so the lanes of
+// a group gather nearby atoms and a 64-lane load touches fewer cache lines.
+// The TPA partial sums are combined with cross-lane shuffles; lane 0 of the
+// group writes the atom.
 //
 // Precision modes (template MIXED):
 //   FP64  : everything in double, as the reference.
-//   MIXED : positions, the image shift, del = x_i - x_j and rsq stay FP64
-//           (so the cutoff test and the geometry match the reference); the
-//           potential arithmetic (1/r^2, r^6, splines) is FP32; forces,
-//           energies and virials accumulate in FP64.
+//   MIXED : neighbours are read from the 32-bit fixed-point mirror and
+//           del = x_i - x_j and rsq are rebuilt in FP64 (a pair within a
+//           narrow band of the cutoff is re-tested on the FP64 positions, so
+//           the cutoff test is the reference's); the potential arithmetic
+//           (1/r^2, r^6, splines) is FP32; forces, energies and virials
+//           accumulate in FP64.
 //
 // EAM: the reference needs rho -> reverse_comm -> fp -> forward_comm ->
 // force (pair_eam.cpp:206,232).  With a full list rho_i is complete inside
